@@ -1,0 +1,131 @@
+/*
+ * zkhip — C ABI of the MI355X (gfx950) prover hot path: BN254 G1 MSM and
+ * Fr NTT / iNTT / coset NTT, plus the device-buffer and conversion calls the
+ * operator layer needs.
+ *
+ * Conventions
+ *   - Every call returns ZK_OK (0) or an error code; zk_last_error(ctx)
+ *     gives the message.  ZK_ERR_INVALID maps to ValueError and the other
+ *     codes to RuntimeError in the Python operator layer.
+ *   - Field elements are 32 bytes: 4 x u64 (equivalently 8 x u32) little-
+ *     endian words.  Device buffers hold Montgomery form (x * 2^256 mod p),
+ *     the same layout as the reference's device buffers
+ *     ("fr_mont_u64x4", runtime/metal/buffer.py:8-15,
+ *     runtime/kernels/cpu/kernels.py:98-144).
+ *   - G1 bases are affine Montgomery (x, y), 64 bytes each; x = y = 0 marks
+ *     the point at infinity.  MSM results are Jacobian Montgomery (X, Y, Z),
+ *     12 x u64, with Z = 0 for infinity — the layout the reference unpacks
+ *     in runtime/metal/msm.py:36-63.
+ *   - `stream` is a hipStream_t (NULL = the context's own stream).  Calls are
+ *     stream-ordered; the caller owns every pointer it passes in.  A context
+ *     is used from one host thread at a time.
+ */
+#ifndef ZKHIP_H
+#define ZKHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZK_OK 0
+#define ZK_ERR_INVALID 1
+#define ZK_ERR_RUNTIME 2
+#define ZK_ERR_NO_DEVICE 3
+
+typedef struct zk_ctx zk_ctx;
+
+/* Library version (major * 10000 + minor * 100 + patch). */
+int zk_version(void);
+
+/* Number of visible HIP devices (0 on a machine without a GPU). */
+int zk_device_count(int* count);
+
+/* Create / destroy a context on device `ordinal`.  Replaces the reference's
+ * MetalContext.create_default (runtime/context.py:31-36) and
+ * MetalRuntime.create_default (runtime/metal/runtime.py:494-629). */
+int zk_init(int ordinal, zk_ctx** out);
+int zk_destroy(zk_ctx* ctx);
+/* Message of the last failing call on ctx (or of the last failing zk_init
+ * when ctx is NULL). */
+const char* zk_last_error(const zk_ctx* ctx);
+
+/* Device and pinned-host memory; replace MemoryPool.alloc_metal /
+ * MTLBuffer storage (runtime/memory.py:75-110). */
+int zk_alloc(zk_ctx* ctx, size_t bytes, void** out);
+int zk_free(zk_ctx* ctx, void* ptr);
+int zk_pinned_alloc(zk_ctx* ctx, size_t bytes, void** out);
+int zk_pinned_free(zk_ctx* ctx, void* ptr);
+int zk_h2d(zk_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
+int zk_d2h(zk_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
+int zk_d2d(zk_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
+int zk_memset_zero(zk_ctx* ctx, void* dst, size_t bytes, void* stream);
+int zk_sync(zk_ctx* ctx, void* stream);
+
+/* Canonical <-> Montgomery conversion of n elements on the device; replace
+ * the host loops of _to_device / _from_device
+ * (runtime/kernels/cpu/kernels.py:135-140, 168-170) and of
+ * _pack_g1_point (runtime/metal/msm.py:13-34). Inputs >= p are reduced. */
+int zk_fr_to_mont(zk_ctx* ctx, const void* in, void* out, uint64_t n, void* stream);
+int zk_fr_from_mont(zk_ctx* ctx, const void* in, void* out, uint64_t n, void* stream);
+int zk_fq_to_mont(zk_ctx* ctx, const void* in, void* out, uint64_t n, void* stream);
+int zk_fq_from_mont(zk_ctx* ctx, const void* in, void* out, uint64_t n, void* stream);
+
+/* Element-wise Fr (Montgomery) ops over n elements.  zk_fr_mul replaces
+ * pointwise_mul_fr_mont (runtime/metal/runtime.py:204-213, dispatched at
+ * runtime/kernels/metal/kernels.py:175-222); zk_fr_sub replaces
+ * poly_sub_fr_mont (runtime/metal/runtime.py:215-224,
+ * runtime/kernels/metal/kernels.py:122-167). */
+int zk_fr_mul(zk_ctx* ctx, const void* a, const void* b, void* out, uint64_t n, void* stream);
+int zk_fr_add(zk_ctx* ctx, const void* a, const void* b, void* out, uint64_t n, void* stream);
+int zk_fr_sub(zk_ctx* ctx, const void* a, const void* b, void* out, uint64_t n, void* stream);
+/* out[i] = a[i] * c for one Montgomery constant c (host words). */
+int zk_fr_scale(zk_ctx* ctx, const void* a, const uint64_t c_mont[4], void* out, uint64_t n,
+                void* stream);
+
+/* NTT of size n = 2^log_n (log_n <= 28) over Montgomery Fr.
+ *   inverse = 0: out[i] = sum_{j < min(n, in_len)} in[j] * s^j * w^(ij)
+ *   inverse = 1: out[j] = n^-1 * s^-j * sum_i in[i] * w^(-ij)   (in_len must be n)
+ * w = omega_mont (NULL: the standard root omega_for_size(n) =
+ * ROOT_2_28^(2^(28-log_n))), s = shift_mont (NULL: no coset, s = 1).
+ * Replaces ROOTS_EVALS_FROM_COEFFS / ROOTS_COEFFS_FROM_EVALS /
+ * COSET_EVALS_FROM_COEFFS / COSET_COEFFS_FROM_EVALS on the Metal backend
+ * (runtime/kernels/metal/kernels.py:225-381, 418-827) and the CPU ones
+ * (runtime/kernels/cpu/kernels.py:175-250, crypto/poly/ntt.py:42-146).
+ * in == out is allowed. */
+int zk_ntt(zk_ctx* ctx, const void* in, uint64_t in_len, void* out, uint32_t log_n, int inverse,
+           const uint64_t omega_mont[4], const uint64_t shift_mont[4], void* stream);
+/* `batch` independent transforms of size 2^log_n laid out back to back. */
+int zk_ntt_batch(zk_ctx* ctx, const void* in, void* out, uint32_t log_n, uint32_t batch,
+                 int inverse, const uint64_t omega_mont[4], const uint64_t shift_mont[4],
+                 void* stream);
+
+/* G1 MSM: sum_i scalars[i] * bases[i] over n points, signed-digit
+ * Pippenger with window_bits c (0: c = max(4, min(16, bitlen(n) - 2))).
+ * Replaces metal_msm_g1 / metal_msm_g1_v2 (runtime/metal/msm.py:65-143,
+ * 179-332) behind the MSM_G1 op (runtime/kernels/metal/kernels.py:84-120)
+ * and the CPU msm_pippenger (crypto/msm/pippenger.py:10-64).  Synchronises
+ * `stream` before returning the result in out_jac_mont. */
+int zk_msm_g1(zk_ctx* ctx, const void* bases, const void* scalars, uint64_t n, int window_bits,
+              uint64_t out_jac_mont[12], void* stream);
+
+/* Tuning knobs: "ntt_max_pass_bits" (stages per NTT pass, 1..11; 0 = auto),
+ * "msm_segment" (pairs per accumulation thread), "msm_reduce_chunk"
+ * (buckets per reduction thread).  No counterpart in the reference. */
+int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value);
+
+/* Per-phase device time from HIP events recorded on the launch stream
+ * ("ntt", "msm_digits", "msm_sort", "msm_accumulate", "msm_reduce").
+ * The counterpart of the reference's per-op trace
+ * (runtime/executor.py:196-218), at kernel-phase granularity. */
+int zk_profile_enable(zk_ctx* ctx, int on);
+int zk_profile_get(zk_ctx* ctx, const char* phase, double* total_ms, uint64_t* launches);
+int zk_profile_reset(zk_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZKHIP_H */

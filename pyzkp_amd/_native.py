@@ -1,0 +1,263 @@
+"""ctypes binding of libzkhip.so (the C-ABI in include/zkhip.h).
+
+The library is loaded from this package directory only (it is built in-tree
+by ``pyzkp_amd.build``).  There is no fallback: if the library is missing or
+no device is present, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+import threading
+from typing import Optional, Sequence
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libzkhip.so")
+
+ZK_OK = 0
+ZK_ERR_INVALID = 1
+ZK_ERR_RUNTIME = 2
+ZK_ERR_NO_DEVICE = 3
+
+_u64p = ct.POINTER(ct.c_uint64)
+_vp = ct.c_void_p
+
+# (name, argtypes) for every symbol declared in include/zkhip.h
+SIGNATURES = {
+    "zk_version": [],
+    "zk_device_count": [ct.POINTER(ct.c_int)],
+    "zk_init": [ct.c_int, ct.POINTER(_vp)],
+    "zk_destroy": [_vp],
+    "zk_last_error": [_vp],
+    "zk_alloc": [_vp, ct.c_size_t, ct.POINTER(_vp)],
+    "zk_free": [_vp, _vp],
+    "zk_pinned_alloc": [_vp, ct.c_size_t, ct.POINTER(_vp)],
+    "zk_pinned_free": [_vp, _vp],
+    "zk_h2d": [_vp, _vp, _vp, ct.c_size_t, _vp],
+    "zk_d2h": [_vp, _vp, _vp, ct.c_size_t, _vp],
+    "zk_d2d": [_vp, _vp, _vp, ct.c_size_t, _vp],
+    "zk_memset_zero": [_vp, _vp, ct.c_size_t, _vp],
+    "zk_sync": [_vp, _vp],
+    "zk_fr_to_mont": [_vp, _vp, _vp, ct.c_uint64, _vp],
+    "zk_fr_from_mont": [_vp, _vp, _vp, ct.c_uint64, _vp],
+    "zk_fq_to_mont": [_vp, _vp, _vp, ct.c_uint64, _vp],
+    "zk_fq_from_mont": [_vp, _vp, _vp, ct.c_uint64, _vp],
+    "zk_fr_mul": [_vp, _vp, _vp, _vp, ct.c_uint64, _vp],
+    "zk_fr_add": [_vp, _vp, _vp, _vp, ct.c_uint64, _vp],
+    "zk_fr_sub": [_vp, _vp, _vp, _vp, ct.c_uint64, _vp],
+    "zk_fr_scale": [_vp, _vp, _u64p, _vp, ct.c_uint64, _vp],
+    "zk_ntt": [_vp, _vp, ct.c_uint64, _vp, ct.c_uint32, ct.c_int, _u64p, _u64p, _vp],
+    "zk_ntt_batch": [_vp, _vp, _vp, ct.c_uint32, ct.c_uint32, ct.c_int, _u64p, _u64p, _vp],
+    "zk_msm_g1": [_vp, _vp, _vp, ct.c_uint64, ct.c_int, _u64p, _vp],
+    "zk_set_tuning": [_vp, ct.c_char_p, ct.c_int64],
+    "zk_profile_enable": [_vp, ct.c_int],
+    "zk_profile_get": [_vp, ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_uint64)],
+    "zk_profile_reset": [_vp],
+}
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+class ZkError(RuntimeError):
+    pass
+
+
+def load_library(path: str = LIB_PATH) -> ct.CDLL:
+    global _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise ZkError(f"{path} is not built: run `python -m pyzkp_amd.build` first")
+        lib = ct.CDLL(path)
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = ct.c_char_p if name == "zk_last_error" else ct.c_int
+        _lib = lib
+        return lib
+
+
+def native_available() -> bool:
+    """True when the library loads and a HIP device is visible."""
+    try:
+        lib = load_library()
+    except (ZkError, OSError):
+        return False
+    n = ct.c_int(0)
+    lib.zk_device_count(ct.byref(n))
+    return n.value > 0
+
+
+def _words(x: int) -> "ct.Array":
+    arr = (ct.c_uint64 * 4)()
+    for i in range(4):
+        arr[i] = (int(x) >> (64 * i)) & 0xFFFFFFFFFFFFFFFF
+    return arr
+
+
+class DeviceBuffer:
+    """A device allocation owned by one Context (freed on ``free()``)."""
+
+    __slots__ = ("ctx", "ptr", "nbytes", "__weakref__")
+
+    def __init__(self, ctx: "Context", ptr: int, nbytes: int) -> None:
+        self.ctx = ctx
+        self.ptr = ptr
+        self.nbytes = nbytes
+
+    def free(self) -> None:
+        if self.ptr:
+            self.ctx._check(self.ctx.lib.zk_free(self.ctx.handle, self.ptr))
+            self.ptr = 0
+
+    def offset(self, nbytes: int) -> int:
+        return self.ptr + nbytes
+
+
+class Context:
+    """One HIP device context (zk_init / zk_destroy)."""
+
+    def __init__(self, ordinal: int = 0) -> None:
+        self.lib = load_library()
+        h = _vp()
+        rc = self.lib.zk_init(int(ordinal), ct.byref(h))
+        if rc != ZK_OK:
+            msg = self.lib.zk_last_error(None).decode()
+            raise (ValueError if rc == ZK_ERR_INVALID else RuntimeError)(f"zk_init: {msg}")
+        self.handle = h
+        self.ordinal = ordinal
+
+    # ---- errors
+    def _check(self, rc: int) -> None:
+        if rc == ZK_OK:
+            return
+        msg = self.lib.zk_last_error(self.handle)
+        msg = msg.decode() if msg else "unknown error"
+        if rc == ZK_ERR_INVALID:
+            raise ValueError(msg)
+        raise RuntimeError(msg)
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.zk_destroy(self.handle)
+            self.handle = None
+
+    # ---- memory
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        p = _vp()
+        self._check(self.lib.zk_alloc(self.handle, int(nbytes), ct.byref(p)))
+        return DeviceBuffer(self, p.value, int(nbytes))
+
+    def pinned_alloc(self, nbytes: int) -> int:
+        p = _vp()
+        self._check(self.lib.zk_pinned_alloc(self.handle, int(nbytes), ct.byref(p)))
+        return p.value
+
+    def pinned_free(self, ptr: int) -> None:
+        self._check(self.lib.zk_pinned_free(self.handle, ptr))
+
+    def h2d(self, dst: int, src, nbytes: int, stream: Optional[int] = None) -> None:
+        self._check(self.lib.zk_h2d(self.handle, dst, src, int(nbytes), stream))
+
+    def d2h(self, dst, src: int, nbytes: int, stream: Optional[int] = None) -> None:
+        self._check(self.lib.zk_d2h(self.handle, dst, src, int(nbytes), stream))
+
+    def d2d(self, dst: int, src: int, nbytes: int, stream: Optional[int] = None) -> None:
+        self._check(self.lib.zk_d2d(self.handle, dst, src, int(nbytes), stream))
+
+    def memset_zero(self, dst: int, nbytes: int, stream: Optional[int] = None) -> None:
+        self._check(self.lib.zk_memset_zero(self.handle, dst, int(nbytes), stream))
+
+    def sync(self, stream: Optional[int] = None) -> None:
+        self._check(self.lib.zk_sync(self.handle, stream))
+
+    def upload(self, data: bytes, stream: Optional[int] = None) -> DeviceBuffer:
+        buf = self.alloc(max(len(data), 1))
+        if data:
+            self.h2d(buf.ptr, data, len(data), stream)
+            self.sync(stream)
+        return buf
+
+    def download(self, src: int, nbytes: int, stream: Optional[int] = None) -> bytes:
+        out = ct.create_string_buffer(int(nbytes))
+        if nbytes:
+            self.d2h(out, src, nbytes, stream)
+            self.sync(stream)
+        return out.raw
+
+    # ---- element-wise
+    def fr_to_mont(self, src: int, dst: int, n: int, stream=None) -> None:
+        self._check(self.lib.zk_fr_to_mont(self.handle, src, dst, int(n), stream))
+
+    def fr_from_mont(self, src: int, dst: int, n: int, stream=None) -> None:
+        self._check(self.lib.zk_fr_from_mont(self.handle, src, dst, int(n), stream))
+
+    def fq_to_mont(self, src: int, dst: int, n: int, stream=None) -> None:
+        self._check(self.lib.zk_fq_to_mont(self.handle, src, dst, int(n), stream))
+
+    def fq_from_mont(self, src: int, dst: int, n: int, stream=None) -> None:
+        self._check(self.lib.zk_fq_from_mont(self.handle, src, dst, int(n), stream))
+
+    def fr_mul(self, a: int, b: int, out: int, n: int, stream=None) -> None:
+        self._check(self.lib.zk_fr_mul(self.handle, a, b, out, int(n), stream))
+
+    def fr_add(self, a: int, b: int, out: int, n: int, stream=None) -> None:
+        self._check(self.lib.zk_fr_add(self.handle, a, b, out, int(n), stream))
+
+    def fr_sub(self, a: int, b: int, out: int, n: int, stream=None) -> None:
+        self._check(self.lib.zk_fr_sub(self.handle, a, b, out, int(n), stream))
+
+    def fr_scale(self, a: int, c_mont: int, out: int, n: int, stream=None) -> None:
+        self._check(self.lib.zk_fr_scale(self.handle, a, _words(c_mont), out, int(n), stream))
+
+    # ---- NTT / MSM
+    def ntt(self, src: int, in_len: int, dst: int, log_n: int, inverse: bool,
+            omega_mont: Optional[int] = None, shift_mont: Optional[int] = None, stream=None) -> None:
+        w = _words(omega_mont) if omega_mont is not None else None
+        s = _words(shift_mont) if shift_mont is not None else None
+        self._check(self.lib.zk_ntt(self.handle, src, int(in_len), dst, int(log_n), int(bool(inverse)),
+                                    w, s, stream))
+
+    def ntt_batch(self, src: int, dst: int, log_n: int, batch: int, inverse: bool,
+                  omega_mont: Optional[int] = None, shift_mont: Optional[int] = None, stream=None) -> None:
+        w = _words(omega_mont) if omega_mont is not None else None
+        s = _words(shift_mont) if shift_mont is not None else None
+        self._check(self.lib.zk_ntt_batch(self.handle, src, dst, int(log_n), int(batch), int(bool(inverse)),
+                                          w, s, stream))
+
+    def msm_g1(self, bases: int, scalars: int, n: int, window_bits: int = 0, stream=None) -> Sequence[int]:
+        """Returns Jacobian Montgomery (X, Y, Z) as three ints (Z = 0: identity)."""
+        out = (ct.c_uint64 * 12)()
+        self._check(self.lib.zk_msm_g1(self.handle, bases, scalars, int(n), int(window_bits), out, stream))
+        w = list(out)
+        f = lambda a: w[a] | (w[a + 1] << 64) | (w[a + 2] << 128) | (w[a + 3] << 192)
+        return f(0), f(4), f(8)
+
+    # ---- tuning / profiling
+    def set_tuning(self, key: str, value: int) -> None:
+        self._check(self.lib.zk_set_tuning(self.handle, key.encode(), int(value)))
+
+    def profile_enable(self, on: bool = True) -> None:
+        self._check(self.lib.zk_profile_enable(self.handle, int(bool(on))))
+
+    def profile_get(self, phase: str):
+        ms = ct.c_double(0.0)
+        cnt = ct.c_uint64(0)
+        self._check(self.lib.zk_profile_get(self.handle, phase.encode(), ct.byref(ms), ct.byref(cnt)))
+        return ms.value, cnt.value
+
+    def profile_reset(self) -> None:
+        self._check(self.lib.zk_profile_reset(self.handle))
+
+
+_default_ctx: dict = {}
+
+
+def get_context(ordinal: int = 0) -> Context:
+    """Process-wide context per device (created on first use)."""
+    ctx = _default_ctx.get(ordinal)
+    if ctx is None:
+        ctx = Context(ordinal)
+        _default_ctx[ordinal] = ctx
+    return ctx

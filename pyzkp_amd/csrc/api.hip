@@ -1,0 +1,351 @@
+// C-ABI entry points (declared in include/zkhip.h): context lifetime, error
+// reporting, memory and copies, and argument checking in front of the
+// NTT / MSM / element-wise drivers.
+#include <string.h>
+#include <stdio.h>
+#include "internal.hpp"
+#include "../../include/zkhip.h"
+
+struct zk_ctx {
+  zk::Ctx c;
+};
+
+namespace zk {
+
+static thread_local std::string g_init_err;
+
+int set_err(Ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_check(Ctx* c, hipError_t e, const char* what) {
+  char buf[512];
+  snprintf(buf, sizeof(buf), "HIP error %d (%s) in %s", (int)e, hipGetErrorString(e), what);
+  return set_err(c, ZK_ERR_RUNTIME, buf);
+}
+
+int ensure_scratch(Ctx* c, int slot, size_t bytes, void** out) {
+  DevBuf& b = c->scratch[slot];
+  if (b.bytes < bytes) {
+    if (b.ptr) {
+      // earlier launches on any stream may still read the old arena
+      ZK_HIP(c, hipDeviceSynchronize());
+      ZK_HIP(c, hipFree(b.ptr));
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    size_t want = bytes + bytes / 8;
+    ZK_HIP(c, hipMalloc(&b.ptr, want));
+    b.bytes = want;
+  }
+  *out = b.ptr;
+  return ZK_OK;
+}
+
+int upload_table(Ctx* c, const std::vector<zkh::Fe>& host, DevBuf* out, hipStream_t st) {
+  out->bytes = host.size() * 32;
+  ZK_HIP(c, hipMalloc(&out->ptr, out->bytes));
+  // synchronous copy: the host vector dies when the caller returns
+  ZK_HIP(c, hipStreamSynchronize(st));
+  ZK_HIP(c, hipMemcpy(out->ptr, host.data(), out->bytes, hipMemcpyHostToDevice));
+  return ZK_OK;
+}
+
+void phase_begin(Ctx* c, const char* name, hipStream_t st, hipEvent_t* ev) {
+  (void)name;
+  *ev = nullptr;
+  if (!c->profiling) return;
+  if (hipEventCreate(ev) != hipSuccess) {
+    *ev = nullptr;
+    return;
+  }
+  hipEventRecord(*ev, st);
+}
+
+void phase_end(Ctx* c, const char* name, hipStream_t st, hipEvent_t ev) {
+  if (!c->profiling || ev == nullptr) return;
+  hipEvent_t stop;
+  if (hipEventCreate(&stop) != hipSuccess) return;
+  hipEventRecord(stop, st);
+  c->phases[name].pending.emplace_back(ev, stop);
+}
+
+static void collect(Ctx* c) {
+  for (auto& kv : c->phases) {
+    PhaseTimer& t = kv.second;
+    for (auto& pr : t.pending) {
+      hipEventSynchronize(pr.second);
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+        t.total_ms += ms;
+        t.launches += 1;
+      }
+      hipEventDestroy(pr.first);
+      hipEventDestroy(pr.second);
+    }
+    t.pending.clear();
+  }
+}
+
+static zkh::Fe words_to_fe(const uint64_t w[4]) {
+  zkh::Fe f;
+  memcpy(f.v, w, 32);
+  return f;
+}
+
+}  // namespace zk
+
+using namespace zk;
+
+#define CTX_CHECK(ctx)                       \
+  do {                                       \
+    if (!(ctx)) return ZK_ERR_INVALID;       \
+    hipSetDevice((ctx)->c.device);           \
+  } while (0)
+
+static inline hipStream_t pick(zk_ctx* ctx, void* stream) {
+  return stream ? (hipStream_t)stream : ctx->c.stream;
+}
+
+extern "C" {
+
+int zk_version(void) { return 100; }
+
+int zk_device_count(int* count) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return ZK_OK;
+}
+
+int zk_init(int ordinal, zk_ctx** out) {
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    g_init_err = "no HIP device available";
+    return ZK_ERR_NO_DEVICE;
+  }
+  if (ordinal < 0 || ordinal >= n) {
+    g_init_err = "device ordinal out of range";
+    return ZK_ERR_INVALID;
+  }
+  hipError_t e = hipSetDevice(ordinal);
+  if (e != hipSuccess) {
+    g_init_err = std::string("hipSetDevice failed: ") + hipGetErrorString(e);
+    return ZK_ERR_RUNTIME;
+  }
+  zk_ctx* ctx = new zk_ctx();
+  ctx->c.device = ordinal;
+  e = hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    g_init_err = std::string("hipStreamCreate failed: ") + hipGetErrorString(e);
+    delete ctx;
+    return ZK_ERR_RUNTIME;
+  }
+  *out = ctx;
+  return ZK_OK;
+}
+
+int zk_destroy(zk_ctx* ctx) {
+  if (!ctx) return ZK_OK;
+  hipSetDevice(ctx->c.device);
+  hipDeviceSynchronize();
+  collect(&ctx->c);
+  for (auto& kv : ctx->c.twiddles) hipFree(kv.second.ptr);
+  for (auto& kv : ctx->c.pow_tables) {
+    hipFree(kv.second.first.ptr);
+    hipFree(kv.second.second.ptr);
+  }
+  for (auto& b : ctx->c.scratch)
+    if (b.ptr) hipFree(b.ptr);
+  hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+  return ZK_OK;
+}
+
+const char* zk_last_error(const zk_ctx* ctx) {
+  if (!ctx) return g_init_err.c_str();
+  return ctx->c.err.c_str();
+}
+
+int zk_alloc(zk_ctx* ctx, size_t bytes, void** out) {
+  CTX_CHECK(ctx);
+  *out = nullptr;
+  if (bytes == 0) bytes = 256;
+  ZK_HIP(&ctx->c, hipMalloc(out, bytes));
+  return ZK_OK;
+}
+
+int zk_free(zk_ctx* ctx, void* ptr) {
+  CTX_CHECK(ctx);
+  if (ptr) ZK_HIP(&ctx->c, hipFree(ptr));
+  return ZK_OK;
+}
+
+int zk_pinned_alloc(zk_ctx* ctx, size_t bytes, void** out) {
+  CTX_CHECK(ctx);
+  *out = nullptr;
+  if (bytes == 0) bytes = 256;
+  ZK_HIP(&ctx->c, hipHostMalloc(out, bytes, hipHostMallocDefault));
+  return ZK_OK;
+}
+
+int zk_pinned_free(zk_ctx* ctx, void* ptr) {
+  CTX_CHECK(ctx);
+  if (ptr) ZK_HIP(&ctx->c, hipHostFree(ptr));
+  return ZK_OK;
+}
+
+int zk_h2d(zk_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream) {
+  CTX_CHECK(ctx);
+  if (bytes == 0) return ZK_OK;
+  ZK_HIP(&ctx->c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, pick(ctx, stream)));
+  return ZK_OK;
+}
+
+int zk_d2h(zk_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream) {
+  CTX_CHECK(ctx);
+  if (bytes == 0) return ZK_OK;
+  ZK_HIP(&ctx->c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, pick(ctx, stream)));
+  return ZK_OK;
+}
+
+int zk_d2d(zk_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream) {
+  CTX_CHECK(ctx);
+  if (bytes == 0) return ZK_OK;
+  ZK_HIP(&ctx->c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, pick(ctx, stream)));
+  return ZK_OK;
+}
+
+int zk_memset_zero(zk_ctx* ctx, void* dst, size_t bytes, void* stream) {
+  CTX_CHECK(ctx);
+  if (bytes == 0) return ZK_OK;
+  ZK_HIP(&ctx->c, hipMemsetAsync(dst, 0, bytes, pick(ctx, stream)));
+  return ZK_OK;
+}
+
+int zk_sync(zk_ctx* ctx, void* stream) {
+  CTX_CHECK(ctx);
+  ZK_HIP(&ctx->c, hipStreamSynchronize(pick(ctx, stream)));
+  return ZK_OK;
+}
+
+static int elt(zk_ctx* ctx, int op, const void* a, const void* b, void* out, uint64_t n, void* stream) {
+  CTX_CHECK(ctx);
+  if (n && (!a || !out)) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  return elementwise(&ctx->c, op, (const fe*)a, (const fe*)b, (fe*)out, n, pick(ctx, stream));
+}
+
+int zk_fr_to_mont(zk_ctx* ctx, const void* in, void* out, uint64_t n, void* stream) {
+  return elt(ctx, ELT_FR_TO_MONT, in, nullptr, out, n, stream);
+}
+int zk_fr_from_mont(zk_ctx* ctx, const void* in, void* out, uint64_t n, void* stream) {
+  return elt(ctx, ELT_FR_FROM_MONT, in, nullptr, out, n, stream);
+}
+int zk_fq_to_mont(zk_ctx* ctx, const void* in, void* out, uint64_t n, void* stream) {
+  return elt(ctx, ELT_FQ_TO_MONT, in, nullptr, out, n, stream);
+}
+int zk_fq_from_mont(zk_ctx* ctx, const void* in, void* out, uint64_t n, void* stream) {
+  return elt(ctx, ELT_FQ_FROM_MONT, in, nullptr, out, n, stream);
+}
+int zk_fr_mul(zk_ctx* ctx, const void* a, const void* b, void* out, uint64_t n, void* stream) {
+  if (n && !b) return ctx ? set_err(&ctx->c, ZK_ERR_INVALID, "null buffer") : ZK_ERR_INVALID;
+  return elt(ctx, ELT_FR_MUL, a, b, out, n, stream);
+}
+int zk_fr_add(zk_ctx* ctx, const void* a, const void* b, void* out, uint64_t n, void* stream) {
+  if (n && !b) return ctx ? set_err(&ctx->c, ZK_ERR_INVALID, "null buffer") : ZK_ERR_INVALID;
+  return elt(ctx, ELT_FR_ADD, a, b, out, n, stream);
+}
+int zk_fr_sub(zk_ctx* ctx, const void* a, const void* b, void* out, uint64_t n, void* stream) {
+  if (n && !b) return ctx ? set_err(&ctx->c, ZK_ERR_INVALID, "null buffer") : ZK_ERR_INVALID;
+  return elt(ctx, ELT_FR_SUB, a, b, out, n, stream);
+}
+
+int zk_fr_scale(zk_ctx* ctx, const void* a, const uint64_t c_mont[4], void* out, uint64_t n,
+                void* stream) {
+  CTX_CHECK(ctx);
+  if (!c_mont) return set_err(&ctx->c, ZK_ERR_INVALID, "null constant");
+  return elementwise_scale(&ctx->c, (const fe*)a, c_mont, (fe*)out, n, pick(ctx, stream));
+}
+
+int zk_ntt(zk_ctx* ctx, const void* in, uint64_t in_len, void* out, uint32_t log_n, int inverse,
+           const uint64_t omega_mont[4], const uint64_t shift_mont[4], void* stream) {
+  CTX_CHECK(ctx);
+  if (!out || (in_len && !in)) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  if (inverse && in_len != (1ull << (log_n > 63 ? 63 : log_n)))
+    return set_err(&ctx->c, ZK_ERR_INVALID, "inverse NTT needs in_len == n");
+  zkh::Fe w, s;
+  if (omega_mont) w = words_to_fe(omega_mont);
+  if (shift_mont) s = words_to_fe(shift_mont);
+  return ntt_run(&ctx->c, (const fe*)in, in_len, (fe*)out, log_n, 1, inverse != 0,
+                 omega_mont ? &w : nullptr, shift_mont ? &s : nullptr, pick(ctx, stream),
+                 ctx->c.ntt_max_pass_bits);
+}
+
+int zk_ntt_batch(zk_ctx* ctx, const void* in, void* out, uint32_t log_n, uint32_t batch,
+                 int inverse, const uint64_t omega_mont[4], const uint64_t shift_mont[4],
+                 void* stream) {
+  CTX_CHECK(ctx);
+  if (batch && (!in || !out)) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  if (log_n > 28) return set_err(&ctx->c, ZK_ERR_INVALID, "domain too large for BN254 root of unity");
+  zkh::Fe w, s;
+  if (omega_mont) w = words_to_fe(omega_mont);
+  if (shift_mont) s = words_to_fe(shift_mont);
+  return ntt_run(&ctx->c, (const fe*)in, 1ull << log_n, (fe*)out, log_n, batch, inverse != 0,
+                 omega_mont ? &w : nullptr, shift_mont ? &s : nullptr, pick(ctx, stream),
+                 ctx->c.ntt_max_pass_bits);
+}
+
+int zk_msm_g1(zk_ctx* ctx, const void* bases, const void* scalars, uint64_t n, int window_bits,
+              uint64_t out_jac_mont[12], void* stream) {
+  CTX_CHECK(ctx);
+  if (!out_jac_mont) return set_err(&ctx->c, ZK_ERR_INVALID, "null output");
+  if (n && (!bases || !scalars)) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  return msm_g1_run(&ctx->c, (const fe*)bases, (const fe*)scalars, n, window_bits, out_jac_mont,
+                    pick(ctx, stream));
+}
+
+int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value) {
+  CTX_CHECK(ctx);
+  if (!key) return set_err(&ctx->c, ZK_ERR_INVALID, "null key");
+  std::string k(key);
+  if (k == "ntt_max_pass_bits") {
+    if (value < 0 || value > 11) return set_err(&ctx->c, ZK_ERR_INVALID, "ntt_max_pass_bits must be in [0, 11]");
+    ctx->c.ntt_max_pass_bits = (int)value;
+  } else if (k == "msm_segment") {
+    if (value < 1 || value > 4096) return set_err(&ctx->c, ZK_ERR_INVALID, "msm_segment must be in [1, 4096]");
+    ctx->c.msm_segment = (uint32_t)value;
+  } else if (k == "msm_reduce_chunk") {
+    if (value < 1 || value > 65536) return set_err(&ctx->c, ZK_ERR_INVALID, "msm_reduce_chunk must be in [1, 65536]");
+    ctx->c.msm_reduce_chunk = (uint32_t)value;
+  } else {
+    return set_err(&ctx->c, ZK_ERR_INVALID, "unknown tuning key: " + k);
+  }
+  return ZK_OK;
+}
+
+int zk_profile_enable(zk_ctx* ctx, int on) {
+  CTX_CHECK(ctx);
+  ctx->c.profiling = on != 0;
+  return ZK_OK;
+}
+
+int zk_profile_get(zk_ctx* ctx, const char* phase, double* total_ms, uint64_t* launches) {
+  CTX_CHECK(ctx);
+  collect(&ctx->c);
+  auto it = ctx->c.phases.find(phase ? phase : "");
+  *total_ms = it == ctx->c.phases.end() ? 0.0 : it->second.total_ms;
+  *launches = it == ctx->c.phases.end() ? 0 : it->second.launches;
+  return ZK_OK;
+}
+
+int zk_profile_reset(zk_ctx* ctx) {
+  CTX_CHECK(ctx);
+  collect(&ctx->c);
+  ctx->c.phases.clear();
+  return ZK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,159 @@
+// Host-side BN254 Fr/Fq Montgomery arithmetic (4 x 64-bit limbs, R = 2^256).
+// Used by the library's host code for table generation and for the final
+// window combination of an MSM; never on the per-element device path.
+#pragma once
+#include <stdint.h>
+#include <string.h>
+
+namespace zkh {
+
+typedef uint64_t u64;
+typedef unsigned __int128 u128;
+
+struct Fe { u64 v[4]; };
+
+struct Mod {
+  u64 p[4];
+  u64 pinv;   // -p^{-1} mod 2^64
+  u64 one[4]; // R mod p
+  u64 r2[4];  // R^2 mod p
+};
+
+static const Mod FR = {
+    {0x43e1f593f0000001ull, 0x2833e84879b97091ull, 0xb85045b68181585dull, 0x30644e72e131a029ull},
+    0xc2e1f593efffffffull,
+    {0xac96341c4ffffffbull, 0x36fc76959f60cd29ull, 0x666ea36f7879462eull, 0x0e0a77c19a07df2full},
+    {0x1bb8e645ae216da7ull, 0x53fe3ab1e35c59e3ull, 0x8c49833d53bb8085ull, 0x0216d0b17f4e44a5ull}};
+
+static const Mod FQ = {
+    {0x3c208c16d87cfd47ull, 0x97816a916871ca8dull, 0xb85045b68181585dull, 0x30644e72e131a029ull},
+    0x87d20782e4866389ull,
+    {0xd35d438dc58f0d9dull, 0x0a78eb28f5c70b3dull, 0x666ea36f7879462cull, 0x0e0a77c19a07df2full},
+    {0xf32cfc5b538afa89ull, 0xb5e71911d44501fbull, 0x47ab1eff0a417ff6ull, 0x06d89f71cab8351full}};
+
+inline bool geq(const u64* a, const u64* b) {
+  for (int i = 3; i >= 0; i--) {
+    if (a[i] != b[i]) return a[i] > b[i];
+  }
+  return true;
+}
+
+inline void sub_raw(u64* r, const u64* a, const u64* b) {
+  u64 br = 0;
+  for (int i = 0; i < 4; i++) {
+    u128 d = (u128)a[i] - b[i] - br;
+    r[i] = (u64)d;
+    br = (u64)(d >> 127);
+  }
+}
+
+inline bool is_zero(const Fe& a) { return (a.v[0] | a.v[1] | a.v[2] | a.v[3]) == 0; }
+inline bool eq(const Fe& a, const Fe& b) { return memcmp(a.v, b.v, 32) == 0; }
+
+inline Fe add(const Mod& M, const Fe& a, const Fe& b) {
+  Fe r;
+  u64 c = 0;
+  for (int i = 0; i < 4; i++) {
+    u128 s = (u128)a.v[i] + b.v[i] + c;
+    r.v[i] = (u64)s;
+    c = (u64)(s >> 64);
+  }
+  if (geq(r.v, M.p)) sub_raw(r.v, r.v, M.p);
+  return r;
+}
+
+inline Fe sub(const Mod& M, const Fe& a, const Fe& b) {
+  Fe r;
+  if (geq(a.v, b.v)) {
+    sub_raw(r.v, a.v, b.v);
+  } else {
+    u64 t[4];
+    sub_raw(t, M.p, b.v);
+    u64 c = 0;
+    for (int i = 0; i < 4; i++) {
+      u128 s = (u128)a.v[i] + t[i] + c;
+      r.v[i] = (u64)s;
+      c = (u64)(s >> 64);
+    }
+  }
+  return r;
+}
+
+inline Fe neg(const Mod& M, const Fe& a) {
+  if (is_zero(a)) return a;
+  Fe r;
+  sub_raw(r.v, M.p, a.v);
+  return r;
+}
+
+// CIOS Montgomery product with an explicit carry word.
+inline Fe mul(const Mod& M, const Fe& a, const Fe& b) {
+  u64 t[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < 4; i++) {
+    u64 c = 0;
+    for (int j = 0; j < 4; j++) {
+      u128 s = (u128)a.v[j] * b.v[i] + t[j] + c;
+      t[j] = (u64)s;
+      c = (u64)(s >> 64);
+    }
+    u128 s = (u128)t[4] + c;
+    t[4] = (u64)s;
+    t[5] = (u64)(s >> 64);
+    u64 m = t[0] * M.pinv;
+    s = (u128)m * M.p[0] + t[0];
+    c = (u64)(s >> 64);
+    for (int j = 1; j < 4; j++) {
+      s = (u128)m * M.p[j] + t[j] + c;
+      t[j - 1] = (u64)s;
+      c = (u64)(s >> 64);
+    }
+    s = (u128)t[4] + c;
+    t[3] = (u64)s;
+    t[4] = t[5] + (u64)(s >> 64);
+  }
+  Fe r;
+  memcpy(r.v, t, 32);
+  if (t[4] != 0 || geq(r.v, M.p)) sub_raw(r.v, r.v, M.p);
+  return r;
+}
+
+inline Fe one(const Mod& M) { Fe r; memcpy(r.v, M.one, 32); return r; }
+inline Fe zero() { Fe r = {{0, 0, 0, 0}}; return r; }
+
+inline Fe to_mont(const Mod& M, const Fe& a) {
+  Fe r2; memcpy(r2.v, M.r2, 32);
+  return mul(M, a, r2);
+}
+inline Fe from_mont(const Mod& M, const Fe& a) {
+  Fe o = zero(); o.v[0] = 1;
+  return mul(M, a, o);
+}
+inline Fe from_u64(const Mod& M, u64 x) {
+  Fe a = zero(); a.v[0] = x;
+  return to_mont(M, a);
+}
+
+// a^e for a Montgomery-form base and a plain 256-bit exponent
+inline Fe pow(const Mod& M, const Fe& a, const u64 e[4]) {
+  Fe r = one(M);
+  for (int i = 3; i >= 0; i--) {
+    for (int b = 63; b >= 0; b--) {
+      r = mul(M, r, r);
+      if ((e[i] >> b) & 1) r = mul(M, r, a);
+    }
+  }
+  return r;
+}
+inline Fe pow_u64(const Mod& M, const Fe& a, u64 e) {
+  u64 ee[4] = {e, 0, 0, 0};
+  return pow(M, a, ee);
+}
+// inverse by Fermat: a^(p-2); returns 0 for 0
+inline Fe inv(const Mod& M, const Fe& a) {
+  u64 e[4];
+  memcpy(e, M.p, 32);
+  e[0] -= 2;  // p is odd and > 2: no borrow out of limb 0
+  return pow(M, a, e);
+}
+
+}  // namespace zkh

@@ -1,0 +1,91 @@
+// Library-internal declarations shared by the HIP translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <map>
+#include <string>
+#include <vector>
+#include <array>
+
+#include "field.hpp"
+#include "host_field.hpp"
+
+// Status codes of the C-ABI (mirrored in include/zkhip.h).
+#define ZK_OK 0
+#define ZK_ERR_INVALID 1
+#define ZK_ERR_RUNTIME 2
+#define ZK_ERR_NO_DEVICE 3
+
+namespace zk {
+
+// A device allocation owned by the context (tables, scratch).
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+// Per-phase device timing, recorded with HIP events on the launch stream
+// when profiling is switched on for a context.
+struct PhaseTimer {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;  // (start, stop)
+  double total_ms = 0.0;
+  uint64_t launches = 0;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // NTT twiddle tables omega^e, e < n/2, keyed by (log_n, omega Montgomery words)
+  std::map<std::array<uint64_t, 5>, DevBuf> twiddles;
+  // two-level power tables (hi, lo) for coset / inverse scaling,
+  // keyed by (log_n, base words, scale words)
+  std::map<std::array<uint64_t, 9>, std::pair<DevBuf, DevBuf>> pow_tables;
+  // growable scratch arenas
+  DevBuf scratch[4];
+  // tuning knobs (set through zk_set_tuning)
+  int ntt_max_pass_bits = 0;      // 0: planner default
+  uint32_t msm_segment = 64;      // sorted (bucket, point) pairs per accumulation thread
+  uint32_t msm_reduce_chunk = 32; // buckets per reduction thread
+  bool profiling = false;
+  std::map<std::string, PhaseTimer> phases;
+};
+
+// error helpers --------------------------------------------------------------
+int set_err(Ctx* c, int code, const std::string& msg);
+int hip_check(Ctx* c, hipError_t e, const char* what);
+#define ZK_HIP(ctx, call)                                          \
+  do {                                                             \
+    hipError_t _e = (call);                                        \
+    if (_e != hipSuccess) return ::zk::hip_check((ctx), _e, #call); \
+  } while (0)
+
+// device memory helpers (context-owned) -------------------------------------
+int ensure_scratch(Ctx* c, int slot, size_t bytes, void** out);
+int upload_table(Ctx* c, const std::vector<zkh::Fe>& host, DevBuf* out, hipStream_t st);
+
+// profiling helpers: bracket a launch with events when profiling is on
+void phase_begin(Ctx* c, const char* name, hipStream_t st, hipEvent_t* ev);
+void phase_end(Ctx* c, const char* name, hipStream_t st, hipEvent_t ev);
+
+// NTT ----------------------------------------------------------------------
+int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint32_t batch,
+            bool inverse, const zkh::Fe* omega, const zkh::Fe* shift, hipStream_t st,
+            int max_pass_bits);
+
+// element-wise -----------------------------------------------------------------
+enum EltOp { ELT_FR_TO_MONT = 0, ELT_FR_FROM_MONT, ELT_FQ_TO_MONT, ELT_FQ_FROM_MONT,
+             ELT_FR_MUL, ELT_FR_ADD, ELT_FR_SUB };
+int elementwise(Ctx* c, int op, const fe* a, const fe* b, fe* out, uint64_t n, hipStream_t st);
+int elementwise_scale(Ctx* c, const fe* a, const uint64_t c_mont[4], fe* out, uint64_t n, hipStream_t st);
+
+// MSM ------------------------------------------------------------------------
+int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
+               uint64_t out_jac[12], hipStream_t st);
+
+inline unsigned grid_for(uint64_t n, unsigned block) {
+  uint64_t g = (n + block - 1) / block;
+  return (unsigned)(g == 0 ? 1 : g);
+}
+
+}  // namespace zk

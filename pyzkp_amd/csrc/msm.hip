@@ -1,0 +1,423 @@
+// BN254 G1 multi-scalar multiplication on gfx950: signed-digit Pippenger
+// with CSR buckets built by a device counting sort, load-balanced bucket
+// accumulation in XYZZ coordinates, and a parallel bucket reduction.
+//
+// Follows the reference's Metal MSM v2 (runtime/metal/msm.py:145-332,
+// runtime/metal/ecc_source.py:437-566) in what is computed:
+//   * window count W = ceil(255 / c), signed digits in [-2^(c-1), 2^(c-1)-1]
+//     with a carry into the next window (runtime/metal/msm.py:145-177),
+//   * 2^(c-1) + 1 buckets per window, bucket |d| takes +-P (sign from d),
+//   * window sums Sum_d d * B_d, combined as Sum_w 2^(cw) S_w on the host
+//     (runtime/metal/msm.py:322-332).
+// It differs in how: digits come from the device-resident Montgomery
+// scalars on the device (no host loop), the CSR build is a parallel
+// histogram / scan / scatter, each thread owns a fixed-length segment of the
+// sorted (bucket, point) list instead of a whole bucket (so skewed buckets
+// do not serialise a thread), and the bucket reduction is split into chunks
+// whose weighted partial sums are recombined in parallel.
+#include "internal.hpp"
+#include "curve.hpp"
+
+namespace zk {
+
+static constexpr uint32_t NO_KEY = 0xffffffffu;
+static constexpr uint32_t SIGN = 0x80000000u;
+
+__device__ __forceinline__ aff load_aff(const fe* __restrict__ bases, uint32_t idx) {
+  aff a;
+  a.x = load_fe(bases + 2 * (uint64_t)idx);
+  a.y = load_fe(bases + 2 * (uint64_t)idx + 1);
+  return a;
+}
+
+__device__ __forceinline__ xyzz load_xyzz(const xyzz* p) {
+  const fe* f = reinterpret_cast<const fe*>(p);
+  xyzz r;
+  r.x = load_fe(f);
+  r.y = load_fe(f + 1);
+  r.zz = load_fe(f + 2);
+  r.zzz = load_fe(f + 3);
+  return r;
+}
+
+__device__ __forceinline__ void store_xyzz(xyzz* p, const xyzz& v) {
+  fe* f = reinterpret_cast<fe*>(p);
+  store_fe(f, v.x);
+  store_fe(f + 1, v.y);
+  store_fe(f + 2, v.zz);
+  store_fe(f + 3, v.zzz);
+}
+
+// 1. scalar (Montgomery) -> canonical -> W signed c-bit digits; histogram of
+//    the nonzero digit magnitudes per (window, bucket).
+//    digits[w * n + i] = |d| | (d < 0 ? SIGN : 0)
+__global__ __launch_bounds__(256) void msm_digits_kernel(const fe* __restrict__ scalars, uint64_t n,
+                                                         uint32_t c, uint32_t W, uint32_t B,
+                                                         uint32_t* __restrict__ digits,
+                                                         uint32_t* __restrict__ counts) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const fe s = Fr::from_mont(load_fe(scalars + i));
+  uint64_t buf = (uint64_t)s.w[0] | ((uint64_t)s.w[1] << 32);
+  int have = 64;
+  uint32_t nx[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) nx[k] = s.w[k + 2];
+  const uint32_t mask = (1u << c) - 1;
+  const uint32_t half = 1u << (c - 1);
+  uint32_t carry = 0;
+  for (uint32_t w = 0; w < W; w++) {
+    if (have < (int)c) {
+      buf |= (uint64_t)nx[0] << have;
+      have += 32;
+#pragma unroll
+      for (int k = 0; k < 5; k++) nx[k] = nx[k + 1];
+      nx[5] = 0;
+    }
+    const uint32_t val = (uint32_t)(buf & mask) + carry;
+    buf >>= c;
+    have -= (int)c;
+    uint32_t out;
+    if (val >= half) {
+      const uint32_t mag = (1u << c) - val;  // digit = val - 2^c
+      carry = 1;
+      out = mag ? (mag | SIGN) : 0u;
+    } else {
+      carry = 0;
+      out = val;
+    }
+    digits[(uint64_t)w * n + i] = out;
+    if (out & ~SIGN) atomicAdd(&counts[w * B + (out & ~SIGN)], 1u);
+  }
+}
+
+// 2. exclusive scan of the bucket counts -> row pointers (3 small kernels)
+static constexpr int SCAN_THREADS = 256;
+static constexpr int SCAN_PER_THREAD = 8;
+static constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_PER_THREAD;
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
+  const uint32_t tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
+    uint32_t x = (tid >= off) ? sh[tid - off] : 0;
+    __syncthreads();
+    sh[tid] += x;
+    __syncthreads();
+  }
+  total = sh[blockDim.x - 1];
+  const uint32_t incl = sh[tid];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void scan_chunks_kernel(const uint32_t* __restrict__ in,
+                                                                   uint32_t* __restrict__ out,
+                                                                   uint32_t* __restrict__ sums,
+                                                                   uint32_t len) {
+  __shared__ uint32_t sh[SCAN_THREADS];
+  const uint32_t base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER_THREAD;
+  uint32_t v[SCAN_PER_THREAD];
+  uint32_t local = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_PER_THREAD; k++) {
+    v[k] = (base + k < len) ? in[base + k] : 0;
+    local += v[k];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(local, sh, total);
+#pragma unroll
+  for (int k = 0; k < SCAN_PER_THREAD; k++) {
+    if (base + k < len) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// one block scans all chunk sums (at most SCAN_CHUNK of them) in place and
+// writes the grand total to out_total
+__global__ __launch_bounds__(SCAN_THREADS) void scan_sums_kernel(uint32_t* __restrict__ sums,
+                                                                 uint32_t nsums,
+                                                                 uint32_t* __restrict__ out_total) {
+  __shared__ uint32_t sh[SCAN_THREADS];
+  const uint32_t base = threadIdx.x * SCAN_PER_THREAD;
+  uint32_t v[SCAN_PER_THREAD];
+  uint32_t local = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_PER_THREAD; k++) {
+    v[k] = (base + k < nsums) ? sums[base + k] : 0;
+    local += v[k];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(local, sh, total);
+#pragma unroll
+  for (int k = 0; k < SCAN_PER_THREAD; k++) {
+    if (base + k < nsums) sums[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0) *out_total = total;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void scan_add_kernel(uint32_t* __restrict__ out,
+                                                                uint32_t* __restrict__ cursor,
+                                                                const uint32_t* __restrict__ sums,
+                                                                uint32_t len) {
+  const uint32_t base = blockIdx.x * SCAN_CHUNK;
+  const uint32_t add = sums[blockIdx.x];
+  for (uint32_t k = threadIdx.x; k < SCAN_CHUNK; k += SCAN_THREADS) {
+    const uint32_t idx = base + k;
+    if (idx < len) {
+      const uint32_t v = out[idx] + add;
+      out[idx] = v;
+      cursor[idx] = v;
+    }
+  }
+}
+
+// 3. scatter: sorted[pos] = point index | sign, grouped by (window, bucket)
+__global__ __launch_bounds__(256) void msm_scatter_kernel(const uint32_t* __restrict__ digits,
+                                                          uint64_t n, uint32_t W, uint32_t B,
+                                                          uint32_t* __restrict__ cursor,
+                                                          uint32_t* __restrict__ sorted) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= n * W) return;
+  const uint32_t d = digits[gid];
+  const uint32_t mag = d & ~SIGN;
+  if (!mag) return;
+  const uint32_t w = (uint32_t)(gid / n);
+  const uint32_t i = (uint32_t)(gid - (uint64_t)w * n);
+  const uint32_t pos = atomicAdd(&cursor[w * B + mag], 1u);
+  sorted[pos] = i | (d & SIGN);
+}
+
+// 4. bucket accumulation over fixed-length segments of the sorted list.
+//    A run (bucket) that starts inside the segment is owned by it and its
+//    partial sum goes straight to buckets[key]; the run the segment starts
+//    in the middle of is its "head" partial, added in by the fix-up pass.
+__global__ __launch_bounds__(256) void msm_accumulate_kernel(
+    const fe* __restrict__ bases, const uint32_t* __restrict__ sorted,
+    const uint32_t* __restrict__ row_ptr, uint32_t nkeys, uint32_t L, xyzz* __restrict__ buckets,
+    uint32_t* __restrict__ head_key, xyzz* __restrict__ head) {
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t M = row_ptr[nkeys];
+  const uint64_t start = s * L;
+  if (start >= M) return;
+  const uint64_t end = (start + L < M) ? start + L : M;
+  // largest k with row_ptr[k] <= start  (row_ptr[nkeys] = M > start)
+  uint32_t lo = 0, hi = nkeys;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (row_ptr[mid] <= start) lo = mid; else hi = mid;
+  }
+  uint32_t k = lo;
+  uint32_t rbeg = row_ptr[k], rend = row_ptr[k + 1];
+  uint32_t hk = NO_KEY;
+  xyzz acc = xyzz_inf();
+  for (uint64_t p = start; p < end; p++) {
+    while (p >= rend) {
+      if (rend > rbeg) {
+        if (rbeg < start) {
+          hk = k;
+          store_xyzz(head + s, acc);
+        } else {
+          store_xyzz(buckets + k, acc);
+        }
+      }
+      acc = xyzz_inf();
+      k++;
+      rbeg = rend;
+      rend = row_ptr[k + 1];
+    }
+    const uint32_t v = sorted[p];
+    aff a = load_aff(bases, v & ~SIGN);
+    if (!aff_is_inf(a)) {
+      if (v & SIGN) a.y = Fq::neg(a.y);
+      xyzz_madd(acc, a);
+    }
+  }
+  if (rbeg < start) {
+    hk = k;
+    store_xyzz(head + s, acc);
+  } else {
+    store_xyzz(buckets + k, acc);
+  }
+  head_key[s] = hk;
+}
+
+// 5. fix-up: the first segment holding a continuation of bucket k adds all
+//    consecutive head partials of k into buckets[k] (owner already wrote it)
+__global__ __launch_bounds__(256) void msm_fixup_kernel(const uint32_t* __restrict__ row_ptr,
+                                                        uint32_t nkeys, uint32_t L,
+                                                        xyzz* __restrict__ buckets,
+                                                        const uint32_t* __restrict__ head_key,
+                                                        const xyzz* __restrict__ head) {
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t M = row_ptr[nkeys];
+  const uint64_t S = (M + L - 1) / L;
+  if (s >= S) return;
+  const uint32_t k = head_key[s];
+  if (k == NO_KEY) return;
+  if (s > 0 && head_key[s - 1] == k) return;
+  xyzz acc = xyzz_add(load_xyzz(buckets + k), load_xyzz(head + s));
+  for (uint64_t s2 = s + 1; s2 < S && head_key[s2] == k; s2++) acc = xyzz_add(acc, load_xyzz(head + s2));
+  store_xyzz(buckets + k, acc);
+}
+
+// 6. bucket reduction, chunked: chunk t of window w covers buckets
+//    [lo, lo + K) with lo = 1 + t K; its weighted sum
+//      Sum_b b B_b = Sum_b (b - lo + 1) B_b + (lo - 1) Sum_b B_b
+//    is a running sum from the top plus one small scalar multiple.
+__global__ __launch_bounds__(128) void msm_reduce_chunks_kernel(const xyzz* __restrict__ buckets,
+                                                                const uint32_t* __restrict__ row_ptr,
+                                                                uint32_t W, uint32_t B, uint32_t K,
+                                                                uint32_t cpw,
+                                                                xyzz* __restrict__ partial) {
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= W * cpw) return;
+  const uint32_t w = gid / cpw, t = gid - w * cpw;
+  const uint32_t lo = 1 + t * K;
+  uint32_t hi = lo + K - 1;
+  if (hi > B - 1) hi = B - 1;
+  xyzz run = xyzz_inf(), sum = xyzz_inf();
+  for (uint32_t b = hi; b >= lo; b--) {
+    const uint32_t key = w * B + b;
+    if (row_ptr[key + 1] > row_ptr[key]) run = xyzz_add(run, load_xyzz(buckets + key));
+    sum = xyzz_add(sum, run);
+  }
+  if (lo > 1) sum = xyzz_add(sum, xyzz_mul_small(run, lo - 1));
+  store_xyzz(partial + gid, sum);
+}
+
+// 7. per-window tree sum of the chunk partials
+__global__ __launch_bounds__(128) void msm_reduce_windows_kernel(const xyzz* __restrict__ partial,
+                                                                 uint32_t cpw,
+                                                                 xyzz* __restrict__ window_sums) {
+  __shared__ xyzz sh[128];
+  const uint32_t w = blockIdx.x, tid = threadIdx.x;
+  xyzz acc = xyzz_inf();
+  for (uint32_t t = tid; t < cpw; t += blockDim.x) acc = xyzz_add(acc, load_xyzz(partial + (uint64_t)w * cpw + t));
+  sh[tid] = acc;
+  __syncthreads();
+  for (uint32_t step = blockDim.x >> 1; step > 0; step >>= 1) {
+    if (tid < step) sh[tid] = xyzz_add(sh[tid], sh[tid + step]);
+    __syncthreads();
+  }
+  if (tid == 0) store_xyzz(window_sums + w, sh[0]);
+}
+
+// ---------------------------------------------------------------- host side
+
+// c = max(4, min(16, bitlen(n) - 2))   (runtime/metal/msm.py:191,
+// runtime/kernels/cpu/kernels.py:557-560)
+static int default_window_bits(uint64_t n) {
+  int bl = 0;
+  while ((n >> bl) != 0) bl++;
+  int c = bl - 2;
+  if (c > 16) c = 16;
+  if (c < 4) c = 4;
+  return c;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
+               uint64_t out_jac[12], hipStream_t st) {
+  for (int i = 0; i < 12; i++) out_jac[i] = 0;
+  if (n == 0) return ZK_OK;
+  if (n >= (1ull << 31)) return set_err(c, ZK_ERR_INVALID, "MSM size must be below 2^31");
+  const int cb = window_bits > 0 ? window_bits : default_window_bits(n);
+  if (cb < 2 || cb > 24) return set_err(c, ZK_ERR_INVALID, "window_bits must be in [2, 24]");
+  const uint32_t W = (255 + cb - 1) / cb;
+  const uint32_t B = (1u << (cb - 1)) + 1;
+  const uint32_t nkeys = W * B;
+  const uint32_t L = c->msm_segment;
+  uint32_t K = c->msm_reduce_chunk;
+  if (K > B - 1) K = B - 1;
+  const uint32_t cpw = (B - 1 + K - 1) / K;
+  const uint64_t pairs = n * W;
+  if (pairs >= (1ull << 32)) return set_err(c, ZK_ERR_INVALID, "n * window count must be below 2^32");
+  const uint64_t Smax = (pairs + L - 1) / L;
+  const uint32_t nchunks = (nkeys + SCAN_CHUNK - 1) / SCAN_CHUNK;
+  if (nchunks > SCAN_CHUNK) return set_err(c, ZK_ERR_INVALID, "too many buckets for the scan");
+
+  // scratch layout
+  size_t off = 0;
+  const size_t o_digits = off; off = align256(off + pairs * 4);
+  const size_t o_rowptr = off; off = align256(off + (size_t)(nkeys + 1) * 4);
+  const size_t o_cursor = off; off = align256(off + (size_t)(nkeys + 1) * 4);
+  const size_t o_sums = off; off = align256(off + (size_t)nchunks * 4);
+  const size_t o_sorted = off; off = align256(off + pairs * 4);
+  const size_t o_buckets = off; off = align256(off + (size_t)nkeys * sizeof(xyzz));
+  const size_t o_hkey = off; off = align256(off + Smax * 4);
+  const size_t o_head = off; off = align256(off + Smax * sizeof(xyzz));
+  const size_t o_partial = off; off = align256(off + (size_t)W * cpw * sizeof(xyzz));
+  const size_t o_wsum = off; off = align256(off + (size_t)W * sizeof(xyzz));
+  void* base;
+  int rc = ensure_scratch(c, 2, off, &base);
+  if (rc) return rc;
+  char* b8 = (char*)base;
+  uint32_t* digits = (uint32_t*)(b8 + o_digits);
+  uint32_t* row_ptr = (uint32_t*)(b8 + o_rowptr);
+  uint32_t* cursor = (uint32_t*)(b8 + o_cursor);
+  uint32_t* sums = (uint32_t*)(b8 + o_sums);
+  uint32_t* sorted = (uint32_t*)(b8 + o_sorted);
+  xyzz* buckets = (xyzz*)(b8 + o_buckets);
+  uint32_t* hkey = (uint32_t*)(b8 + o_hkey);
+  xyzz* head = (xyzz*)(b8 + o_head);
+  xyzz* partial = (xyzz*)(b8 + o_partial);
+  xyzz* wsum = (xyzz*)(b8 + o_wsum);
+
+  hipEvent_t ev;
+  // counts live in cursor[] until the scan turns them into row pointers
+  phase_begin(c, "msm_digits", st, &ev);
+  ZK_HIP(c, hipMemsetAsync(cursor, 0, (size_t)(nkeys + 1) * 4, st));
+  msm_digits_kernel<<<grid_for(n, 256), 256, 0, st>>>(scalars, n, (uint32_t)cb, W, B, digits, cursor);
+  ZK_HIP(c, hipGetLastError());
+  phase_end(c, "msm_digits", st, ev);
+
+  phase_begin(c, "msm_sort", st, &ev);
+  scan_chunks_kernel<<<nchunks, SCAN_THREADS, 0, st>>>(cursor, row_ptr, sums, nkeys);
+  scan_sums_kernel<<<1, SCAN_THREADS, 0, st>>>(sums, nchunks, row_ptr + nkeys);
+  scan_add_kernel<<<nchunks, SCAN_THREADS, 0, st>>>(row_ptr, cursor, sums, nkeys);
+  msm_scatter_kernel<<<grid_for(pairs, 256), 256, 0, st>>>(digits, n, W, B, cursor, sorted);
+  ZK_HIP(c, hipGetLastError());
+  phase_end(c, "msm_sort", st, ev);
+
+  phase_begin(c, "msm_accumulate", st, &ev);
+  msm_accumulate_kernel<<<grid_for(Smax, 256), 256, 0, st>>>(bases, sorted, row_ptr, nkeys, L, buckets,
+                                                            hkey, head);
+  msm_fixup_kernel<<<grid_for(Smax, 256), 256, 0, st>>>(row_ptr, nkeys, L, buckets, hkey, head);
+  ZK_HIP(c, hipGetLastError());
+  phase_end(c, "msm_accumulate", st, ev);
+
+  phase_begin(c, "msm_reduce", st, &ev);
+  msm_reduce_chunks_kernel<<<grid_for((uint64_t)W * cpw, 128), 128, 0, st>>>(buckets, row_ptr, W, B, K,
+                                                                           cpw, partial);
+  msm_reduce_windows_kernel<<<W, 128, 0, st>>>(partial, cpw, wsum);
+  ZK_HIP(c, hipGetLastError());
+  phase_end(c, "msm_reduce", st, ev);
+
+  // window combination on the host: acc = Sum_w 2^(c w) S_w (Horner)
+  std::vector<xyzz> hs(W);
+  ZK_HIP(c, hipMemcpyAsync(hs.data(), wsum, (size_t)W * sizeof(xyzz), hipMemcpyDeviceToHost, st));
+  ZK_HIP(c, hipStreamSynchronize(st));
+  xyzz acc = hs[W - 1];
+  for (int w = (int)W - 2; w >= 0; w--) {
+    for (int d = 0; d < cb; d++) acc = xyzz_dbl(acc);
+    acc = xyzz_add(acc, hs[w]);
+  }
+  if (!xyzz_is_inf(acc)) {
+    // Jacobian (X ZZ^2, Y ZZZ^2, ZZZ) represents (X/ZZ, Y/ZZZ)
+    fe zz2 = Fq::sqr(acc.zz), zzz2 = Fq::sqr(acc.zzz);
+    fe X = Fq::mul(acc.x, zz2), Y = Fq::mul(acc.y, zzz2), Z = acc.zzz;
+    for (int i = 0; i < 4; i++) {
+      out_jac[i] = (uint64_t)X.w[2 * i] | ((uint64_t)X.w[2 * i + 1] << 32);
+      out_jac[4 + i] = (uint64_t)Y.w[2 * i] | ((uint64_t)Y.w[2 * i + 1] << 32);
+      out_jac[8 + i] = (uint64_t)Z.w[2 * i] | ((uint64_t)Z.w[2 * i + 1] << 32);
+    }
+  }
+  return ZK_OK;
+}
+
+}  // namespace zk

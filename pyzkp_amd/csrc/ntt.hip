@@ -1,0 +1,336 @@
+// Fr NTT / iNTT / coset NTT on gfx950: self-sorting (Stockham) radix-2
+// stages, grouped into passes of k stages that run on an LDS tile.
+//
+// Semantics follow the reference's CPU NTT (crypto/poly/ntt.py:42-146) and
+// its Metal Stockham v2 (runtime/kernels/metal/kernels.py:383-498, 601-702,
+// runtime/metal/runtime.py:336-362): natural-order input and output,
+// evals[i] = sum_j a_j w^(ij); the inverse uses w^-1 and scales by n^-1;
+// the coset forms scale coefficient j by g^j (forward, before) or g^-j
+// (inverse, after).  Forward input shorter than n is zero-padded and longer
+// input is truncated (crypto/poly/ntt.py:103-110).
+//
+// Pass structure.  Global radix-2 Stockham stage s (sub-transform size 2^s)
+// maps input index [b | mid | low_s] to output [mid | b' | low_s]: it takes
+// the top index bit, butterflies it with twiddle w_(2^(s+1))^(low_s) and
+// parks it at bit s.  k consecutive stages starting at s0 therefore touch
+// disjoint groups of R = 2^k elements spaced n/R apart, i.e. one radix-R
+// Stockham pass.  A workgroup takes T adjacent groups (an R x T tile, T
+// consecutive "columns" j so global reads and writes are T*32-byte runs),
+// runs the k stages in LDS with the exact global twiddles (one Fr product
+// per butterfly, none extra for the pass split), and writes the tile back.
+// Coset / n^-1 scaling is fused into the first pass's loads and the last
+// pass's stores.
+#include "internal.hpp"
+
+namespace zk {
+
+static constexpr int NTT_TILE = 2048;   // elements per LDS tile (64 KiB)
+static constexpr int NTT_THREADS = 256;
+static constexpr int NTT_BFLY_PER_THREAD = NTT_TILE / 2 / NTT_THREADS;
+static constexpr int POW_LO_BITS = 10;  // two-level power tables: lo has 2^10 entries
+
+__device__ __forceinline__ fe pow_lookup(const fe* __restrict__ hi, const fe* __restrict__ lo,
+                                         uint64_t idx) {
+  return Fr::mul(load_fe(hi + (idx >> POW_LO_BITS)), load_fe(lo + (idx & ((1u << POW_LO_BITS) - 1))));
+}
+
+// scale modes
+enum { SCALE_NONE = 0, SCALE_CONST = 1, SCALE_TABLE = 2 };
+
+__global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
+    const fe* __restrict__ in, fe* __restrict__ out, const fe* __restrict__ tw, uint32_t log_n,
+    uint32_t s0, uint32_t k, uint32_t log_t, uint64_t in_len, int pre_mode,
+    const fe* __restrict__ pre_hi, const fe* __restrict__ pre_lo, int post_mode,
+    const fe* __restrict__ post_hi, const fe* __restrict__ post_lo) {
+  __shared__ fe lds[NTT_TILE];
+  const uint64_t n = 1ull << log_n;
+  const uint32_t R = 1u << k;
+  const uint32_t T = 1u << log_t;
+  const uint32_t RT = R << log_t;
+  const uint64_t cols = n >> k;  // n / R
+  const uint64_t boff = (uint64_t)blockIdx.y * n;
+  in += boff;
+  out += boff;
+  const uint64_t j0 = (uint64_t)blockIdx.x << log_t;
+  const uint64_t Ns = 1ull << s0;
+  const uint32_t tid = threadIdx.x;
+
+  // ---- load the R x T tile: element (r, c) <- in[j0 + c + r * n/R]
+  for (uint32_t e = tid; e < RT; e += NTT_THREADS) {
+    const uint32_t r = e >> log_t, c = e & (T - 1);
+    const uint64_t idx = j0 + c + (uint64_t)r * cols;
+    fe v;
+    if (idx < in_len) {
+      v = load_fe(in + idx);
+      if (pre_mode == SCALE_TABLE) v = Fr::mul(v, pow_lookup(pre_hi, pre_lo, idx));
+    } else {
+      v = Fr::zero();
+    }
+    lds[e] = v;
+  }
+  __syncthreads();
+
+  // ---- k radix-2 Stockham stages in LDS
+  const uint32_t halfRT = RT >> 1;
+  for (uint32_t t = 0; t < k; t++) {
+    const uint32_t s = s0 + t;
+    fe u[NTT_BFLY_PER_THREAD], v[NTT_BFLY_PER_THREAD];
+#pragma unroll
+    for (int i = 0; i < NTT_BFLY_PER_THREAD; i++) {
+      const uint32_t b = tid + i * NTT_THREADS;
+      if (b < halfRT) {
+        const uint32_t c = b & (T - 1), q = b >> log_t;
+        u[i] = lds[(q << log_t) + c];
+        v[i] = lds[((q + (R >> 1)) << log_t) + c];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NTT_BFLY_PER_THREAD; i++) {
+      const uint32_t b = tid + i * NTT_THREADS;
+      if (b < halfRT) {
+        const uint32_t c = b & (T - 1), q = b >> log_t;
+        const uint32_t d = q & ((1u << t) - 1);
+        fe vv = v[i];
+        if (s != 0) {
+          const uint64_t low = (j0 + c) & (Ns - 1);
+          const uint64_t e = (low + (uint64_t)d * Ns) << (log_n - 1 - s);
+          vv = Fr::mul(vv, load_fe(tw + e));
+        }
+        const uint32_t o = ((q - d) << 1) + d;
+        lds[(o << log_t) + c] = Fr::add(u[i], vv);
+        lds[((o + (1u << t)) << log_t) + c] = Fr::sub(u[i], vv);
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- store: element (sp, c) -> out[(j / Ns) * Ns * R + (j % Ns) + sp * Ns], j = j0 + c
+  const fe post_c = (post_mode == SCALE_CONST) ? load_fe(post_hi) : Fr::zero();
+  for (uint32_t e = tid; e < RT; e += NTT_THREADS) {
+    uint32_t sp, c;
+    if (s0 == 0) {  // Ns = 1: a column's R outputs are contiguous
+      c = e >> k;
+      sp = e & (R - 1);
+    } else {
+      sp = e >> log_t;
+      c = e & (T - 1);
+    }
+    const uint64_t j = j0 + c;
+    const uint64_t pos = ((j >> s0) << (s0 + k)) + (j & (Ns - 1)) + ((uint64_t)sp << s0);
+    fe x = lds[(sp << log_t) + c];
+    if (post_mode == SCALE_CONST) {
+      x = Fr::mul(x, post_c);
+    } else if (post_mode == SCALE_TABLE) {
+      x = Fr::mul(x, pow_lookup(post_hi, post_lo, pos));
+    }
+    store_fe(out + pos, x);
+  }
+}
+
+// twiddle table tw[e] = hi[e >> 10] * lo[e & 1023]
+__global__ __launch_bounds__(256) void pow_expand_kernel(const fe* __restrict__ hi,
+                                                         const fe* __restrict__ lo,
+                                                         fe* __restrict__ out, uint64_t count) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) store_fe(out + i, pow_lookup(hi, lo, i));
+}
+
+// n == 1: out[0] = (in_len ? in[0] : 0) * scale
+__global__ void ntt_size1_kernel(const fe* __restrict__ in, fe* __restrict__ out, uint64_t in_len,
+                                 uint32_t batch) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < batch) store_fe(out + b, in_len ? load_fe(in + b) : Fr::zero());
+}
+
+// ---------------------------------------------------------------- host side
+
+// hi[h] = scale * base^(1024 h), lo[i] = base^i   (all Montgomery)
+static void build_pow_tables(const zkh::Fe& base, const zkh::Fe& scale, uint64_t count,
+                             std::vector<zkh::Fe>& hi, std::vector<zkh::Fe>& lo) {
+  const uint64_t L = 1ull << POW_LO_BITS;
+  lo.resize(L);
+  zkh::Fe x = zkh::one(zkh::FR);
+  for (uint64_t i = 0; i < L; i++) {
+    lo[i] = x;
+    x = zkh::mul(zkh::FR, x, base);
+  }
+  const zkh::Fe step = x;  // base^1024
+  uint64_t nh = (count + L - 1) / L;
+  if (nh == 0) nh = 1;
+  hi.resize(nh);
+  zkh::Fe y = scale;
+  for (uint64_t h = 0; h < nh; h++) {
+    hi[h] = y;
+    y = zkh::mul(zkh::FR, y, step);
+  }
+}
+
+static int get_pow_tables(Ctx* c, uint32_t log_n, const zkh::Fe& base, const zkh::Fe& scale,
+                          const fe** hi, const fe** lo, hipStream_t st) {
+  std::array<uint64_t, 9> key;
+  key[0] = log_n;
+  for (int i = 0; i < 4; i++) key[1 + i] = base.v[i];
+  for (int i = 0; i < 4; i++) key[5 + i] = scale.v[i];
+  auto it = c->pow_tables.find(key);
+  if (it == c->pow_tables.end()) {
+    std::vector<zkh::Fe> h, l;
+    build_pow_tables(base, scale, 1ull << log_n, h, l);
+    std::pair<DevBuf, DevBuf> bufs;
+    int rc = upload_table(c, h, &bufs.first, st);
+    if (rc) return rc;
+    rc = upload_table(c, l, &bufs.second, st);
+    if (rc) return rc;
+    it = c->pow_tables.emplace(key, bufs).first;
+  }
+  *hi = (const fe*)it->second.first.ptr;
+  *lo = (const fe*)it->second.second.ptr;
+  return ZK_OK;
+}
+
+static int get_twiddles(Ctx* c, uint32_t log_n, const zkh::Fe& w, const fe** out, hipStream_t st) {
+  std::array<uint64_t, 5> key = {log_n, w.v[0], w.v[1], w.v[2], w.v[3]};
+  auto it = c->twiddles.find(key);
+  if (it == c->twiddles.end()) {
+    const uint64_t count = (1ull << log_n) / 2;
+    const fe *hi, *lo;
+    int rc = get_pow_tables(c, log_n, w, zkh::one(zkh::FR), &hi, &lo, st);
+    if (rc) return rc;
+    DevBuf b;
+    b.bytes = count * 32;
+    ZK_HIP(c, hipMalloc(&b.ptr, b.bytes));
+    pow_expand_kernel<<<grid_for(count, 256), 256, 0, st>>>(hi, lo, (fe*)b.ptr, count);
+    ZK_HIP(c, hipGetLastError());
+    it = c->twiddles.emplace(key, b).first;
+  }
+  *out = (const fe*)it->second.ptr;
+  return ZK_OK;
+}
+
+// omega_for_size (crypto/poly/ntt.py:13-24): ROOT_2_28^(2^(28 - log_n))
+static zkh::Fe standard_root(uint32_t log_n) {
+  // ROOT_2_28 = 5^((r-1)/2^28) (crypto/poly/ntt.py:8), canonical little-endian words
+  const zkh::Fe root_plain = {{0x9bd61b6e725b19f0ull, 0x402d111e41112ed4ull,
+                               0x00e0a7eb8ef62abcull, 0x2a3c09f0a58a7e85ull}};
+  zkh::Fe w = zkh::to_mont(zkh::FR, root_plain);
+  for (uint32_t i = log_n; i < 28; i++) w = zkh::mul(zkh::FR, w, w);
+  return w;
+}
+
+// choose stages per pass: as few passes as allow a column run of >= 4
+// elements (128 B) per global access, with near-equal pass sizes
+static void plan_passes(uint32_t log_n, int max_pass_bits, std::vector<uint32_t>& ks) {
+  ks.clear();
+  if (log_n == 0) return;
+  const uint32_t tile_bits = 11;  // log2(NTT_TILE)
+  uint32_t kmax = (max_pass_bits > 0) ? (uint32_t)max_pass_bits : 9;
+  if (kmax > tile_bits) kmax = tile_bits;
+  if (log_n <= tile_bits && (max_pass_bits <= 0 || log_n <= kmax)) {
+    ks.push_back(log_n);
+    return;
+  }
+  uint32_t p = (log_n + kmax - 1) / kmax;
+  uint32_t base = log_n / p, extra = log_n % p;
+  for (uint32_t i = 0; i < p; i++) ks.push_back(base + (i < extra ? 1 : 0));
+}
+
+int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint32_t batch,
+            bool inverse, const zkh::Fe* omega, const zkh::Fe* shift, hipStream_t st,
+            int max_pass_bits) {
+  if (log_n > 28) return set_err(c, ZK_ERR_INVALID, "domain too large for BN254 root of unity");
+  if (batch == 0) return ZK_OK;
+  const uint64_t n = 1ull << log_n;
+  if (batch > 1 && in_len != n)
+    return set_err(c, ZK_ERR_INVALID, "batched NTT needs in_len == n");
+  if (log_n == 0) {
+    // n^-1 = 1 and g^0 = 1: the transform of one element is the element
+    ntt_size1_kernel<<<grid_for(batch, 64), 64, 0, st>>>(in, out, in_len, batch);
+    ZK_HIP(c, hipGetLastError());
+    return ZK_OK;
+  }
+
+  zkh::Fe w = omega ? *omega : standard_root(log_n);
+  // w must be a primitive n-th root: w^(n/2) == -1
+  {
+    zkh::Fe h = zkh::pow_u64(zkh::FR, w, n / 2);
+    zkh::Fe m1 = zkh::neg(zkh::FR, zkh::one(zkh::FR));
+    if (!zkh::eq(h, m1)) return set_err(c, ZK_ERR_INVALID, "omega is not a primitive n-th root of unity");
+  }
+  if (inverse) w = zkh::inv(zkh::FR, w);
+
+  const fe* tw;
+  int rc = get_twiddles(c, log_n, w, &tw, st);
+  if (rc) return rc;
+
+  int pre_mode = 0, post_mode = 0;
+  const fe *pre_hi = nullptr, *pre_lo = nullptr, *post_hi = nullptr, *post_lo = nullptr;
+  if (!inverse) {
+    if (shift) {
+      pre_mode = 2;
+      rc = get_pow_tables(c, log_n, *shift, zkh::one(zkh::FR), &pre_hi, &pre_lo, st);
+      if (rc) return rc;
+    }
+  } else {
+    zkh::Fe ninv = zkh::inv(zkh::FR, zkh::from_u64(zkh::FR, n));
+    if (shift) {
+      zkh::Fe ginv = zkh::is_zero(*shift) ? zkh::zero() : zkh::inv(zkh::FR, *shift);
+      post_mode = 2;
+      rc = get_pow_tables(c, log_n, ginv, ninv, &post_hi, &post_lo, st);
+    } else {
+      post_mode = 1;
+      // constant n^-1: a one-entry "hi" table
+      rc = get_pow_tables(c, 0, zkh::zero(), ninv, &post_hi, &post_lo, st);
+    }
+    if (rc) return rc;
+  }
+
+  std::vector<uint32_t> ks;
+  plan_passes(log_n, max_pass_bits, ks);
+  const size_t P = ks.size();
+
+  // buffer sequence: in -> ... -> out, intermediates never alias their source
+  fe* tmp = nullptr;
+  if (P > 1) {
+    void* p;
+    rc = ensure_scratch(c, 0, (size_t)n * batch * 32, &p);
+    if (rc) return rc;
+    tmp = (fe*)p;
+  }
+  std::vector<const fe*> src(P);
+  std::vector<fe*> dst(P);
+  for (size_t i = 0; i < P; i++) {
+    // pass i writes out when (P - 1 - i) is even, tmp otherwise
+    dst[i] = (((P - 1 - i) & 1) == 0) ? out : tmp;
+  }
+  if (P > 1 && (const void*)in == (const void*)out && dst[0] == out) {
+    // in-place with an odd pass count: copy the input aside first
+    void* p2;
+    rc = ensure_scratch(c, 1, (size_t)n * batch * 32, &p2);
+    if (rc) return rc;
+    ZK_HIP(c, hipMemcpyAsync(p2, in, (size_t)(batch > 1 ? n * batch : (in_len < n ? in_len : n)) * 32,
+                             hipMemcpyDeviceToDevice, st));
+    in = (const fe*)p2;
+  }
+  uint32_t s0 = 0;
+  hipEvent_t ev;
+  phase_begin(c, "ntt", st, &ev);
+  for (size_t i = 0; i < P; i++) {
+    const uint32_t k = ks[i];
+    const uint64_t cols = n >> k;
+    uint32_t log_t = 11 - k;
+    while ((1ull << log_t) > cols) log_t--;
+    const uint64_t tiles = cols >> log_t;
+    const fe* s = (i == 0) ? in : dst[i - 1];
+    const uint64_t len = (i == 0) ? in_len : n;
+    dim3 grid((unsigned)tiles, batch);
+    ntt_pass_kernel<<<grid, NTT_THREADS, 0, st>>>(
+        s, dst[i], tw, log_n, s0, k, log_t, len, i == 0 ? pre_mode : 0, pre_hi, pre_lo,
+        i == P - 1 ? post_mode : 0, post_hi, post_lo);
+    ZK_HIP(c, hipGetLastError());
+    s0 += k;
+  }
+  phase_end(c, "ntt", st, ev);
+  return ZK_OK;
+}
+
+}  // namespace zk

@@ -1,0 +1,149 @@
+"""GPU parity of the NTT and MSM kernels against the oracle, through the
+C-ABI (ctypes Context)."""
+import random
+
+import pytest
+
+from oracle import curve as C
+from oracle import msm as M
+from oracle import ntt as N
+from oracle.field import R_MOD, Q_MOD
+
+from tests.gpu_util import download_fr, msm_affine, upload_bases, upload_fr_mont
+
+pytestmark = pytest.mark.gpu
+
+
+def _mont(x):
+    return x * (1 << 256) % R_MOD
+
+
+@pytest.mark.parametrize("log_n", list(range(0, 13)) + [14, 16])
+def test_ntt_forward_inverse(zctx, log_n):
+    n = 1 << log_n
+    rng = random.Random(log_n)
+    a = [rng.randrange(R_MOD) for _ in range(n)]
+    w = N.omega_for_size(n)
+    src = upload_fr_mont(zctx, a)
+    dst = zctx.alloc(32 * n)
+    zctx.ntt(src.ptr, n, dst.ptr, log_n, False)
+    assert download_fr(zctx, dst.ptr, n) == N.evals_from_coeffs_on_roots(a, n, w)
+    zctx.ntt(src.ptr, n, dst.ptr, log_n, True)
+    assert download_fr(zctx, dst.ptr, n) == N.coeffs_from_evals_on_roots(a, w)
+    for s in (5, 0):
+        zctx.ntt(src.ptr, n, dst.ptr, log_n, False, shift_mont=_mont(s))
+        assert download_fr(zctx, dst.ptr, n) == N.evals_from_coeffs_on_coset(a, n=n, omega=w, shift=s)
+        zctx.ntt(src.ptr, n, dst.ptr, log_n, True, shift_mont=_mont(s))
+        assert download_fr(zctx, dst.ptr, n) == N.coeffs_from_evals_on_coset(a, omega=w, shift=s)
+    src.free()
+    dst.free()
+
+
+@pytest.mark.parametrize("log_n,in_len", [(4, 5), (4, 40), (10, 1000), (10, 3000), (13, 1), (13, 0)])
+def test_ntt_pad_truncate(zctx, log_n, in_len):
+    n = 1 << log_n
+    rng = random.Random(in_len)
+    a = [rng.randrange(R_MOD) for _ in range(in_len)]
+    src = upload_fr_mont(zctx, a)
+    dst = zctx.alloc(32 * n)
+    zctx.ntt(src.ptr, in_len, dst.ptr, log_n, False)
+    assert download_fr(zctx, dst.ptr, n) == N.evals_from_coeffs_on_roots(a, n, N.omega_for_size(n))
+
+
+@pytest.mark.parametrize("log_n", [3, 11, 15])
+def test_ntt_in_place_and_pass_splits(zctx, log_n):
+    n = 1 << log_n
+    rng = random.Random(99)
+    a = [rng.randrange(R_MOD) for _ in range(n)]
+    w = N.omega_for_size(n)
+    expect = N.evals_from_coeffs_on_roots(a, n, w)
+    for bits in (0, 1, 2, 3, 5, 7, 11):
+        zctx.set_tuning("ntt_max_pass_bits", bits)
+        buf = upload_fr_mont(zctx, a)
+        zctx.ntt(buf.ptr, n, buf.ptr, log_n, False)
+        assert download_fr(zctx, buf.ptr, n) == expect, bits
+        buf.free()
+    zctx.set_tuning("ntt_max_pass_bits", 0)
+
+
+def test_ntt_batch(zctx):
+    log_n, batch = 9, 5
+    n = 1 << log_n
+    rng = random.Random(3)
+    rows = [[rng.randrange(R_MOD) for _ in range(n)] for _ in range(batch)]
+    src = upload_fr_mont(zctx, [x for r in rows for x in r])
+    dst = zctx.alloc(32 * n * batch)
+    zctx.ntt_batch(src.ptr, dst.ptr, log_n, batch, False, shift_mont=_mont(5))
+    out = download_fr(zctx, dst.ptr, n * batch)
+    w = N.omega_for_size(n)
+    for b in range(batch):
+        assert out[b * n:(b + 1) * n] == N.evals_from_coeffs_on_coset(rows[b], n=n, omega=w, shift=5)
+
+
+def test_ntt_errors(zctx):
+    buf = zctx.alloc(64)
+    with pytest.raises(ValueError):
+        zctx.ntt(buf.ptr, 2, buf.ptr, 29, False)
+    with pytest.raises(ValueError):
+        zctx.ntt(buf.ptr, 1, buf.ptr, 1, True)  # inverse needs in_len == n
+    with pytest.raises(ValueError):
+        zctx.ntt(buf.ptr, 2, buf.ptr, 1, False, omega_mont=_mont(1))  # not primitive
+
+
+def _bases(ks):
+    return [C.g1_mul_affine(C.G1_GENERATOR, k) for k in ks]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 31, 64, 100, 257])
+def test_msm_closed_form(zctx, n):
+    rng = random.Random(1000 + n)
+    ks = [rng.randrange(1, R_MOD) for _ in range(n)]
+    ss = [rng.randrange(R_MOD) for _ in range(n)]
+    bases = upload_bases(zctx, _bases(ks))
+    sc = upload_fr_mont(zctx, ss)
+    expect = M.msm_closed_form(ks, ss)
+    for c in (0, 2, 4, 7, 10, 13, 16):
+        assert msm_affine(zctx, bases, sc, n, c) == expect, c
+
+
+def test_msm_edge_cases(zctx):
+    G = C.G1_GENERATOR
+    P = C.g1_mul_affine(G, 77)
+    pts = [P, P, P, C.aff_neg(P), G, None, P]
+    ss = [1, 2, R_MOD - 1, 5, 0, 9, 0]
+    expect = C.g1_mul_affine(G, (77 * (1 + 2 + R_MOD - 1 - 5)) % R_MOD)
+    bases = upload_bases(zctx, pts)
+    sc = upload_fr_mont(zctx, ss)
+    for c in (2, 3, 8, 16):
+        assert msm_affine(zctx, bases, sc, len(pts), c) == expect
+    # all-zero scalars and cancellation to the identity
+    sc0 = upload_fr_mont(zctx, [0] * 7)
+    assert msm_affine(zctx, bases, sc0, 7, 5) is None
+    sc2 = upload_fr_mont(zctx, [1, 0, 0, 1, 0, 0, 0])
+    assert msm_affine(zctx, bases, sc2, 7, 5) is None
+
+
+def test_msm_same_scalar_everywhere(zctx):
+    # one bucket per window holds every point: segments span many threads
+    n = 3000
+    rng = random.Random(4)
+    ks = [rng.randrange(1, R_MOD) for _ in range(n)]
+    s = rng.randrange(R_MOD)
+    bases = upload_bases(zctx, _bases(ks))
+    sc = upload_fr_mont(zctx, [s] * n)
+    for seg in (1, 3, 64):
+        zctx.set_tuning("msm_segment", seg)
+        assert msm_affine(zctx, bases, sc, n, 10) == M.msm_closed_form(ks, [s] * n)
+    zctx.set_tuning("msm_segment", 64)
+
+
+def test_msm_golden_commitments(zctx):
+    import json, os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "plonk_pk_n16.json")) as f:
+        pk = json.load(f)
+    srs = [tuple(p) for p in pk["srs_g1"]]
+    bases = upload_bases(zctx, srs)
+    for name, coeffs in pk["coeffs"].items():
+        sc = upload_fr_mont(zctx, coeffs)
+        expect = tuple(pk["commits"][name])
+        assert msm_affine(zctx, bases, sc, len(coeffs)) == expect, name
