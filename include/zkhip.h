@@ -111,6 +111,16 @@ int zk_ntt_batch(zk_ctx* ctx, const void* in, void* out, uint32_t log_n, uint32_
 int zk_msm_g1(zk_ctx* ctx, const void* bases, const void* scalars, uint64_t n, int window_bits,
               uint64_t out_jac_mont[12], void* stream);
 
+/* Fixed-base scalar multiplication of one G1 point P by n scalars:
+ * out[i] = [scalars[i]] P as 64-byte affine Montgomery (x = y = 0 for the
+ * identity).  `table` holds T[j][d] = [d * 2^(8j)] P for j < 32, d < 256
+ * (32 * 256 affine Montgomery points, T[j][0] = identity).  Replaces the
+ * one-g1_mul-per-element loops of SRS / query-vector generation
+ * (crypto/kzg/cpu_ref.py:28-39, protocols/groth16/setup.py:63-96) and is
+ * the device form of crypto/msm/pippenger.py:252-272's fixed-base table. */
+int zk_g1_fixed_base(zk_ctx* ctx, const void* table, const void* scalars, uint64_t n, void* out,
+                     void* stream);
+
 /* Tuning knobs: "ntt_max_pass_bits" (stages per NTT pass, 1..11; 0 = auto),
  * "msm_segment" (pairs per accumulation thread), "msm_reduce_chunk"
  * (buckets per reduction thread).  No counterpart in the reference. */
@@ -123,6 +133,11 @@ int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value);
 int zk_profile_enable(zk_ctx* ctx, int on);
 int zk_profile_get(zk_ctx* ctx, const char* phase, double* total_ms, uint64_t* launches);
 int zk_profile_reset(zk_ctx* ctx);
+
+/* Measured issue rate of v_mad_u64_u32 on this device (lane multiply-adds
+ * per second): the VALU roofline the MSM / NTT kernels are reported
+ * against.  No counterpart in the reference. */
+int zk_probe_mad_rate(zk_ctx* ctx, double* mads_per_s);
 
 #ifdef __cplusplus
 }

@@ -48,7 +48,9 @@ SIGNATURES = {
     "zk_ntt": [_vp, _vp, ct.c_uint64, _vp, ct.c_uint32, ct.c_int, _u64p, _u64p, _vp],
     "zk_ntt_batch": [_vp, _vp, _vp, ct.c_uint32, ct.c_uint32, ct.c_int, _u64p, _u64p, _vp],
     "zk_msm_g1": [_vp, _vp, _vp, ct.c_uint64, ct.c_int, _u64p, _vp],
+    "zk_g1_fixed_base": [_vp, _vp, _vp, ct.c_uint64, _vp, _vp],
     "zk_set_tuning": [_vp, ct.c_char_p, ct.c_int64],
+    "zk_probe_mad_rate": [_vp, ct.POINTER(ct.c_double)],
     "zk_profile_enable": [_vp, ct.c_int],
     "zk_profile_get": [_vp, ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_uint64)],
     "zk_profile_reset": [_vp],
@@ -234,9 +236,17 @@ class Context:
         f = lambda a: w[a] | (w[a + 1] << 64) | (w[a + 2] << 128) | (w[a + 3] << 192)
         return f(0), f(4), f(8)
 
+    def g1_fixed_base(self, table: int, scalars: int, n: int, out: int, stream=None) -> None:
+        self._check(self.lib.zk_g1_fixed_base(self.handle, table, scalars, int(n), out, stream))
+
     # ---- tuning / profiling
     def set_tuning(self, key: str, value: int) -> None:
         self._check(self.lib.zk_set_tuning(self.handle, key.encode(), int(value)))
+
+    def probe_mad_rate(self) -> float:
+        r = ct.c_double(0.0)
+        self._check(self.lib.zk_probe_mad_rate(self.handle, ct.byref(r)))
+        return r.value
 
     def profile_enable(self, on: bool = True) -> None:
         self._check(self.lib.zk_profile_enable(self.handle, int(bool(on))))

@@ -307,6 +307,13 @@ int zk_msm_g1(zk_ctx* ctx, const void* bases, const void* scalars, uint64_t n, i
                     pick(ctx, stream));
 }
 
+int zk_g1_fixed_base(zk_ctx* ctx, const void* table, const void* scalars, uint64_t n, void* out,
+                     void* stream) {
+  CTX_CHECK(ctx);
+  if (n && (!table || !scalars || !out)) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  return g1_fixed_base_run(&ctx->c, (const fe*)table, (const fe*)scalars, n, (fe*)out, pick(ctx, stream));
+}
+
 int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value) {
   CTX_CHECK(ctx);
   if (!key) return set_err(&ctx->c, ZK_ERR_INVALID, "null key");
@@ -324,6 +331,12 @@ int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value) {
     return set_err(&ctx->c, ZK_ERR_INVALID, "unknown tuning key: " + k);
   }
   return ZK_OK;
+}
+
+int zk_probe_mad_rate(zk_ctx* ctx, double* mads_per_s) {
+  CTX_CHECK(ctx);
+  if (!mads_per_s) return set_err(&ctx->c, ZK_ERR_INVALID, "null output");
+  return probe_mad_rate(&ctx->c, mads_per_s);
 }
 
 int zk_profile_enable(zk_ctx* ctx, int on) {

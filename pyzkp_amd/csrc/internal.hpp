@@ -83,6 +83,12 @@ int elementwise_scale(Ctx* c, const fe* a, const uint64_t c_mont[4], fe* out, ui
 int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
                uint64_t out_jac[12], hipStream_t st);
 
+// fixed-base scalar multiplication --------------------------------------------
+int g1_fixed_base_run(Ctx* c, const fe* table, const fe* scalars, uint64_t n, fe* out, hipStream_t st);
+
+// measurement probe ---------------------------------------------------------
+int probe_mad_rate(Ctx* c, double* mads_per_s);
+
 inline unsigned grid_for(uint64_t n, unsigned block) {
   uint64_t g = (n + block - 1) / block;
   return (unsigned)(g == 0 ? 1 : g);

@@ -387,9 +387,13 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   phase_begin(c, "msm_accumulate", st, &ev);
   msm_accumulate_kernel<<<grid_for(Smax, 256), 256, 0, st>>>(bases, sorted, row_ptr, nkeys, L, buckets,
                                                             hkey, head);
-  msm_fixup_kernel<<<grid_for(Smax, 256), 256, 0, st>>>(row_ptr, nkeys, L, buckets, hkey, head);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_accumulate", st, ev);
+
+  phase_begin(c, "msm_fixup", st, &ev);
+  msm_fixup_kernel<<<grid_for(Smax, 256), 256, 0, st>>>(row_ptr, nkeys, L, buckets, hkey, head);
+  ZK_HIP(c, hipGetLastError());
+  phase_end(c, "msm_fixup", st, ev);
 
   phase_begin(c, "msm_reduce", st, &ev);
   msm_reduce_chunks_kernel<<<grid_for((uint64_t)W * cpw, 128), 128, 0, st>>>(buckets, row_ptr, W, B, K,

@@ -131,3 +131,19 @@ def g1_mul(p: G1Point, k: int) -> G1Point:
         if bit == "1":
             acc = g1_add(acc, p)
     return acc
+
+
+def fixed_base_table(p: G1Point = G1_GENERATOR) -> bytes:
+    """T[j][d] = [d * 2^(8j)] p for j < 32, d < 256, packed as 64-byte affine
+    Montgomery points (the table zk_g1_fixed_base expects)."""
+    rows = []
+    base = p
+    for _ in range(32):
+        row = [G1_ZERO]
+        acc = G1_ZERO
+        for _ in range(255):
+            acc = g1_add(acc, base)
+            row.append(acc)
+        rows.extend(row)
+        base = g1_add(acc, base)  # 256 * base
+    return affine_mont_bytes(rows)

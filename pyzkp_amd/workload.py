@@ -58,3 +58,65 @@ def fr_uniform(seed: int, n: int) -> np.ndarray:
 
 def uniform01(seed: int, n: int) -> np.ndarray:
     return (splitmix_block(seed, 0, n) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+
+
+# ---------------------------------------------------------------- MSM inputs
+def scalars_skewed(seed: int, n: int) -> np.ndarray:
+    """Bucket-imbalance stress: per i with u ~ U[0,1) (stream seed):
+    u < 0.10 -> 0, u < 0.30 -> uniform below 2^16, else uniform in Fr."""
+    u = uniform01(seed, n)
+    full = fr_uniform(seed ^ 0x5EED, n)
+    small = splitmix_block(seed ^ 0x5111, 0, n) & np.uint64(0xFFFF)
+    out = full.copy()
+    z = u < 0.10
+    s = (u >= 0.10) & (u < 0.30)
+    out[z] = 0
+    out[s] = 0
+    out[s, 0] = small[s]
+    return out
+
+
+def repeat_some(seed: int, ks: np.ndarray, frac: float = 0.05) -> np.ndarray:
+    """With probability frac, k_i := k_j for a uniformly random earlier j."""
+    n = ks.shape[0]
+    u = uniform01(seed, n)
+    pick = (splitmix_block(seed ^ 0xB00B, 0, n) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    out = ks.copy()
+    for i in np.nonzero(u < frac)[0]:
+        if i == 0:
+            continue
+        j = int(pick[i] * i)
+        out[i] = out[j]
+    return out
+
+
+_TABLE_CACHE: dict = {}
+
+
+def device_fr_mont(ctx, words: np.ndarray):
+    """(n, 4) canonical uint64 words -> device Montgomery Fr buffer."""
+    n = words.shape[0]
+    buf = ctx.alloc(max(32 * n, 32))
+    if n:
+        ctx.h2d(buf.ptr, np.ascontiguousarray(words, dtype="<u8").tobytes(), 32 * n)
+        ctx.fr_to_mont(buf.ptr, buf.ptr, n)
+        ctx.sync()
+    return buf
+
+
+def device_bases(ctx, ks_words: np.ndarray):
+    """Affine Montgomery bases B_i = [k_i] G1 built on the device."""
+    from .curve import fixed_base_table
+
+    n = ks_words.shape[0]
+    key = ctx.handle.value if hasattr(ctx.handle, "value") else id(ctx)
+    tab = _TABLE_CACHE.get(key)
+    if tab is None:
+        tab = ctx.upload(fixed_base_table())
+        _TABLE_CACHE[key] = tab
+    kbuf = device_fr_mont(ctx, ks_words)
+    out = ctx.alloc(max(64 * n, 64))
+    ctx.g1_fixed_base(tab.ptr, kbuf.ptr, n, out.ptr)
+    ctx.sync()
+    kbuf.free()
+    return out

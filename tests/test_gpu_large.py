@@ -1,0 +1,127 @@
+"""GPU parity at the benchmark sizes: NTT 2^20 bit-exact against the C
+oracle, 2^24 by round trip and Horner rows; MSM 2^12 (c = 10) against the C
+Pippenger and the closed form; 2^16 / 2^22 uniform and skewed by the closed
+form [sum s_i k_i] G1 (bases B_i = [k_i] G1)."""
+import numpy as np
+import pytest
+
+from oracle import cref
+from oracle import curve as C
+from oracle import msm as M
+from oracle import ntt as N
+from oracle.field import R_MOD
+from pyzkp_amd import workload
+from pyzkp_amd.curve import from_jacobian_mont
+from pyzkp_amd.field import FR_GENERATOR, fr_to_mont, words_to_ints
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev_words(ctx, ptr, n):
+    """Device Montgomery Fr -> canonical (n, 4) words (converted on device)."""
+    tmp = ctx.alloc(32 * n)
+    ctx.fr_from_mont(ptr, tmp.ptr, n)
+    w = np.frombuffer(ctx.download(tmp.ptr, 32 * n), dtype="<u8").reshape(n, 4).copy()
+    tmp.free()
+    return w
+
+
+@pytest.mark.parametrize("variant", ["fwd", "inv", "coset_fwd", "coset_inv"])
+def test_ntt_2_20_bit_exact(zctx, variant):
+    log_n = 20
+    n = 1 << log_n
+    a = workload.fr_uniform(0x2001, n)
+    src = workload.device_fr_mont(zctx, a)
+    dst = zctx.alloc(32 * n)
+    inv = variant.endswith("inv")
+    shift = FR_GENERATOR if variant.startswith("coset") else None
+    zctx.ntt(src.ptr, n, dst.ptr, log_n, inv, shift_mont=fr_to_mont(shift) if shift else None)
+    got = _dev_words(zctx, dst.ptr, n).reshape(-1)
+    expect = cref.ntt_words(a.reshape(-1), n, log_n, N.omega_for_size(n), inv, shift, threads=0)
+    assert np.array_equal(got, expect)
+    src.free()
+    dst.free()
+
+
+def test_ntt_2_24_roundtrip_and_rows(zctx):
+    log_n = 24
+    n = 1 << log_n
+    a = workload.fr_uniform(0x2002, n)
+    src = workload.device_fr_mont(zctx, a)
+    ev = zctx.alloc(32 * n)
+    back = zctx.alloc(32 * n)
+    zctx.ntt(src.ptr, n, ev.ptr, log_n, False)
+    got = _dev_words(zctx, ev.ptr, n)
+    w = N.omega_for_size(n)
+    flat = a.reshape(-1)
+    for i in (0, 1, 2, 12345, n // 2, n - 1):
+        assert words_to_ints(got[i:i + 1])[0] == cref.poly_eval_words(flat, n, pow(w, i, R_MOD)), i
+    zctx.ntt(ev.ptr, n, back.ptr, log_n, True)
+    assert np.array_equal(_dev_words(zctx, back.ptr, n), a)
+    # coset round trip in place
+    sh = fr_to_mont(FR_GENERATOR)
+    zctx.ntt(src.ptr, n, ev.ptr, log_n, False, shift_mont=sh)
+    got = _dev_words(zctx, ev.ptr, n)
+    g = FR_GENERATOR
+    for i in (0, 7, n - 3):
+        assert words_to_ints(got[i:i + 1])[0] == cref.poly_eval_words(flat, n, g * pow(w, i, R_MOD) % R_MOD)
+    zctx.ntt(ev.ptr, n, ev.ptr, log_n, True, shift_mont=sh)
+    assert np.array_equal(_dev_words(zctx, ev.ptr, n), a)
+    for b in (src, ev, back):
+        b.free()
+
+
+def test_fixed_base_kernel(zctx):
+    ks = workload.fr_uniform(0x1002, 300)
+    ks[5] = 0
+    ks[6] = 0
+    ks[6, 0] = 1
+    bases = workload.device_bases(zctx, ks)
+    tmp = zctx.alloc(64 * 300)
+    zctx.fq_from_mont(bases.ptr, tmp.ptr, 600)
+    xy = words_to_ints(np.frombuffer(zctx.download(tmp.ptr, 64 * 300), dtype="<u8").reshape(-1, 4))
+    for i, k in enumerate(words_to_ints(ks)):
+        p = None if xy[2 * i] == 0 and xy[2 * i + 1] == 0 else (xy[2 * i], xy[2 * i + 1])
+        assert p == C.g1_mul_affine(C.G1_GENERATOR, k), i
+
+
+def _msm(ctx, bases, scal, n, c=0):
+    p = from_jacobian_mont(*ctx.msm_g1(bases.ptr, scal.ptr, n, c))
+    return None if p[2] == 0 else (p[0], p[1])
+
+
+def _closed(ks, ss):
+    return M.msm_closed_form(words_to_ints(ks), words_to_ints(ss))
+
+
+def test_msm_2_12_config(zctx):
+    n = 1 << 12
+    ks = workload.fr_uniform(0x1002, n)
+    ss = workload.fr_uniform(0x1001, n)
+    bases = workload.device_bases(zctx, ks)
+    scal = workload.device_fr_mont(zctx, ss)
+    got = _msm(zctx, bases, scal, n, 10)
+    assert got == _closed(ks, ss)
+    tmp = zctx.alloc(64 * n)
+    zctx.fq_from_mont(bases.ptr, tmp.ptr, 2 * n)
+    bw = np.frombuffer(zctx.download(tmp.ptr, 64 * n), dtype="<u8")
+    assert cref.msm_pippenger_words(bw, ss.reshape(-1), n, M.default_window_bits(n), threads=0) == got
+
+
+@pytest.mark.parametrize("log_n,skewed", [(16, False), (16, True), (22, False), (22, True)])
+def test_msm_closed_form_large(zctx, log_n, skewed):
+    n = 1 << log_n
+    ks = workload.fr_uniform(0x3002, n)
+    if skewed:
+        ss = workload.scalars_skewed(0x3003, n)
+        ks = workload.repeat_some(0x3004, ks, 0.05)
+    else:
+        ss = workload.fr_uniform(0x3001, n)
+    bases = workload.device_bases(zctx, ks)
+    scal = workload.device_fr_mont(zctx, ss)
+    expect = _closed(ks, ss)
+    cs = (13, 16, 18) if log_n == 22 else (8, 13, 16)
+    for c in cs:
+        assert _msm(zctx, bases, scal, n, c) == expect, c
+    bases.free()
+    scal.free()
