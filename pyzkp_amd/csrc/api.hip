@@ -325,7 +325,7 @@ int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value) {
     if (value < 1 || value > 4096) return set_err(&ctx->c, ZK_ERR_INVALID, "msm_segment must be in [1, 4096]");
     ctx->c.msm_segment = (uint32_t)value;
   } else if (k == "msm_reduce_chunk") {
-    if (value < 1 || value > 65536) return set_err(&ctx->c, ZK_ERR_INVALID, "msm_reduce_chunk must be in [1, 65536]");
+    if (value < 0 || value > 65536) return set_err(&ctx->c, ZK_ERR_INVALID, "msm_reduce_chunk must be in [0, 65536]");
     ctx->c.msm_reduce_chunk = (uint32_t)value;
   } else {
     return set_err(&ctx->c, ZK_ERR_INVALID, "unknown tuning key: " + k);

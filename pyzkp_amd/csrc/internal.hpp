@@ -46,7 +46,7 @@ struct Ctx {
   // tuning knobs (set through zk_set_tuning)
   int ntt_max_pass_bits = 0;      // 0: planner default
   uint32_t msm_segment = 64;      // sorted (bucket, point) pairs per accumulation thread
-  uint32_t msm_reduce_chunk = 32; // buckets per reduction thread
+  uint32_t msm_reduce_chunk = 0;  // buckets per reduction thread (0: sized to fill the device)
   bool profiling = false;
   std::map<std::string, PhaseTimer> phases;
 };

@@ -48,13 +48,11 @@ __device__ __forceinline__ void store_xyzz(xyzz* p, const xyzz& v) {
   store_fe(f + 3, v.zzz);
 }
 
-// 1. scalar (Montgomery) -> canonical -> W signed c-bit digits; histogram of
-//    the nonzero digit magnitudes per (window, bucket).
+// 1. scalar (Montgomery) -> canonical -> W signed c-bit digits
 //    digits[w * n + i] = |d| | (d < 0 ? SIGN : 0)
 __global__ __launch_bounds__(256) void msm_digits_kernel(const fe* __restrict__ scalars, uint64_t n,
-                                                         uint32_t c, uint32_t W, uint32_t B,
-                                                         uint32_t* __restrict__ digits,
-                                                         uint32_t* __restrict__ counts) {
+                                                         uint32_t c, uint32_t W,
+                                                         uint32_t* __restrict__ digits) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const fe s = Fr::from_mont(load_fe(scalars + i));
@@ -87,8 +85,65 @@ __global__ __launch_bounds__(256) void msm_digits_kernel(const fe* __restrict__ 
       out = val;
     }
     digits[(uint64_t)w * n + i] = out;
-    if (out & ~SIGN) atomicAdd(&counts[w * B + (out & ~SIGN)], 1u);
   }
+}
+
+// 2. counting sort of the (window, |d|) pairs, one window per grid row.
+//    A tile of SORT_TILE digits of one window is histogrammed in LDS with
+//    16-bit counters (two per word; a tile has at most 2^15 entries so a
+//    counter cannot overflow).  Bins are |d| - 1 (zero digits are dropped);
+//    a window with more than SORT_RANGE bins is covered by several bin
+//    ranges (grid z), each re-reading the tile.
+//    tile_hist[(w * ntiles + tile) * nbins + bin] = count
+static constexpr uint32_t SORT_TILE = 1u << 15;
+static constexpr uint32_t SORT_RANGE = 1u << 15;  // bins per LDS pass (64 KiB)
+static constexpr uint32_t SORT_THREADS = 1024;
+
+__global__ __launch_bounds__(SORT_THREADS) void msm_tile_count_kernel(
+    const uint32_t* __restrict__ digits, uint64_t n, uint32_t nbins, uint32_t ntiles,
+    uint32_t* __restrict__ tile_hist) {
+  __shared__ uint32_t cnt[SORT_RANGE / 2];
+  const uint32_t tile = blockIdx.x, w = blockIdx.y;
+  const uint32_t lo = blockIdx.z * SORT_RANGE;
+  const uint32_t hi = (lo + SORT_RANGE < nbins) ? lo + SORT_RANGE : nbins;
+  for (uint32_t k = threadIdx.x; k < SORT_RANGE / 2; k += SORT_THREADS) cnt[k] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)w * n + (uint64_t)tile * SORT_TILE;
+  const uint64_t end = (uint64_t)w * n + (((uint64_t)tile + 1) * SORT_TILE < n ? ((uint64_t)tile + 1) * SORT_TILE : n);
+  for (uint64_t p = base + threadIdx.x; p < end; p += SORT_THREADS) {
+    const uint32_t mag = digits[p] & ~SIGN;
+    if (mag == 0) continue;
+    const uint32_t bin = mag - 1;
+    if (bin < lo || bin >= hi) continue;
+    const uint32_t b = bin - lo;
+    atomicAdd(&cnt[b >> 1], 1u << ((b & 1) * 16));
+  }
+  __syncthreads();
+  uint32_t* out = tile_hist + ((uint64_t)w * ntiles + tile) * nbins;
+  for (uint32_t b = lo + threadIdx.x; b < hi; b += SORT_THREADS) {
+    const uint32_t k = b - lo;
+    out[b] = (cnt[k >> 1] >> ((k & 1) * 16)) & 0xffffu;
+  }
+}
+
+// exclusive prefix of each (window, bin) column over the tiles, in place;
+// the column total becomes counts[w * B + bin + 1] (bucket |d| = bin + 1)
+__global__ __launch_bounds__(256) void msm_tile_prefix_kernel(uint32_t* __restrict__ tile_hist,
+                                                              uint32_t W, uint32_t nbins,
+                                                              uint32_t ntiles, uint32_t B,
+                                                              uint32_t* __restrict__ counts) {
+  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (uint64_t)W * nbins) return;
+  const uint32_t w = (uint32_t)(gid / nbins), bin = (uint32_t)(gid - (uint64_t)w * nbins);
+  uint32_t run = 0;
+  uint32_t* col = tile_hist + (uint64_t)w * ntiles * nbins + bin;
+  for (uint32_t t = 0; t < ntiles; t++) {
+    const uint32_t v = col[(uint64_t)t * nbins];
+    col[(uint64_t)t * nbins] = run;
+    run += v;
+  }
+  counts[w * B + bin + 1] = run;
+  if (bin == 0) counts[w * B] = 0;
 }
 
 // 2. exclusive scan of the bucket counts -> row pointers (3 small kernels)
@@ -160,35 +215,43 @@ __global__ __launch_bounds__(SCAN_THREADS) void scan_sums_kernel(uint32_t* __res
 }
 
 __global__ __launch_bounds__(SCAN_THREADS) void scan_add_kernel(uint32_t* __restrict__ out,
-                                                                uint32_t* __restrict__ cursor,
                                                                 const uint32_t* __restrict__ sums,
                                                                 uint32_t len) {
   const uint32_t base = blockIdx.x * SCAN_CHUNK;
   const uint32_t add = sums[blockIdx.x];
   for (uint32_t k = threadIdx.x; k < SCAN_CHUNK; k += SCAN_THREADS) {
     const uint32_t idx = base + k;
-    if (idx < len) {
-      const uint32_t v = out[idx] + add;
-      out[idx] = v;
-      cursor[idx] = v;
-    }
+    if (idx < len) out[idx] += add;
   }
 }
 
-// 3. scatter: sorted[pos] = point index | sign, grouped by (window, bucket)
-__global__ __launch_bounds__(256) void msm_scatter_kernel(const uint32_t* __restrict__ digits,
-                                                          uint64_t n, uint32_t W, uint32_t B,
-                                                          uint32_t* __restrict__ cursor,
-                                                          uint32_t* __restrict__ sorted) {
-  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= n * W) return;
-  const uint32_t d = digits[gid];
-  const uint32_t mag = d & ~SIGN;
-  if (!mag) return;
-  const uint32_t w = (uint32_t)(gid / n);
-  const uint32_t i = (uint32_t)(gid - (uint64_t)w * n);
-  const uint32_t pos = atomicAdd(&cursor[w * B + mag], 1u);
-  sorted[pos] = i | (d & SIGN);
+// 3. scatter: sorted[row_ptr[w B + |d|] + tile prefix + rank in tile] =
+//    point index | sign; ranks come from 16-bit LDS counters as in the count pass
+__global__ __launch_bounds__(SORT_THREADS) void msm_tile_scatter_kernel(
+    const uint32_t* __restrict__ digits, uint64_t n, uint32_t nbins, uint32_t ntiles, uint32_t B,
+    const uint32_t* __restrict__ tile_hist, const uint32_t* __restrict__ row_ptr,
+    uint32_t* __restrict__ sorted) {
+  __shared__ uint32_t cnt[SORT_RANGE / 2];
+  const uint32_t tile = blockIdx.x, w = blockIdx.y;
+  const uint32_t lo = blockIdx.z * SORT_RANGE;
+  const uint32_t hi = (lo + SORT_RANGE < nbins) ? lo + SORT_RANGE : nbins;
+  for (uint32_t k = threadIdx.x; k < SORT_RANGE / 2; k += SORT_THREADS) cnt[k] = 0;
+  __syncthreads();
+  const uint32_t* pre = tile_hist + ((uint64_t)w * ntiles + tile) * nbins;
+  const uint32_t* rp = row_ptr + (uint64_t)w * B + 1;
+  const uint64_t i0 = (uint64_t)tile * SORT_TILE;
+  const uint64_t i1 = (i0 + SORT_TILE < n) ? i0 + SORT_TILE : n;
+  for (uint64_t i = i0 + threadIdx.x; i < i1; i += SORT_THREADS) {
+    const uint32_t d = digits[(uint64_t)w * n + i];
+    const uint32_t mag = d & ~SIGN;
+    if (mag == 0) continue;
+    const uint32_t bin = mag - 1;
+    if (bin < lo || bin >= hi) continue;
+    const uint32_t b = bin - lo;
+    const uint32_t sh = (b & 1) * 16;
+    const uint32_t rank = (atomicAdd(&cnt[b >> 1], 1u << sh) >> sh) & 0xffffu;
+    sorted[rp[bin] + pre[bin] + rank] = (uint32_t)i | (d & SIGN);
+  }
 }
 
 // 4. bucket accumulation over fixed-length segments of the sorted list.
@@ -245,23 +308,60 @@ __global__ __launch_bounds__(256) void msm_accumulate_kernel(
   head_key[s] = hk;
 }
 
-// 5. fix-up: the first segment holding a continuation of bucket k adds all
-//    consecutive head partials of k into buckets[k] (owner already wrote it)
-__global__ __launch_bounds__(256) void msm_fixup_kernel(const uint32_t* __restrict__ row_ptr,
-                                                        uint32_t nkeys, uint32_t L,
-                                                        xyzz* __restrict__ buckets,
-                                                        const uint32_t* __restrict__ head_key,
-                                                        const xyzz* __restrict__ head) {
+// 5. fix-up of buckets that span several segments.  The head partials of
+//    bucket k sit in consecutive segments first..first+len-1 with
+//    first = row_ptr[k] / L + 1.  Round r folds FIXUP_FAN partials spaced
+//    FIXUP_FAN^r apart into the one at offset 0 mod FIXUP_FAN^(r+1); after
+//    enough rounds the run's sum is at `first` and is added to buckets[k]
+//    (which the owning segment wrote).  Depth log_FAN(len) instead of len.
+static constexpr uint32_t FIXUP_FAN = 32;
+
+__device__ __forceinline__ bool head_run(const uint32_t* __restrict__ row_ptr, uint32_t k, uint32_t L,
+                                         uint64_t s, uint64_t& first, uint64_t& len) {
+  const uint64_t a = row_ptr[k], b = row_ptr[k + 1];
+  first = a / L + 1;
+  len = (b - 1) / L - a / L;
+  return s >= first && s < first + len;
+}
+
+__global__ __launch_bounds__(256) void msm_fixup_round_kernel(const uint32_t* __restrict__ row_ptr,
+                                                              uint32_t nkeys, uint32_t L,
+                                                              uint64_t stride,
+                                                              const uint32_t* __restrict__ head_key,
+                                                              xyzz* __restrict__ head) {
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t M = row_ptr[nkeys];
   const uint64_t S = (M + L - 1) / L;
   if (s >= S) return;
   const uint32_t k = head_key[s];
   if (k == NO_KEY) return;
-  if (s > 0 && head_key[s - 1] == k) return;
-  xyzz acc = xyzz_add(load_xyzz(buckets + k), load_xyzz(head + s));
-  for (uint64_t s2 = s + 1; s2 < S && head_key[s2] == k; s2++) acc = xyzz_add(acc, load_xyzz(head + s2));
-  store_xyzz(buckets + k, acc);
+  uint64_t first, len;
+  if (!head_run(row_ptr, k, L, s, first, len)) return;
+  const uint64_t off = s - first;
+  if (off % (stride * FIXUP_FAN) != 0 || off + stride >= len) return;
+  xyzz acc = load_xyzz(head + s);
+  for (uint32_t j = 1; j < FIXUP_FAN; j++) {
+    const uint64_t t = off + (uint64_t)j * stride;
+    if (t >= len) break;
+    acc = xyzz_add(acc, load_xyzz(head + first + t));
+  }
+  store_xyzz(head + s, acc);
+}
+
+__global__ __launch_bounds__(256) void msm_fixup_final_kernel(const uint32_t* __restrict__ row_ptr,
+                                                              uint32_t nkeys, uint32_t L,
+                                                              xyzz* __restrict__ buckets,
+                                                              const uint32_t* __restrict__ head_key,
+                                                              const xyzz* __restrict__ head) {
+  const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t M = row_ptr[nkeys];
+  const uint64_t S = (M + L - 1) / L;
+  if (s >= S) return;
+  const uint32_t k = head_key[s];
+  if (k == NO_KEY) return;
+  uint64_t first, len;
+  if (!head_run(row_ptr, k, L, s, first, len) || s != first) return;
+  store_xyzz(buckets + k, xyzz_add(load_xyzz(buckets + k), load_xyzz(head + s)));
 }
 
 // 6. bucket reduction, chunked: chunk t of window w covers buckets
@@ -333,6 +433,11 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   const uint32_t nkeys = W * B;
   const uint32_t L = c->msm_segment;
   uint32_t K = c->msm_reduce_chunk;
+  if (K == 0) {
+    // enough reduction threads to fill the device (~64K), at least 2 buckets each
+    const uint64_t per = ((uint64_t)W * (B - 1)) / 65536;
+    K = per < 2 ? 2 : (uint32_t)per;
+  }
   if (K > B - 1) K = B - 1;
   const uint32_t cpw = (B - 1 + K - 1) / K;
   const uint64_t pairs = n * W;
@@ -340,12 +445,16 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   const uint64_t Smax = (pairs + L - 1) / L;
   const uint32_t nchunks = (nkeys + SCAN_CHUNK - 1) / SCAN_CHUNK;
   if (nchunks > SCAN_CHUNK) return set_err(c, ZK_ERR_INVALID, "too many buckets for the scan");
+  const uint32_t nbins = B - 1;  // |d| in 1..2^(c-1)
+  const uint32_t ntiles = (uint32_t)((n + SORT_TILE - 1) / SORT_TILE);
+  const uint32_t nranges = (nbins + SORT_RANGE - 1) / SORT_RANGE;
 
   // scratch layout
   size_t off = 0;
   const size_t o_digits = off; off = align256(off + pairs * 4);
+  const size_t o_hist = off; off = align256(off + (size_t)W * ntiles * nbins * 4);
+  const size_t o_counts = off; off = align256(off + (size_t)(nkeys + 1) * 4);
   const size_t o_rowptr = off; off = align256(off + (size_t)(nkeys + 1) * 4);
-  const size_t o_cursor = off; off = align256(off + (size_t)(nkeys + 1) * 4);
   const size_t o_sums = off; off = align256(off + (size_t)nchunks * 4);
   const size_t o_sorted = off; off = align256(off + pairs * 4);
   const size_t o_buckets = off; off = align256(off + (size_t)nkeys * sizeof(xyzz));
@@ -358,8 +467,9 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   if (rc) return rc;
   char* b8 = (char*)base;
   uint32_t* digits = (uint32_t*)(b8 + o_digits);
+  uint32_t* hist = (uint32_t*)(b8 + o_hist);
+  uint32_t* counts = (uint32_t*)(b8 + o_counts);
   uint32_t* row_ptr = (uint32_t*)(b8 + o_rowptr);
-  uint32_t* cursor = (uint32_t*)(b8 + o_cursor);
   uint32_t* sums = (uint32_t*)(b8 + o_sums);
   uint32_t* sorted = (uint32_t*)(b8 + o_sorted);
   xyzz* buckets = (xyzz*)(b8 + o_buckets);
@@ -369,18 +479,22 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   xyzz* wsum = (xyzz*)(b8 + o_wsum);
 
   hipEvent_t ev;
-  // counts live in cursor[] until the scan turns them into row pointers
   phase_begin(c, "msm_digits", st, &ev);
-  ZK_HIP(c, hipMemsetAsync(cursor, 0, (size_t)(nkeys + 1) * 4, st));
-  msm_digits_kernel<<<grid_for(n, 256), 256, 0, st>>>(scalars, n, (uint32_t)cb, W, B, digits, cursor);
+  msm_digits_kernel<<<grid_for(n, 256), 256, 0, st>>>(scalars, n, (uint32_t)cb, W, digits);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_digits", st, ev);
 
   phase_begin(c, "msm_sort", st, &ev);
-  scan_chunks_kernel<<<nchunks, SCAN_THREADS, 0, st>>>(cursor, row_ptr, sums, nkeys);
-  scan_sums_kernel<<<1, SCAN_THREADS, 0, st>>>(sums, nchunks, row_ptr + nkeys);
-  scan_add_kernel<<<nchunks, SCAN_THREADS, 0, st>>>(row_ptr, cursor, sums, nkeys);
-  msm_scatter_kernel<<<grid_for(pairs, 256), 256, 0, st>>>(digits, n, W, B, cursor, sorted);
+  {
+    dim3 g(ntiles, W, nranges);
+    msm_tile_count_kernel<<<g, SORT_THREADS, 0, st>>>(digits, n, nbins, ntiles, hist);
+    msm_tile_prefix_kernel<<<grid_for((uint64_t)W * nbins, 256), 256, 0, st>>>(hist, W, nbins, ntiles, B,
+                                                                               counts);
+    scan_chunks_kernel<<<nchunks, SCAN_THREADS, 0, st>>>(counts, row_ptr, sums, nkeys);
+    scan_sums_kernel<<<1, SCAN_THREADS, 0, st>>>(sums, nchunks, row_ptr + nkeys);
+    scan_add_kernel<<<nchunks, SCAN_THREADS, 0, st>>>(row_ptr, sums, nkeys);
+    msm_tile_scatter_kernel<<<g, SORT_THREADS, 0, st>>>(digits, n, nbins, ntiles, B, hist, row_ptr, sorted);
+  }
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_sort", st, ev);
 
@@ -391,7 +505,9 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   phase_end(c, "msm_accumulate", st, ev);
 
   phase_begin(c, "msm_fixup", st, &ev);
-  msm_fixup_kernel<<<grid_for(Smax, 256), 256, 0, st>>>(row_ptr, nkeys, L, buckets, hkey, head);
+  for (uint64_t stride = 1; stride < Smax; stride *= FIXUP_FAN)
+    msm_fixup_round_kernel<<<grid_for(Smax, 256), 256, 0, st>>>(row_ptr, nkeys, L, stride, hkey, head);
+  msm_fixup_final_kernel<<<grid_for(Smax, 256), 256, 0, st>>>(row_ptr, nkeys, L, buckets, hkey, head);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_fixup", st, ev);
 
