@@ -1,0 +1,167 @@
+// BN254 G1 XYZZ arithmetic on the 29-bit-limb field (field29.hpp), used by
+// the MSM accumulation, fix-up and reduction kernels.  Same formulas and
+// exceptional-case handling as curve.hpp (add-2008-s, madd-2008-s,
+// dbl-2008-s, mdbl-2008-s for a = 0); coordinates are R = 2^261 Montgomery
+// values in [0, 2p); the identity is ZZ == 0 (mod p); an affine base with
+// x = y = 0 is the point at infinity.
+#pragma once
+#include "curve.hpp"
+#include "field29.hpp"
+
+namespace zk {
+
+struct aff9 {
+  f9 x, y;
+};
+
+struct xyzz9 {
+  f9 x, y, zz, zzz;
+};
+
+ZK_HD bool aff9_is_inf(const aff9& p) {
+  u32 z = 0;
+#pragma unroll
+  for (int i = 0; i < 9; i++) z |= p.x.l[i] | p.y.l[i];
+  return z == 0;
+}
+
+ZK_HD bool xyzz9_is_inf(const xyzz9& p) { return Fq9::is_zero(p.zz); }
+
+ZK_HD xyzz9 xyzz9_inf() {
+  xyzz9 r;
+  r.x = Fq9::one();
+  r.y = Fq9::one();
+  r.zz = Fq9::zero();
+  r.zzz = Fq9::zero();
+  return r;
+}
+
+ZK_HD xyzz9 xyzz9_from_aff(const aff9& a) {
+  if (aff9_is_inf(a)) return xyzz9_inf();
+  xyzz9 r;
+  r.x = a.x;
+  r.y = a.y;
+  r.zz = Fq9::one();
+  r.zzz = Fq9::one();
+  return r;
+}
+
+// mdbl-2008-s
+ZK_HD xyzz9 xyzz9_dbl_aff(const aff9& a) {
+  f9 u = Fq9::dbl(a.y);
+  f9 v = Fq9::sqr(u);
+  f9 w = Fq9::mul(u, v);
+  f9 s = Fq9::mul(a.x, v);
+  f9 x2 = Fq9::sqr(a.x);
+  f9 m = Fq9::add(Fq9::dbl(x2), x2);
+  xyzz9 r;
+  r.x = Fq9::sub(Fq9::sqr(m), Fq9::dbl(s));
+  r.y = Fq9::sub(Fq9::mul(m, Fq9::sub(s, r.x)), Fq9::mul(w, a.y));
+  r.zz = v;
+  r.zzz = w;
+  return r;
+}
+
+// dbl-2008-s
+ZK_HD xyzz9 xyzz9_dbl(const xyzz9& p) {
+  if (xyzz9_is_inf(p)) return p;
+  f9 u = Fq9::dbl(p.y);
+  f9 v = Fq9::sqr(u);
+  f9 w = Fq9::mul(u, v);
+  f9 s = Fq9::mul(p.x, v);
+  f9 x2 = Fq9::sqr(p.x);
+  f9 m = Fq9::add(Fq9::dbl(x2), x2);
+  xyzz9 r;
+  r.x = Fq9::sub(Fq9::sqr(m), Fq9::dbl(s));
+  r.y = Fq9::sub(Fq9::mul(m, Fq9::sub(s, r.x)), Fq9::mul(w, p.y));
+  r.zz = Fq9::mul(v, p.zz);
+  r.zzz = Fq9::mul(w, p.zzz);
+  return r;
+}
+
+// madd-2008-s: p += a (a affine, not the identity)
+ZK_HD void xyzz9_madd(xyzz9& p, const aff9& a) {
+  if (xyzz9_is_inf(p)) {
+    p = xyzz9_from_aff(a);
+    return;
+  }
+  f9 u2 = Fq9::mul(a.x, p.zz);
+  f9 s2 = Fq9::mul(a.y, p.zzz);
+  f9 P = Fq9::sub(u2, p.x);
+  f9 R = Fq9::sub(s2, p.y);
+  if (Fq9::is_zero(P)) {
+    if (Fq9::is_zero(R)) {
+      p = xyzz9_dbl_aff(a);
+    } else {
+      p = xyzz9_inf();
+    }
+    return;
+  }
+  f9 pp = Fq9::sqr(P);
+  f9 ppp = Fq9::mul(P, pp);
+  f9 q = Fq9::mul(p.x, pp);
+  f9 x3 = Fq9::sub(Fq9::sub(Fq9::sqr(R), ppp), Fq9::dbl(q));
+  f9 y3 = Fq9::sub(Fq9::mul(R, Fq9::sub(q, x3)), Fq9::mul(p.y, ppp));
+  p.x = x3;
+  p.y = y3;
+  p.zz = Fq9::mul(p.zz, pp);
+  p.zzz = Fq9::mul(p.zzz, ppp);
+}
+
+// add-2008-s
+ZK_HD xyzz9 xyzz9_add(const xyzz9& p, const xyzz9& q) {
+  if (xyzz9_is_inf(p)) return q;
+  if (xyzz9_is_inf(q)) return p;
+  f9 u1 = Fq9::mul(p.x, q.zz);
+  f9 u2 = Fq9::mul(q.x, p.zz);
+  f9 s1 = Fq9::mul(p.y, q.zzz);
+  f9 s2 = Fq9::mul(q.y, p.zzz);
+  f9 P = Fq9::sub(u2, u1);
+  f9 R = Fq9::sub(s2, s1);
+  if (Fq9::is_zero(P)) {
+    if (Fq9::is_zero(R)) return xyzz9_dbl(p);
+    return xyzz9_inf();
+  }
+  f9 pp = Fq9::sqr(P);
+  f9 ppp = Fq9::mul(P, pp);
+  f9 qq = Fq9::mul(u1, pp);
+  xyzz9 r;
+  r.x = Fq9::sub(Fq9::sub(Fq9::sqr(R), ppp), Fq9::dbl(qq));
+  r.y = Fq9::sub(Fq9::mul(R, Fq9::sub(qq, r.x)), Fq9::mul(s1, ppp));
+  r.zz = Fq9::mul(Fq9::mul(p.zz, q.zz), pp);
+  r.zzz = Fq9::mul(Fq9::mul(p.zzz, q.zzz), ppp);
+  return r;
+}
+
+ZK_HD xyzz9 xyzz9_mul_small(const xyzz9& p, uint32_t k) {
+  xyzz9 acc = xyzz9_inf();
+  if (k == 0 || xyzz9_is_inf(p)) return acc;
+  int top = 31;
+  while (top > 0 && !((k >> top) & 1)) top--;
+  acc = p;
+  for (int b = top - 1; b >= 0; b--) {
+    acc = xyzz9_dbl(acc);
+    if ((k >> b) & 1) acc = xyzz9_add(acc, p);
+  }
+  return acc;
+}
+
+// R = 2^256 32-bit-word point (curve.hpp) <-> R = 2^261 limb point
+ZK_HD aff9 aff9_from_wire(const fe& x256, const fe& y256) {
+  aff9 r;
+  f9 x = unpack9(x256), y = unpack9(y256);
+  r.x = Fq9::to261(x);  // 0 stays 0: the infinity marker survives
+  r.y = Fq9::to261(y);
+  return r;
+}
+
+ZK_HD xyzz xyzz_from9(const xyzz9& p) {
+  xyzz r;
+  r.x = pack9(Fq9::canonical(Fq9::to256(p.x)));
+  r.y = pack9(Fq9::canonical(Fq9::to256(p.y)));
+  r.zz = pack9(Fq9::canonical(Fq9::to256(p.zz)));
+  r.zzz = pack9(Fq9::canonical(Fq9::to256(p.zzz)));
+  return r;
+}
+
+}  // namespace zk

@@ -1,0 +1,256 @@
+// BN254 Fq / Fr arithmetic on 9 x 29-bit limbs for the hot kernels
+// (MSM bucket accumulation and reduction, NTT butterflies).
+//
+// Why 29 bits: with 29-bit limbs a product-scanning Montgomery multiply
+// (FIPS: every column of a*b + m*p summed in one 64-bit accumulator) never
+// overflows 64 bits (at most 18 products of < 2^58 plus a < 2^35 carry), so
+// each partial product is exactly one v_mad_u64_u32 with the accumulator as
+// its 64-bit addend and no carry handling.  The 8 x 32-bit CIOS form needs
+// an add-with-carry and register moves around every multiply-add; on gfx950
+// this form issues about 2.5x fewer VALU instructions per product.
+//
+// Representation: Montgomery with R = 2^261 (9 * 29), values kept lazily
+// in [0, 2p).  R > 4p, so mul() takes inputs below 2p and returns below 2p
+// without a final subtraction.  add / sub / neg return values in [0, 2p);
+// canonical() maps to [0, p).  unpack / pack convert to and from the
+// 32-byte little-endian wire format.
+//
+// Domain bookkeeping: a 32-byte buffer holding x * 2^256 (the reference's
+// Montgomery form) read as an R = 2^261 value represents x * 2^-5.
+// Linear maps (the NTT) can therefore run on such buffers unchanged, with
+// twiddles in R = 2^261 form; the point arithmetic converts coordinates with
+// to261() / to256() (one product each).
+#pragma once
+#include "field.hpp"
+
+namespace zk {
+
+struct f9 {
+  u32 l[9];
+};
+
+static constexpr u32 M29 = (1u << 29) - 1;
+
+struct Fq29Params {
+  static constexpr u32 P[9] = {0x187cfd47u, 0x010460b6u, 0x1c72a34fu, 0x02d522d0u, 0x1585d978u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
+  static constexpr u32 P2[9] = {0x10f9fa8eu, 0x0208c16du, 0x18e5469eu, 0x05aa45a1u, 0x0b0bb2f0u, 0x05b68181u, 0x014dc282u, 0x1cb84c68u, 0x0060c89cu};
+  static constexpr u32 PINV = 0x04866389u;  // -p^-1 mod 2^29
+  static constexpr u32 ONE[9] = {0x157ccc21u, 0x141c2758u, 0x185230d3u, 0x014c0419u, 0x0aa36fb9u, 0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};                    // 2^261 mod p
+  static constexpr u32 TO261[9] = {0x13349ca1u, 0x1a5d84a8u, 0x0a3e5cacu, 0x100249e0u, 0x12b951e8u, 0x0e92d304u, 0x14cb95b3u, 0x041b9d3du, 0x00058003u};         // 2^266 mod p: x 2^256 -> x 2^261
+  static constexpr u32 TO256[9] = {0x058f0d9du, 0x1aea1c6eu, 0x11c2cf74u, 0x11d651ebu, 0x1462c0a7u, 0x11b7bc3cu, 0x1cbd99bau, 0x183340fbu, 0x000e0a77u};         // 2^256 mod p: x 2^261 -> x 2^256
+};
+struct Fr29Params {
+  static constexpr u32 P[9] = {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
+  static constexpr u32 P2[9] = {0x00000002u, 0x1e1f593fu, 0x1cb848a1u, 0x0fa121e6u, 0x0b0ba506u, 0x05b68181u, 0x014dc282u, 0x1cb84c68u, 0x0060c89cu};
+  static constexpr u32 PINV = 0x0fffffffu;  // -p^-1 mod 2^29
+  static constexpr u32 ONE[9] = {0x0fffff57u, 0x1ea70ab4u, 0x052c068bu, 0x17504f49u, 0x0aa8075bu, 0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};                    // 2^261 mod p
+  static constexpr u32 TO261[9] = {0x0fffead7u, 0x1d5444f4u, 0x04438aa5u, 0x03b4d096u, 0x134c84dau, 0x0e92d304u, 0x14cb95b3u, 0x041b9d3du, 0x00058003u};         // 2^266 mod p: x 2^256 -> x 2^261
+  static constexpr u32 TO256[9] = {0x0ffffffbu, 0x04b1a0e2u, 0x18334a6bu, 0x18ed2b3eu, 0x1462e36fu, 0x11b7bc3cu, 0x1cbd99bau, 0x183340fbu, 0x000e0a77u};         // 2^256 mod p: x 2^261 -> x 2^256
+};
+
+template <class F>
+struct Mont29 {
+  static ZK_HD f9 zero() {
+    f9 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.l[i] = 0;
+    return r;
+  }
+  static ZK_HD f9 one() {
+    f9 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.l[i] = F::ONE[i];
+    return r;
+  }
+  static ZK_HD f9 constant(const u32 (&c)[9]) {
+    f9 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.l[i] = c[i];
+    return r;
+  }
+
+  // FIPS Montgomery product: columns k = 0..16 of a*b + m*p in one u64
+  static ZK_HD f9 mul(const f9& a, const f9& b) {
+    u32 m[9];
+    f9 r;
+    u64 acc = 0;
+#pragma unroll
+    for (int k = 0; k < 17; k++) {
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        const int j = k - i;
+        if (j >= 0 && j < 9) acc += (u64)a.l[i] * b.l[j];
+      }
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        const int j = k - i;
+        if (i < k && j >= 0 && j < 9) acc += (u64)m[i] * F::P[j];
+      }
+      if (k < 9) {
+        m[k] = ((u32)acc * F::PINV) & M29;
+        acc += (u64)m[k] * F::P[0];
+        acc >>= 29;
+      } else {
+        r.l[k - 9] = (u32)acc & M29;
+        acc >>= 29;
+      }
+    }
+    r.l[8] = (u32)acc;
+    return r;
+  }
+
+  // squaring: cross products once with a doubled operand
+  static ZK_HD f9 sqr(const f9& a) {
+    u32 a2[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) a2[i] = a.l[i] << 1;
+    u32 m[9];
+    f9 r;
+    u64 acc = 0;
+#pragma unroll
+    for (int k = 0; k < 17; k++) {
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        const int j = k - i;
+        if (j > i && j < 9) acc += (u64)a2[i] * a.l[j];
+      }
+      if ((k & 1) == 0 && k / 2 < 9) acc += (u64)a.l[k / 2] * a.l[k / 2];
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        const int j = k - i;
+        if (i < k && j >= 0 && j < 9) acc += (u64)m[i] * F::P[j];
+      }
+      if (k < 9) {
+        m[k] = ((u32)acc * F::PINV) & M29;
+        acc += (u64)m[k] * F::P[0];
+        acc >>= 29;
+      } else {
+        r.l[k - 9] = (u32)acc & M29;
+        acc >>= 29;
+      }
+    }
+    r.l[8] = (u32)acc;
+    return r;
+  }
+
+  // s -> s - 2p if s >= 2p  (s < 4p, limbs normalised)
+  static ZK_HD f9 reduce2p(const f9& s) {
+    f9 d;
+    int32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const int32_t t = (int32_t)s.l[i] - (int32_t)F::P2[i] + br;
+      d.l[i] = (u32)t & M29;
+      br = t >> 29;
+    }
+    f9 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.l[i] = br ? s.l[i] : d.l[i];
+    return r;
+  }
+
+  static ZK_HD f9 add(const f9& a, const f9& b) {
+    f9 s;
+    u32 c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const u32 t = a.l[i] + b.l[i] + c;
+      s.l[i] = t & M29;
+      c = t >> 29;
+    }
+    return reduce2p(s);
+  }
+
+  // a - b + 2p in (0, 4p), then into [0, 2p)
+  static ZK_HD f9 sub(const f9& a, const f9& b) {
+    f9 s;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const int32_t t = (int32_t)a.l[i] - (int32_t)b.l[i] + (int32_t)F::P2[i] + c;
+      s.l[i] = (u32)t & M29;
+      c = t >> 29;
+    }
+    return reduce2p(s);
+  }
+
+  static ZK_HD f9 dbl(const f9& a) { return add(a, a); }
+  static ZK_HD f9 neg(const f9& a) { return sub(zero(), a); }
+
+  // a in [0, 2p): a == 0 mod p  <=>  a == 0 or a == p
+  static ZK_HD bool is_zero(const f9& a) {
+    u32 z = 0, e = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      z |= a.l[i];
+      e |= a.l[i] ^ F::P[i];
+    }
+    return z == 0 || e == 0;
+  }
+
+  // [0, 2p) -> [0, p)
+  static ZK_HD f9 canonical(const f9& a) {
+    f9 d;
+    int32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const int32_t t = (int32_t)a.l[i] - (int32_t)F::P[i] + br;
+      d.l[i] = (u32)t & M29;
+      br = t >> 29;
+    }
+    f9 r;
+#pragma unroll
+    for (int i = 0; i < 9; i++) r.l[i] = br ? a.l[i] : d.l[i];
+    return r;
+  }
+
+  static ZK_HD f9 to261(const f9& x256) { return mul(x256, constant(F::TO261)); }
+  static ZK_HD f9 to256(const f9& x261) { return mul(x261, constant(F::TO256)); }
+};
+
+typedef Mont29<Fq29Params> Fq9;
+typedef Mont29<Fr29Params> Fr9;
+
+// 8 x 32-bit words (value < 2^256) <-> 9 x 29-bit limbs
+ZK_HD f9 unpack9(const fe& x) {
+  f9 r;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const int bit = 29 * i, w = bit >> 5, sh = bit & 31;
+    u32 v = x.w[w] >> sh;
+    if (sh > 3 && w + 1 < 8) v |= x.w[w + 1] << (32 - sh);
+    r.l[i] = v & M29;
+  }
+  return r;
+}
+
+// value must be below 2^256
+ZK_HD fe pack9(const f9& a) {
+  fe r;
+#pragma unroll
+  for (int w = 0; w < 8; w++) {
+    const int bit = 32 * w, i = bit / 29, sh = bit % 29;
+    u32 v = a.l[i] >> sh;
+    if (i + 1 < 9) v |= a.l[i + 1] << (29 - sh);
+    if (sh > 26 && i + 2 < 9) v |= a.l[i + 2] << (58 - sh);
+    r.w[w] = v;
+  }
+  return r;
+}
+
+#if defined(__HIP__)
+__device__ __forceinline__ f9 load_f9(const f9* p) {
+  const u32* q = reinterpret_cast<const u32*>(p);
+  f9 r;
+#pragma unroll
+  for (int i = 0; i < 9; i++) r.l[i] = q[i];
+  return r;
+}
+__device__ __forceinline__ void store_f9(f9* p, const f9& v) {
+  u32* q = reinterpret_cast<u32*>(p);
+#pragma unroll
+  for (int i = 0; i < 9; i++) q[i] = v.l[i];
+}
+#endif
+
+}  // namespace zk

@@ -86,3 +86,81 @@ int zkhost_xyzz(int op, const uint64_t* p, const uint64_t* q, uint32_t k, uint64
   return 0;
 }
 }
+
+// ---- 29-bit limb arithmetic (field29.hpp / curve29.hpp), same word I/O:
+// inputs / outputs are 4 x u64 little-endian values (< 2^256).
+#include "curve29.hpp"
+
+extern "C" {
+// op: 0 mul, 1 add, 2 sub, 3 neg, 4 sqr, 5 canonical, 6 to261, 7 to256, 8 is_zero; field 0 Fr 1 Fq
+int zkhost_field29(int field, int op, const uint64_t* a, const uint64_t* b, uint64_t* out, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++) {
+    f9 x = unpack9(rd(a + 4 * i)), y = b ? unpack9(rd(b + 4 * i)) : Fr9::zero(), r;
+    if (field == 0) {
+      switch (op) {
+        case 0: r = Fr9::mul(x, y); break;
+        case 1: r = Fr9::add(x, y); break;
+        case 2: r = Fr9::sub(x, y); break;
+        case 3: r = Fr9::neg(x); break;
+        case 4: r = Fr9::sqr(x); break;
+        case 5: r = Fr9::canonical(x); break;
+        case 6: r = Fr9::to261(x); break;
+        case 7: r = Fr9::to256(x); break;
+        case 8: r = Fr9::zero(); r.l[0] = Fr9::is_zero(x); break;
+        default: return 1;
+      }
+    } else {
+      switch (op) {
+        case 0: r = Fq9::mul(x, y); break;
+        case 1: r = Fq9::add(x, y); break;
+        case 2: r = Fq9::sub(x, y); break;
+        case 3: r = Fq9::neg(x); break;
+        case 4: r = Fq9::sqr(x); break;
+        case 5: r = Fq9::canonical(x); break;
+        case 6: r = Fq9::to261(x); break;
+        case 7: r = Fq9::to256(x); break;
+        case 8: r = Fq9::zero(); r.l[0] = Fq9::is_zero(x); break;
+        default: return 1;
+      }
+    }
+    wr(out + 4 * i, pack9(r));
+  }
+  return 0;
+}
+
+// XYZZ on 29-bit limbs: points given as 16 words of R = 2^261 values
+// op 0: add(p, q)  1: dbl(p)  2: madd(p, affine q)  3: mul_small(p, k)
+int zkhost_xyzz29(int op, const uint64_t* p, const uint64_t* q, uint32_t k, uint64_t* out) {
+  xyzz9 P, R;
+  P.x = unpack9(rd(p)); P.y = unpack9(rd(p + 4)); P.zz = unpack9(rd(p + 8)); P.zzz = unpack9(rd(p + 12));
+  switch (op) {
+    case 0: {
+      xyzz9 Q;
+      Q.x = unpack9(rd(q)); Q.y = unpack9(rd(q + 4)); Q.zz = unpack9(rd(q + 8)); Q.zzz = unpack9(rd(q + 12));
+      R = xyzz9_add(P, Q);
+      break;
+    }
+    case 1: R = xyzz9_dbl(P); break;
+    case 2: {
+      aff9 a;
+      a.x = unpack9(rd(q));
+      a.y = unpack9(rd(q + 4));
+      R = P;
+      if (!aff9_is_inf(a)) xyzz9_madd(R, a);
+      break;
+    }
+    case 3: R = xyzz9_mul_small(P, k); break;
+    default: return 1;
+  }
+  wr(out, pack9(R.x)); wr(out + 4, pack9(R.y)); wr(out + 8, pack9(R.zz)); wr(out + 12, pack9(R.zzz));
+  return 0;
+}
+
+// wire affine (R = 2^256) -> limb form -> XYZZ (R = 2^256, canonical) round trip
+int zkhost_wire_roundtrip(const uint64_t* xy, uint64_t* out16) {
+  aff9 a = aff9_from_wire(rd(xy), rd(xy + 4));
+  xyzz r = xyzz_from9(xyzz9_from_aff(a));
+  wr(out16, r.x); wr(out16 + 4, r.y); wr(out16 + 8, r.zz); wr(out16 + 12, r.zzz);
+  return 0;
+}
+}

@@ -17,18 +17,12 @@
 // whose weighted partial sums are recombined in parallel.
 #include "internal.hpp"
 #include "curve.hpp"
+#include "curve29.hpp"
 
 namespace zk {
 
 static constexpr uint32_t NO_KEY = 0xffffffffu;
 static constexpr uint32_t SIGN = 0x80000000u;
-
-__device__ __forceinline__ aff load_aff(const fe* __restrict__ bases, uint32_t idx) {
-  aff a;
-  a.x = load_fe(bases + 2 * (uint64_t)idx);
-  a.y = load_fe(bases + 2 * (uint64_t)idx + 1);
-  return a;
-}
 
 __device__ __forceinline__ xyzz load_xyzz(const xyzz* p) {
   const fe* f = reinterpret_cast<const fe*>(p);
@@ -47,6 +41,61 @@ __device__ __forceinline__ void store_xyzz(xyzz* p, const xyzz& v) {
   store_fe(f + 2, v.zz);
   store_fe(f + 3, v.zzz);
 }
+
+// Hot-loop storage: coordinates are R = 2^261 limb values (field29.hpp)
+// packed back into 32 bytes each, so a base is 64 bytes and a bucket 128.
+__device__ __forceinline__ aff9 load_aff9(const fe* __restrict__ bases, uint32_t idx) {
+  aff9 a;
+  a.x = unpack9(load_fe(bases + 2 * (uint64_t)idx));
+  a.y = unpack9(load_fe(bases + 2 * (uint64_t)idx + 1));
+  return a;
+}
+
+__device__ __forceinline__ xyzz9 load_xyzz9(const xyzz* p) {
+  const fe* f = reinterpret_cast<const fe*>(p);
+  xyzz9 r;
+  r.x = unpack9(load_fe(f));
+  r.y = unpack9(load_fe(f + 1));
+  r.zz = unpack9(load_fe(f + 2));
+  r.zzz = unpack9(load_fe(f + 3));
+  return r;
+}
+
+__device__ __forceinline__ void store_xyzz9(xyzz* p, const xyzz9& v) {
+  fe* f = reinterpret_cast<fe*>(p);
+  store_fe(f, pack9(v.x));
+  store_fe(f + 1, pack9(v.y));
+  store_fe(f + 2, pack9(v.zz));
+  store_fe(f + 3, pack9(v.zzz));
+}
+
+// wire bases (affine, R = 2^256, x = y = 0 for infinity) -> R = 2^261 form
+__global__ __launch_bounds__(256) void msm_bases_convert_kernel(const fe* __restrict__ in, uint64_t n,
+                                                                fe* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const aff9 a = aff9_from_wire(load_fe(in + 2 * i), load_fe(in + 2 * i + 1));
+  store_fe(out + 2 * i, pack9(Fq9::canonical(a.x)));
+  store_fe(out + 2 * i + 1, pack9(Fq9::canonical(a.y)));
+}
+
+// window sums: limb form -> R = 2^256 canonical XYZZ for the host combination
+__global__ void msm_wsum_export_kernel(const xyzz* __restrict__ in, uint32_t W, xyzz* __restrict__ out) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  const xyzz9 p = load_xyzz9(in + w);
+  xyzz r;
+  if (xyzz9_is_inf(p)) {
+    r.x = Fq::one();
+    r.y = Fq::one();
+    r.zz = Fq::zero();
+    r.zzz = Fq::zero();
+  } else {
+    r = xyzz_from9(p);
+  }
+  store_xyzz(out + w, r);
+}
+
 
 // 1. scalar (Montgomery) -> canonical -> W signed c-bit digits
 //    digits[w * n + i] = |d| | (d < 0 ? SIGN : 0)
@@ -276,34 +325,34 @@ __global__ __launch_bounds__(256) void msm_accumulate_kernel(
   uint32_t k = lo;
   uint32_t rbeg = row_ptr[k], rend = row_ptr[k + 1];
   uint32_t hk = NO_KEY;
-  xyzz acc = xyzz_inf();
+  xyzz9 acc = xyzz9_inf();
   for (uint64_t p = start; p < end; p++) {
     while (p >= rend) {
       if (rend > rbeg) {
         if (rbeg < start) {
           hk = k;
-          store_xyzz(head + s, acc);
+          store_xyzz9(head + s, acc);
         } else {
-          store_xyzz(buckets + k, acc);
+          store_xyzz9(buckets + k, acc);
         }
       }
-      acc = xyzz_inf();
+      acc = xyzz9_inf();
       k++;
       rbeg = rend;
       rend = row_ptr[k + 1];
     }
     const uint32_t v = sorted[p];
-    aff a = load_aff(bases, v & ~SIGN);
-    if (!aff_is_inf(a)) {
-      if (v & SIGN) a.y = Fq::neg(a.y);
-      xyzz_madd(acc, a);
+    aff9 a = load_aff9(bases, v & ~SIGN);
+    if (!aff9_is_inf(a)) {
+      if (v & SIGN) a.y = Fq9::neg(a.y);
+      xyzz9_madd(acc, a);
     }
   }
   if (rbeg < start) {
     hk = k;
-    store_xyzz(head + s, acc);
+    store_xyzz9(head + s, acc);
   } else {
-    store_xyzz(buckets + k, acc);
+    store_xyzz9(buckets + k, acc);
   }
   head_key[s] = hk;
 }
@@ -339,13 +388,13 @@ __global__ __launch_bounds__(256) void msm_fixup_round_kernel(const uint32_t* __
   if (!head_run(row_ptr, k, L, s, first, len)) return;
   const uint64_t off = s - first;
   if (off % (stride * FIXUP_FAN) != 0 || off + stride >= len) return;
-  xyzz acc = load_xyzz(head + s);
+  xyzz9 acc = load_xyzz9(head + s);
   for (uint32_t j = 1; j < FIXUP_FAN; j++) {
     const uint64_t t = off + (uint64_t)j * stride;
     if (t >= len) break;
-    acc = xyzz_add(acc, load_xyzz(head + first + t));
+    acc = xyzz9_add(acc, load_xyzz9(head + first + t));
   }
-  store_xyzz(head + s, acc);
+  store_xyzz9(head + s, acc);
 }
 
 __global__ __launch_bounds__(256) void msm_fixup_final_kernel(const uint32_t* __restrict__ row_ptr,
@@ -361,7 +410,7 @@ __global__ __launch_bounds__(256) void msm_fixup_final_kernel(const uint32_t* __
   if (k == NO_KEY) return;
   uint64_t first, len;
   if (!head_run(row_ptr, k, L, s, first, len) || s != first) return;
-  store_xyzz(buckets + k, xyzz_add(load_xyzz(buckets + k), load_xyzz(head + s)));
+  store_xyzz9(buckets + k, xyzz9_add(load_xyzz9(buckets + k), load_xyzz9(head + s)));
 }
 
 // 6. bucket reduction, chunked: chunk t of window w covers buckets
@@ -379,31 +428,31 @@ __global__ __launch_bounds__(128) void msm_reduce_chunks_kernel(const xyzz* __re
   const uint32_t lo = 1 + t * K;
   uint32_t hi = lo + K - 1;
   if (hi > B - 1) hi = B - 1;
-  xyzz run = xyzz_inf(), sum = xyzz_inf();
+  xyzz9 run = xyzz9_inf(), sum = xyzz9_inf();
   for (uint32_t b = hi; b >= lo; b--) {
     const uint32_t key = w * B + b;
-    if (row_ptr[key + 1] > row_ptr[key]) run = xyzz_add(run, load_xyzz(buckets + key));
-    sum = xyzz_add(sum, run);
+    if (row_ptr[key + 1] > row_ptr[key]) run = xyzz9_add(run, load_xyzz9(buckets + key));
+    sum = xyzz9_add(sum, run);
   }
-  if (lo > 1) sum = xyzz_add(sum, xyzz_mul_small(run, lo - 1));
-  store_xyzz(partial + gid, sum);
+  if (lo > 1) sum = xyzz9_add(sum, xyzz9_mul_small(run, lo - 1));
+  store_xyzz9(partial + gid, sum);
 }
 
 // 7. per-window tree sum of the chunk partials
 __global__ __launch_bounds__(128) void msm_reduce_windows_kernel(const xyzz* __restrict__ partial,
                                                                  uint32_t cpw,
                                                                  xyzz* __restrict__ window_sums) {
-  __shared__ xyzz sh[128];
+  __shared__ xyzz9 sh[128];
   const uint32_t w = blockIdx.x, tid = threadIdx.x;
-  xyzz acc = xyzz_inf();
-  for (uint32_t t = tid; t < cpw; t += blockDim.x) acc = xyzz_add(acc, load_xyzz(partial + (uint64_t)w * cpw + t));
+  xyzz9 acc = xyzz9_inf();
+  for (uint32_t t = tid; t < cpw; t += blockDim.x) acc = xyzz9_add(acc, load_xyzz9(partial + (uint64_t)w * cpw + t));
   sh[tid] = acc;
   __syncthreads();
   for (uint32_t step = blockDim.x >> 1; step > 0; step >>= 1) {
-    if (tid < step) sh[tid] = xyzz_add(sh[tid], sh[tid + step]);
+    if (tid < step) sh[tid] = xyzz9_add(sh[tid], sh[tid + step]);
     __syncthreads();
   }
-  if (tid == 0) store_xyzz(window_sums + w, sh[0]);
+  if (tid == 0) store_xyzz9(window_sums + w, sh[0]);
 }
 
 // ---------------------------------------------------------------- host side
@@ -462,6 +511,8 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   const size_t o_head = off; off = align256(off + Smax * sizeof(xyzz));
   const size_t o_partial = off; off = align256(off + (size_t)W * cpw * sizeof(xyzz));
   const size_t o_wsum = off; off = align256(off + (size_t)W * sizeof(xyzz));
+  const size_t o_wexp = off; off = align256(off + (size_t)W * sizeof(xyzz));
+  const size_t o_bases = off; off = align256(off + (size_t)n * 64);
   void* base;
   int rc = ensure_scratch(c, 2, off, &base);
   if (rc) return rc;
@@ -477,9 +528,12 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   xyzz* head = (xyzz*)(b8 + o_head);
   xyzz* partial = (xyzz*)(b8 + o_partial);
   xyzz* wsum = (xyzz*)(b8 + o_wsum);
+  xyzz* wexp = (xyzz*)(b8 + o_wexp);
+  fe* bases9 = (fe*)(b8 + o_bases);
 
   hipEvent_t ev;
   phase_begin(c, "msm_digits", st, &ev);
+  msm_bases_convert_kernel<<<grid_for(n, 256), 256, 0, st>>>(bases, n, bases9);
   msm_digits_kernel<<<grid_for(n, 256), 256, 0, st>>>(scalars, n, (uint32_t)cb, W, digits);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_digits", st, ev);
@@ -499,7 +553,7 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   phase_end(c, "msm_sort", st, ev);
 
   phase_begin(c, "msm_accumulate", st, &ev);
-  msm_accumulate_kernel<<<grid_for(Smax, 256), 256, 0, st>>>(bases, sorted, row_ptr, nkeys, L, buckets,
+  msm_accumulate_kernel<<<grid_for(Smax, 256), 256, 0, st>>>(bases9, sorted, row_ptr, nkeys, L, buckets,
                                                             hkey, head);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_accumulate", st, ev);
@@ -515,12 +569,13 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   msm_reduce_chunks_kernel<<<grid_for((uint64_t)W * cpw, 128), 128, 0, st>>>(buckets, row_ptr, W, B, K,
                                                                            cpw, partial);
   msm_reduce_windows_kernel<<<W, 128, 0, st>>>(partial, cpw, wsum);
+  msm_wsum_export_kernel<<<grid_for(W, 64), 64, 0, st>>>(wsum, W, wexp);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_reduce", st, ev);
 
   // window combination on the host: acc = Sum_w 2^(c w) S_w (Horner)
   std::vector<xyzz> hs(W);
-  ZK_HIP(c, hipMemcpyAsync(hs.data(), wsum, (size_t)W * sizeof(xyzz), hipMemcpyDeviceToHost, st));
+  ZK_HIP(c, hipMemcpyAsync(hs.data(), wexp, (size_t)W * sizeof(xyzz), hipMemcpyDeviceToHost, st));
   ZK_HIP(c, hipStreamSynchronize(st));
   xyzz acc = hs[W - 1];
   for (int w = (int)W - 2; w >= 0; w--) {

@@ -20,7 +20,15 @@
 // per butterfly, none extra for the pass split), and writes the tile back.
 // Coset / n^-1 scaling is fused into the first pass's loads and the last
 // pass's stores.
+//
+// Arithmetic runs on 9 x 29-bit limbs (field29.hpp, R = 2^261).  The data
+// buffers hold R = 2^256 Montgomery values; since the transform is linear,
+// reading them as R = 2^261 values scales every element by the same 2^-5,
+// which the output carries back out.  Twiddles and scaling tables are kept
+// in R = 2^261 form (packed 32 bytes per entry), intermediate passes store
+// lazily reduced values (< 2p), and the last pass stores canonical values.
 #include "internal.hpp"
+#include "field29.hpp"
 
 namespace zk {
 
@@ -29,9 +37,11 @@ static constexpr int NTT_THREADS = 256;
 static constexpr int NTT_BFLY_PER_THREAD = NTT_TILE / 2 / NTT_THREADS;
 static constexpr int POW_LO_BITS = 10;  // two-level power tables: lo has 2^10 entries
 
-__device__ __forceinline__ fe pow_lookup(const fe* __restrict__ hi, const fe* __restrict__ lo,
-                                         uint64_t idx) {
-  return Fr::mul(load_fe(hi + (idx >> POW_LO_BITS)), load_fe(lo + (idx & ((1u << POW_LO_BITS) - 1))));
+// tables: R = 2^261 values packed in 32 bytes
+__device__ __forceinline__ f9 tab9(const fe* __restrict__ t, uint64_t i) { return unpack9(load_fe(t + i)); }
+
+__device__ __forceinline__ f9 pow_lookup(const fe* __restrict__ hi, const fe* __restrict__ lo, uint64_t idx) {
+  return Fr9::mul(tab9(hi, idx >> POW_LO_BITS), tab9(lo, idx & ((1u << POW_LO_BITS) - 1)));
 }
 
 // scale modes
@@ -41,8 +51,8 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     const fe* __restrict__ in, fe* __restrict__ out, const fe* __restrict__ tw, uint32_t log_n,
     uint32_t s0, uint32_t k, uint32_t log_t, uint64_t in_len, int pre_mode,
     const fe* __restrict__ pre_hi, const fe* __restrict__ pre_lo, int post_mode,
-    const fe* __restrict__ post_hi, const fe* __restrict__ post_lo) {
-  __shared__ fe lds[NTT_TILE];
+    const fe* __restrict__ post_hi, const fe* __restrict__ post_lo, int last) {
+  __shared__ f9 lds[NTT_TILE];
   const uint64_t n = 1ull << log_n;
   const uint32_t R = 1u << k;
   const uint32_t T = 1u << log_t;
@@ -59,12 +69,13 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
   for (uint32_t e = tid; e < RT; e += NTT_THREADS) {
     const uint32_t r = e >> log_t, c = e & (T - 1);
     const uint64_t idx = j0 + c + (uint64_t)r * cols;
-    fe v;
+    f9 v;
     if (idx < in_len) {
-      v = load_fe(in + idx);
-      if (pre_mode == SCALE_TABLE) v = Fr::mul(v, pow_lookup(pre_hi, pre_lo, idx));
+      v = unpack9(load_fe(in + idx));
+      if (s0 == 0) v = Fr9::reduce2p(Fr9::reduce2p(v));  // caller data: any value < 2^256
+      if (pre_mode == SCALE_TABLE) v = Fr9::mul(v, pow_lookup(pre_hi, pre_lo, idx));
     } else {
-      v = Fr::zero();
+      v = Fr9::zero();
     }
     lds[e] = v;
   }
@@ -74,7 +85,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
   const uint32_t halfRT = RT >> 1;
   for (uint32_t t = 0; t < k; t++) {
     const uint32_t s = s0 + t;
-    fe u[NTT_BFLY_PER_THREAD], v[NTT_BFLY_PER_THREAD];
+    f9 u[NTT_BFLY_PER_THREAD], v[NTT_BFLY_PER_THREAD];
 #pragma unroll
     for (int i = 0; i < NTT_BFLY_PER_THREAD; i++) {
       const uint32_t b = tid + i * NTT_THREADS;
@@ -91,22 +102,22 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
       if (b < halfRT) {
         const uint32_t c = b & (T - 1), q = b >> log_t;
         const uint32_t d = q & ((1u << t) - 1);
-        fe vv = v[i];
+        f9 vv = v[i];
         if (s != 0) {
           const uint64_t low = (j0 + c) & (Ns - 1);
           const uint64_t e = (low + (uint64_t)d * Ns) << (log_n - 1 - s);
-          vv = Fr::mul(vv, load_fe(tw + e));
+          vv = Fr9::mul(vv, tab9(tw, e));
         }
         const uint32_t o = ((q - d) << 1) + d;
-        lds[(o << log_t) + c] = Fr::add(u[i], vv);
-        lds[((o + (1u << t)) << log_t) + c] = Fr::sub(u[i], vv);
+        lds[(o << log_t) + c] = Fr9::add(u[i], vv);
+        lds[((o + (1u << t)) << log_t) + c] = Fr9::sub(u[i], vv);
       }
     }
     __syncthreads();
   }
 
   // ---- store: element (sp, c) -> out[(j / Ns) * Ns * R + (j % Ns) + sp * Ns], j = j0 + c
-  const fe post_c = (post_mode == SCALE_CONST) ? load_fe(post_hi) : Fr::zero();
+  const f9 post_c = (post_mode == SCALE_CONST) ? tab9(post_hi, 0) : Fr9::zero();
   for (uint32_t e = tid; e < RT; e += NTT_THREADS) {
     uint32_t sp, c;
     if (s0 == 0) {  // Ns = 1: a column's R outputs are contiguous
@@ -118,13 +129,14 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     }
     const uint64_t j = j0 + c;
     const uint64_t pos = ((j >> s0) << (s0 + k)) + (j & (Ns - 1)) + ((uint64_t)sp << s0);
-    fe x = lds[(sp << log_t) + c];
+    f9 x = lds[(sp << log_t) + c];
     if (post_mode == SCALE_CONST) {
-      x = Fr::mul(x, post_c);
+      x = Fr9::mul(x, post_c);
     } else if (post_mode == SCALE_TABLE) {
-      x = Fr::mul(x, pow_lookup(post_hi, post_lo, pos));
+      x = Fr9::mul(x, pow_lookup(post_hi, post_lo, pos));
     }
-    store_fe(out + pos, x);
+    if (last) x = Fr9::canonical(x);
+    store_fe(out + pos, pack9(x));
   }
 }
 
@@ -133,7 +145,13 @@ __global__ __launch_bounds__(256) void pow_expand_kernel(const fe* __restrict__ 
                                                          const fe* __restrict__ lo,
                                                          fe* __restrict__ out, uint64_t count) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) store_fe(out + i, pow_lookup(hi, lo, i));
+  if (i < count) store_fe(out + i, pack9(Fr9::canonical(pow_lookup(hi, lo, i))));
+}
+
+// host-built R = 2^256 tables -> R = 2^261 packed form, in place
+__global__ __launch_bounds__(256) void table_to261_kernel(fe* __restrict__ t, uint64_t count) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) store_fe(t + i, pack9(Fr9::canonical(Fr9::to261(unpack9(load_fe(t + i))))));
 }
 
 // n == 1: out[0] = (in_len ? in[0] : 0) * scale
@@ -181,6 +199,9 @@ static int get_pow_tables(Ctx* c, uint32_t log_n, const zkh::Fe& base, const zkh
     if (rc) return rc;
     rc = upload_table(c, l, &bufs.second, st);
     if (rc) return rc;
+    table_to261_kernel<<<grid_for(h.size(), 256), 256, 0, st>>>((fe*)bufs.first.ptr, h.size());
+    table_to261_kernel<<<grid_for(l.size(), 256), 256, 0, st>>>((fe*)bufs.second.ptr, l.size());
+    ZK_HIP(c, hipGetLastError());
     it = c->pow_tables.emplace(key, bufs).first;
   }
   *hi = (const fe*)it->second.first.ptr;
@@ -325,7 +346,7 @@ int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint
     dim3 grid((unsigned)tiles, batch);
     ntt_pass_kernel<<<grid, NTT_THREADS, 0, st>>>(
         s, dst[i], tw, log_n, s0, k, log_t, len, i == 0 ? pre_mode : 0, pre_hi, pre_lo,
-        i == P - 1 ? post_mode : 0, post_hi, post_lo);
+        i == P - 1 ? post_mode : 0, post_hi, post_lo, i == P - 1 ? 1 : 0);
     ZK_HIP(c, hipGetLastError());
     s0 += k;
   }

@@ -110,3 +110,114 @@ def test_xyzz_ops(host):
         assert _call(host, 2, _xyzz_from_affine(None), aq) == Qp              # madd into identity
         k = rng.randrange(0, 1 << 20)
         assert _call(host, 3, sp, None, k) == C.g1_mul_affine(P, k)
+
+
+# ---------------------------------------------------------------- 29-bit limbs
+R261 = 1 << 261
+
+
+@pytest.fixture(scope="module")
+def host29(host):
+    host.zkhost_field29.argtypes = [ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint64]
+    host.zkhost_xyzz29.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_uint32, ct.c_void_p]
+    host.zkhost_wire_roundtrip.argtypes = [ct.c_void_p, ct.c_void_p]
+    return host
+
+
+@pytest.mark.parametrize("field,p", [(0, R_MOD), (1, Q_MOD)])
+def test_field29_ops(host29, field, p):
+    rng = random.Random(100 + field)
+    n = 300
+    # lazy representation: any value in [0, 2p)
+    xs = [rng.randrange(2 * p) for _ in range(n - 8)] + [0, 1, p - 1, p, p + 1, 2 * p - 1, 2 * p - 2, (p + 1) // 2]
+    ys = [rng.randrange(2 * p) for _ in range(n - 8)] + [2 * p - 1, p, p, 1, 0, 2 * p - 1, p - 1, 3]
+    out = (ct.c_uint64 * (4 * n))()
+    rinv = pow(R261, -1, p)
+
+    def run(op, a, b=None):
+        host29.zkhost_field29(field, op, _arr(a), _arr(b) if b is not None else None, out, n)
+        return _vals(out, n)
+
+    for v, x, y in zip(run(0, xs, ys), xs, ys):
+        assert v < 2 * p and v % p == x * y * rinv % p
+    for v, x in zip(run(4, xs), xs):
+        assert v < 2 * p and v % p == x * x * rinv % p
+    for v, x, y in zip(run(1, xs, ys), xs, ys):
+        assert v < 2 * p and v % p == (x + y) % p
+    for v, x, y in zip(run(2, xs, ys), xs, ys):
+        assert v < 2 * p and v % p == (x - y) % p
+    for v, x in zip(run(3, xs), xs):
+        assert v < 2 * p and v % p == (-x) % p
+    assert run(5, xs) == [x % p for x in xs]
+    assert run(8, xs) == [1 if x % p == 0 else 0 for x in xs]
+    canon = [x % p for x in xs]
+    for v, x in zip(run(6, canon), canon):      # x 2^256 -> x 2^261
+        assert v < 2 * p and v % p == x * 32 % p
+    for v, x in zip(run(7, canon), canon):      # x 2^261 -> x 2^256
+        assert v < 2 * p and v % p == x * pow(32, -1, p) % p
+
+
+def _lazy261(x, rng):
+    """x * 2^261 mod q, sometimes in its non-canonical form (+q)."""
+    v = x * R261 % Q_MOD
+    return v + Q_MOD if rng.random() < 0.5 and v + Q_MOD < 2 * Q_MOD else v
+
+
+def _pt29(a, lam, rng):
+    if a is None:
+        return [_lazy261(1, rng), _lazy261(1, rng), 0, 0]
+    x, y = a
+    return [_lazy261(x * lam * lam, rng), _lazy261(y * pow(lam, 3, Q_MOD), rng),
+            _lazy261(lam * lam, rng), _lazy261(pow(lam, 3, Q_MOD), rng)]
+
+
+def _aff29(v):
+    X, Y, ZZ, ZZZ = (x * pow(R261, -1, Q_MOD) % Q_MOD for x in v)
+    assert all(x < 2 * Q_MOD for x in v)
+    if ZZ == 0:
+        return None
+    return (X * pow(ZZ, -1, Q_MOD) % Q_MOD, Y * pow(ZZZ, -1, Q_MOD) % Q_MOD)
+
+
+def _call29(host, op, p, q, k=0):
+    out = (ct.c_uint64 * 16)()
+    host.zkhost_xyzz29(op, _arr(p), _arr(q) if q is not None else None, k, out)
+    return _aff29(_vals(out, 4))
+
+
+def test_xyzz29_ops(host29):
+    rng = random.Random(77)
+    G = C.G1_GENERATOR
+    for _ in range(12):
+        a, b = rng.randrange(1, R_MOD), rng.randrange(1, R_MOD)
+        P, Qp = C.g1_mul_affine(G, a), C.g1_mul_affine(G, b)
+        lp, lq = rng.randrange(2, Q_MOD), rng.randrange(2, Q_MOD)
+        sp, sq = _pt29(P, lp, rng), _pt29(Qp, lq, rng)
+        assert _call29(host29, 0, sp, sq) == C.aff_add(P, Qp)
+        assert _call29(host29, 0, sp, _pt29(P, lq, rng)) == C.aff_add(P, P)
+        assert _call29(host29, 0, sp, _pt29(C.aff_neg(P), lq, rng)) is None
+        assert _call29(host29, 0, sp, _pt29(None, 1, rng)) == P
+        assert _call29(host29, 1, sp, None) == C.aff_add(P, P)
+        aq = [_lazy261(Qp[0], rng), _lazy261(Qp[1], rng), 0, 0]
+        assert _call29(host29, 2, sp, aq) == C.aff_add(P, Qp)
+        ap = [_lazy261(P[0], rng), _lazy261(P[1], rng), 0, 0]
+        assert _call29(host29, 2, sp, ap) == C.aff_add(P, P)
+        an = [_lazy261(P[0], rng), _lazy261((-P[1]) % Q_MOD, rng), 0, 0]
+        assert _call29(host29, 2, sp, an) is None
+        assert _call29(host29, 2, _pt29(None, 1, rng), aq) == Qp
+        k = rng.randrange(0, 1 << 20)
+        assert _call29(host29, 3, sp, None, k) == C.g1_mul_affine(P, k)
+
+
+def test_wire_roundtrip(host29):
+    rng = random.Random(5)
+    for _ in range(20):
+        P = C.g1_mul_affine(C.G1_GENERATOR, rng.randrange(1, R_MOD))
+        xy = [to_mont(P[0], Q_MOD), to_mont(P[1], Q_MOD)]
+        out = (ct.c_uint64 * 16)()
+        host29.zkhost_wire_roundtrip(_arr(xy), out)
+        v = _vals(out, 4)
+        assert v == [to_mont(P[0], Q_MOD), to_mont(P[1], Q_MOD), to_mont(1, Q_MOD), to_mont(1, Q_MOD)]
+    out = (ct.c_uint64 * 16)()
+    host29.zkhost_wire_roundtrip(_arr([0, 0]), out)
+    assert _vals(out, 4)[2] == 0
