@@ -36,6 +36,7 @@ extern "C" {
 #define ZK_ERR_NO_DEVICE 3
 
 typedef struct zk_ctx zk_ctx;
+typedef struct zk_fixed_bases zk_fixed_bases;
 
 /* Library version (major * 10000 + minor * 100 + patch). */
 int zk_version(void);
@@ -120,6 +121,31 @@ int zk_msm_g1(zk_ctx* ctx, const void* bases, const void* scalars, uint64_t n, i
  * the device form of crypto/msm/pippenger.py:252-272's fixed-base table. */
 int zk_g1_fixed_base(zk_ctx* ctx, const void* table, const void* scalars, uint64_t n, void* out,
                      void* stream);
+
+/* Fixed-base sets: a base set used by many MSMs (an SRS, a proving key's
+ * query vector) is expanded once into table[w][i] = 2^(c w) B_i (W windows,
+ * 64 bytes per entry, resident in HBM).  Every MSM over a prefix of the set
+ * then needs no per-window reduction or window combination: all signed
+ * digits of all windows land in one bucket set.  This is the device form of
+ * the reference's shared fixed-base precomputation (FixedBasePrecomp /
+ * fixed_base_precompute / msm_fixed_base(_batch),
+ * crypto/msm/pippenger.py:217-342, runtime/warmup.py:84-150).
+ * window_bits 0: the default rule for n. */
+int zk_fixed_bases_create(zk_ctx* ctx, const void* bases, uint64_t n, int window_bits,
+                          zk_fixed_bases** out, void* stream);
+int zk_fixed_bases_destroy(zk_ctx* ctx, zk_fixed_bases* fb);
+int zk_fixed_bases_info(const zk_fixed_bases* fb, uint64_t* n, int* window_bits);
+
+/* count MSMs over one fixed-base set: MSM j sums scalars[offsets[j] + i] *
+ * B_i for i < sizes[j] (sizes[j] <= n; Montgomery scalars, offsets in
+ * elements).  Results: out_jac_mont[12 j .. 12 j + 12), Jacobian Montgomery.
+ * One launch sequence for the whole batch; counterpart of
+ * msm_fixed_base_batch / msm_pippenger_batch (crypto/msm/pippenger.py:124-214,
+ * 312-342) behind MSM_G1_BATCH and KZG_BATCH_COMMIT
+ * (runtime/kernels/cpu/kernels.py:441-496). */
+int zk_msm_g1_fixed_batch(zk_ctx* ctx, const zk_fixed_bases* fb, const void* scalars,
+                          const uint64_t* offsets, const uint64_t* sizes, uint32_t count,
+                          uint64_t* out_jac_mont, void* stream);
 
 /* Tuning knobs: "ntt_max_pass_bits" (stages per NTT pass, 1..11; 0 = auto),
  * "msm_segment" (pairs per accumulation thread), "msm_reduce_chunk"

@@ -49,6 +49,10 @@ SIGNATURES = {
     "zk_ntt_batch": [_vp, _vp, _vp, ct.c_uint32, ct.c_uint32, ct.c_int, _u64p, _u64p, _vp],
     "zk_msm_g1": [_vp, _vp, _vp, ct.c_uint64, ct.c_int, _u64p, _vp],
     "zk_g1_fixed_base": [_vp, _vp, _vp, ct.c_uint64, _vp, _vp],
+    "zk_fixed_bases_create": [_vp, _vp, ct.c_uint64, ct.c_int, ct.POINTER(_vp), _vp],
+    "zk_fixed_bases_destroy": [_vp, _vp],
+    "zk_fixed_bases_info": [_vp, ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_int)],
+    "zk_msm_g1_fixed_batch": [_vp, _vp, _vp, _u64p, _u64p, ct.c_uint32, _u64p, _vp],
     "zk_set_tuning": [_vp, ct.c_char_p, ct.c_int64],
     "zk_probe_mad_rate": [_vp, ct.POINTER(ct.c_double)],
     "zk_profile_enable": [_vp, ct.c_int],
@@ -115,6 +119,21 @@ class DeviceBuffer:
 
     def offset(self, nbytes: int) -> int:
         return self.ptr + nbytes
+
+
+class FixedBases:
+    """A base set expanded into fixed-base tables on the device."""
+
+    def __init__(self, ctx: "Context", handle, n: int, window_bits: int) -> None:
+        self.ctx = ctx
+        self.handle = handle
+        self.n = n
+        self.window_bits = window_bits
+
+    def free(self) -> None:
+        if self.handle:
+            self.ctx._check(self.ctx.lib.zk_fixed_bases_destroy(self.ctx.handle, self.handle))
+            self.handle = None
 
 
 class Context:
@@ -238,6 +257,26 @@ class Context:
 
     def g1_fixed_base(self, table: int, scalars: int, n: int, out: int, stream=None) -> None:
         self._check(self.lib.zk_g1_fixed_base(self.handle, table, scalars, int(n), out, stream))
+
+    def fixed_bases_create(self, bases: int, n: int, window_bits: int = 0, stream=None) -> "FixedBases":
+        h = _vp()
+        self._check(self.lib.zk_fixed_bases_create(self.handle, bases, int(n), int(window_bits), ct.byref(h),
+                                                   stream))
+        nn = ct.c_uint64(0)
+        cb = ct.c_int(0)
+        self.lib.zk_fixed_bases_info(h, ct.byref(nn), ct.byref(cb))
+        return FixedBases(self, h, nn.value, cb.value)
+
+    def msm_g1_fixed_batch(self, fb: "FixedBases", scalars: int, offsets, sizes, stream=None):
+        """Returns a list of Jacobian Montgomery (X, Y, Z) int triples."""
+        count = len(sizes)
+        off = (ct.c_uint64 * max(count, 1))(*[int(x) for x in offsets])
+        sz = (ct.c_uint64 * max(count, 1))(*[int(x) for x in sizes])
+        out = (ct.c_uint64 * (12 * max(count, 1)))()
+        self._check(self.lib.zk_msm_g1_fixed_batch(self.handle, fb.handle, scalars, off, sz, count, out, stream))
+        w = list(out)
+        f = lambda a: w[a] | (w[a + 1] << 64) | (w[a + 2] << 128) | (w[a + 3] << 192)
+        return [(f(12 * j), f(12 * j + 4), f(12 * j + 8)) for j in range(count)]
 
     # ---- tuning / profiling
     def set_tuning(self, key: str, value: int) -> None:

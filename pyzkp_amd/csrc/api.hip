@@ -314,6 +314,44 @@ int zk_g1_fixed_base(zk_ctx* ctx, const void* table, const void* scalars, uint64
   return g1_fixed_base_run(&ctx->c, (const fe*)table, (const fe*)scalars, n, (fe*)out, pick(ctx, stream));
 }
 
+int zk_fixed_bases_create(zk_ctx* ctx, const void* bases, uint64_t n, int window_bits,
+                          zk_fixed_bases** out, void* stream) {
+  CTX_CHECK(ctx);
+  if (!out) return set_err(&ctx->c, ZK_ERR_INVALID, "null output");
+  *out = nullptr;
+  if (!bases) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  FixedBases* fb = nullptr;
+  int rc = fixed_bases_create(&ctx->c, (const fe*)bases, n, window_bits, &fb, pick(ctx, stream));
+  *out = (zk_fixed_bases*)fb;
+  return rc;
+}
+
+int zk_fixed_bases_destroy(zk_ctx* ctx, zk_fixed_bases* fb) {
+  CTX_CHECK(ctx);
+  hipDeviceSynchronize();
+  fixed_bases_destroy((FixedBases*)fb);
+  return ZK_OK;
+}
+
+int zk_fixed_bases_info(const zk_fixed_bases* fb, uint64_t* n, int* window_bits) {
+  if (!fb) return ZK_ERR_INVALID;
+  const FixedBases* f = (const FixedBases*)fb;
+  if (n) *n = f->n;
+  if (window_bits) *window_bits = f->c;
+  return ZK_OK;
+}
+
+int zk_msm_g1_fixed_batch(zk_ctx* ctx, const zk_fixed_bases* fb, const void* scalars,
+                          const uint64_t* offsets, const uint64_t* sizes, uint32_t count,
+                          uint64_t* out_jac_mont, void* stream) {
+  CTX_CHECK(ctx);
+  if (!fb) return set_err(&ctx->c, ZK_ERR_INVALID, "null fixed-base set");
+  if (count && (!scalars || !offsets || !sizes || !out_jac_mont))
+    return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  return msm_fixed_batch_run(&ctx->c, (const FixedBases*)fb, (const fe*)scalars, offsets, sizes, count,
+                             out_jac_mont, pick(ctx, stream));
+}
+
 int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value) {
   CTX_CHECK(ctx);
   if (!key) return set_err(&ctx->c, ZK_ERR_INVALID, "null key");

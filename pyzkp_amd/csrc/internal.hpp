@@ -42,7 +42,7 @@ struct Ctx {
   // keyed by (log_n, base words, scale words)
   std::map<std::array<uint64_t, 9>, std::pair<DevBuf, DevBuf>> pow_tables;
   // growable scratch arenas
-  DevBuf scratch[4];
+  DevBuf scratch[6];
   // tuning knobs (set through zk_set_tuning)
   int ntt_max_pass_bits = 0;      // 0: planner default
   uint32_t msm_segment = 64;      // sorted (bucket, point) pairs per accumulation thread
@@ -80,6 +80,18 @@ int elementwise(Ctx* c, int op, const fe* a, const fe* b, fe* out, uint64_t n, h
 int elementwise_scale(Ctx* c, const fe* a, const uint64_t c_mont[4], fe* out, uint64_t n, hipStream_t st);
 
 // MSM ------------------------------------------------------------------------
+// A base set prepared for fixed-base MSMs: table[w][i] = 2^(c w) B_i.
+struct FixedBases {
+  uint64_t n = 0;
+  int c = 0;
+  uint32_t W = 0;
+  DevBuf table;
+};
+int fixed_bases_create(Ctx* c, const fe* bases, uint64_t n, int window_bits, FixedBases** out,
+                       hipStream_t st);
+void fixed_bases_destroy(FixedBases* fb);
+int msm_fixed_batch_run(Ctx* c, const FixedBases* fb, const fe* scalars, const uint64_t* offsets,
+                        const uint64_t* sizes, uint32_t count, uint64_t* out_jac, hipStream_t st);
 int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
                uint64_t out_jac[12], hipStream_t st);
 

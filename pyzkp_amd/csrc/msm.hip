@@ -137,30 +137,47 @@ __global__ __launch_bounds__(256) void msm_digits_kernel(const fe* __restrict__ 
   }
 }
 
-// 2. counting sort of the (window, |d|) pairs, one window per grid row.
-//    A tile of SORT_TILE digits of one window is histogrammed in LDS with
-//    16-bit counters (two per word; a tile has at most 2^15 entries so a
+// 2. counting sort of the (row, |d|) pairs.  A row is one window of a plain
+//    MSM, or one whole MSM of a fixed-base batch (all its windows share one
+//    bucket set).  MsmRow describes where a row's digits live and how a
+//    position p inside the row maps to a base index:
+//      base = (p / inner) * stride + p % inner.
+//    A tile of SORT_TILE digits of one row is histogrammed in LDS with
+//    16-bit counters (two per word; a tile has at most 2^15 entries, so a
 //    counter cannot overflow).  Bins are |d| - 1 (zero digits are dropped);
-//    a window with more than SORT_RANGE bins is covered by several bin
-//    ranges (grid z), each re-reading the tile.
-//    tile_hist[(w * ntiles + tile) * nbins + bin] = count
+//    rows with more than SORT_RANGE bins are covered by several bin ranges
+//    (grid z), each re-reading the tile.
+//    tile_hist[(row.tile0 + tile) * nbins + bin] = count
 static constexpr uint32_t SORT_TILE = 1u << 15;
 static constexpr uint32_t SORT_RANGE = 1u << 15;  // bins per LDS pass (64 KiB)
 static constexpr uint32_t SORT_THREADS = 1024;
 
-__global__ __launch_bounds__(SORT_THREADS) void msm_tile_count_kernel(
-    const uint32_t* __restrict__ digits, uint64_t n, uint32_t nbins, uint32_t ntiles,
-    uint32_t* __restrict__ tile_hist) {
+struct MsmRow {
+  uint64_t off;     // first digit of the row
+  uint32_t len;     // digits in the row
+  uint32_t inner;   // digits per window inside the row
+  uint32_t stride;  // base-index stride between windows
+  uint32_t ntiles;  // ceil(len / SORT_TILE)
+  uint32_t tile0;   // first tile slot of the row in the tile histogram
+  uint32_t pad;
+};
+
+__global__ __launch_bounds__(SORT_THREADS) void msm_tile_count_kernel(const uint32_t* __restrict__ digits,
+                                                                      const MsmRow* __restrict__ rows,
+                                                                      uint32_t nbins,
+                                                                      uint32_t* __restrict__ tile_hist) {
   __shared__ uint32_t cnt[SORT_RANGE / 2];
-  const uint32_t tile = blockIdx.x, w = blockIdx.y;
+  const MsmRow row = rows[blockIdx.y];
+  const uint32_t tile = blockIdx.x;
+  if (tile >= row.ntiles) return;
   const uint32_t lo = blockIdx.z * SORT_RANGE;
   const uint32_t hi = (lo + SORT_RANGE < nbins) ? lo + SORT_RANGE : nbins;
   for (uint32_t k = threadIdx.x; k < SORT_RANGE / 2; k += SORT_THREADS) cnt[k] = 0;
   __syncthreads();
-  const uint64_t base = (uint64_t)w * n + (uint64_t)tile * SORT_TILE;
-  const uint64_t end = (uint64_t)w * n + (((uint64_t)tile + 1) * SORT_TILE < n ? ((uint64_t)tile + 1) * SORT_TILE : n);
-  for (uint64_t p = base + threadIdx.x; p < end; p += SORT_THREADS) {
-    const uint32_t mag = digits[p] & ~SIGN;
+  const uint32_t p0 = tile * SORT_TILE;
+  const uint32_t p1 = (p0 + SORT_TILE < row.len) ? p0 + SORT_TILE : row.len;
+  for (uint32_t p = p0 + threadIdx.x; p < p1; p += SORT_THREADS) {
+    const uint32_t mag = digits[row.off + p] & ~SIGN;
     if (mag == 0) continue;
     const uint32_t bin = mag - 1;
     if (bin < lo || bin >= hi) continue;
@@ -168,31 +185,32 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_tile_count_kernel(
     atomicAdd(&cnt[b >> 1], 1u << ((b & 1) * 16));
   }
   __syncthreads();
-  uint32_t* out = tile_hist + ((uint64_t)w * ntiles + tile) * nbins;
+  uint32_t* out = tile_hist + (uint64_t)(row.tile0 + tile) * nbins;
   for (uint32_t b = lo + threadIdx.x; b < hi; b += SORT_THREADS) {
     const uint32_t k = b - lo;
     out[b] = (cnt[k >> 1] >> ((k & 1) * 16)) & 0xffffu;
   }
 }
 
-// exclusive prefix of each (window, bin) column over the tiles, in place;
-// the column total becomes counts[w * B + bin + 1] (bucket |d| = bin + 1)
+// exclusive prefix of each (row, bin) column over the row's tiles, in place;
+// the column total becomes counts[row * B + bin + 1] (bucket |d| = bin + 1)
 __global__ __launch_bounds__(256) void msm_tile_prefix_kernel(uint32_t* __restrict__ tile_hist,
-                                                              uint32_t W, uint32_t nbins,
-                                                              uint32_t ntiles, uint32_t B,
+                                                              const MsmRow* __restrict__ rows,
+                                                              uint32_t nrows, uint32_t nbins, uint32_t B,
                                                               uint32_t* __restrict__ counts) {
   const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (uint64_t)W * nbins) return;
-  const uint32_t w = (uint32_t)(gid / nbins), bin = (uint32_t)(gid - (uint64_t)w * nbins);
+  if (gid >= (uint64_t)nrows * nbins) return;
+  const uint32_t r = (uint32_t)(gid / nbins), bin = (uint32_t)(gid - (uint64_t)r * nbins);
+  const MsmRow row = rows[r];
   uint32_t run = 0;
-  uint32_t* col = tile_hist + (uint64_t)w * ntiles * nbins + bin;
-  for (uint32_t t = 0; t < ntiles; t++) {
+  uint32_t* col = tile_hist + (uint64_t)row.tile0 * nbins + bin;
+  for (uint32_t t = 0; t < row.ntiles; t++) {
     const uint32_t v = col[(uint64_t)t * nbins];
     col[(uint64_t)t * nbins] = run;
     run += v;
   }
-  counts[w * B + bin + 1] = run;
-  if (bin == 0) counts[w * B] = 0;
+  counts[r * B + bin + 1] = run;
+  if (bin == 0) counts[r * B] = 0;
 }
 
 // 2. exclusive scan of the bucket counts -> row pointers (3 small kernels)
@@ -274,24 +292,26 @@ __global__ __launch_bounds__(SCAN_THREADS) void scan_add_kernel(uint32_t* __rest
   }
 }
 
-// 3. scatter: sorted[row_ptr[w B + |d|] + tile prefix + rank in tile] =
-//    point index | sign; ranks come from 16-bit LDS counters as in the count pass
+// 3. scatter: sorted[row_ptr[row B + |d|] + tile prefix + rank in tile] =
+//    base index | sign; ranks come from 16-bit LDS counters as in the count pass
 __global__ __launch_bounds__(SORT_THREADS) void msm_tile_scatter_kernel(
-    const uint32_t* __restrict__ digits, uint64_t n, uint32_t nbins, uint32_t ntiles, uint32_t B,
+    const uint32_t* __restrict__ digits, const MsmRow* __restrict__ rows, uint32_t nbins, uint32_t B,
     const uint32_t* __restrict__ tile_hist, const uint32_t* __restrict__ row_ptr,
     uint32_t* __restrict__ sorted) {
   __shared__ uint32_t cnt[SORT_RANGE / 2];
-  const uint32_t tile = blockIdx.x, w = blockIdx.y;
+  const MsmRow row = rows[blockIdx.y];
+  const uint32_t tile = blockIdx.x;
+  if (tile >= row.ntiles) return;
   const uint32_t lo = blockIdx.z * SORT_RANGE;
   const uint32_t hi = (lo + SORT_RANGE < nbins) ? lo + SORT_RANGE : nbins;
   for (uint32_t k = threadIdx.x; k < SORT_RANGE / 2; k += SORT_THREADS) cnt[k] = 0;
   __syncthreads();
-  const uint32_t* pre = tile_hist + ((uint64_t)w * ntiles + tile) * nbins;
-  const uint32_t* rp = row_ptr + (uint64_t)w * B + 1;
-  const uint64_t i0 = (uint64_t)tile * SORT_TILE;
-  const uint64_t i1 = (i0 + SORT_TILE < n) ? i0 + SORT_TILE : n;
-  for (uint64_t i = i0 + threadIdx.x; i < i1; i += SORT_THREADS) {
-    const uint32_t d = digits[(uint64_t)w * n + i];
+  const uint32_t* pre = tile_hist + (uint64_t)(row.tile0 + tile) * nbins;
+  const uint32_t* rp = row_ptr + (uint64_t)blockIdx.y * B + 1;
+  const uint32_t p0 = tile * SORT_TILE;
+  const uint32_t p1 = (p0 + SORT_TILE < row.len) ? p0 + SORT_TILE : row.len;
+  for (uint32_t p = p0 + threadIdx.x; p < p1; p += SORT_THREADS) {
+    const uint32_t d = digits[row.off + p];
     const uint32_t mag = d & ~SIGN;
     if (mag == 0) continue;
     const uint32_t bin = mag - 1;
@@ -299,7 +319,9 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_tile_scatter_kernel(
     const uint32_t b = bin - lo;
     const uint32_t sh = (b & 1) * 16;
     const uint32_t rank = (atomicAdd(&cnt[b >> 1], 1u << sh) >> sh) & 0xffffu;
-    sorted[rp[bin] + pre[bin] + rank] = (uint32_t)i | (d & SIGN);
+    const uint32_t wi = p / row.inner;
+    const uint32_t base = wi * row.stride + (p - wi * row.inner);
+    sorted[rp[bin] + pre[bin] + rank] = base | (d & SIGN);
   }
 }
 
@@ -455,6 +477,89 @@ __global__ __launch_bounds__(128) void msm_reduce_windows_kernel(const xyzz* __r
   if (tid == 0) store_xyzz9(window_sums + w, sh[0]);
 }
 
+// 8. fixed-base batches: the tables and the digit split.
+//    Table row w holds 2^(c w) * B_i (affine, R = 2^261 packed), so every
+//    signed digit d_(i,w) of any scalar maps to the same bucket set:
+//      sum_i s_i B_i = sum_i sum_w d_(i,w) (2^(c w) B_i).
+__device__ __forceinline__ f9 fq9_inv(const f9& a) {
+  // a^(q-2), q - 2 as little-endian 32-bit words
+  const uint32_t e[8] = {0xd87cfd45u, 0x3c208c16u, 0x6871ca8du, 0x97816a91u,
+                         0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};
+  f9 r = Fq9::one();
+  for (int wi = 7; wi >= 0; wi--) {
+    const uint32_t word = e[wi];
+    for (int b = 31; b >= 0; b--) {
+      r = Fq9::sqr(r);
+      if ((word >> b) & 1u) r = Fq9::mul(r, a);
+    }
+  }
+  return r;
+}
+
+__global__ __launch_bounds__(128) void msm_fixed_table_kernel(const fe* __restrict__ bases9, uint64_t n,
+                                                              uint32_t c, uint32_t W,
+                                                              fe* __restrict__ table) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  xyzz9 acc = xyzz9_from_aff(load_aff9(bases9, (uint32_t)i));
+  for (uint32_t w = 0; w < W; w++) {
+    f9 x = Fq9::zero(), y = Fq9::zero();
+    if (!xyzz9_is_inf(acc)) {
+      const f9 inv = fq9_inv(Fq9::mul(acc.zz, acc.zzz));
+      x = Fq9::canonical(Fq9::mul(acc.x, Fq9::mul(inv, acc.zzz)));
+      y = Fq9::canonical(Fq9::mul(acc.y, Fq9::mul(inv, acc.zz)));
+    }
+    fe* t = table + 2 * ((uint64_t)w * n + i);
+    store_fe(t, pack9(x));
+    store_fe(t + 1, pack9(y));
+    if (w + 1 < W)
+      for (uint32_t d = 0; d < c; d++) acc = xyzz9_dbl(acc);
+  }
+}
+
+// digits of MSM j (scalars at soff[j], count sizes[j]) into row j:
+// digits[row_off[j] + w * sizes[j] + i]
+__global__ __launch_bounds__(256) void msm_digits_fixed_kernel(const fe* __restrict__ scalars,
+                                                               const MsmRow* __restrict__ rows,
+                                                               const uint64_t* __restrict__ soff,
+                                                               uint32_t c, uint32_t W,
+                                                               uint32_t* __restrict__ digits) {
+  const MsmRow row = rows[blockIdx.y];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= row.inner) return;
+  const fe s = Fr::from_mont(load_fe(scalars + soff[blockIdx.y] + i));
+  uint64_t buf = (uint64_t)s.w[0] | ((uint64_t)s.w[1] << 32);
+  int have = 64;
+  uint32_t nx[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) nx[k] = s.w[k + 2];
+  const uint32_t mask = (1u << c) - 1;
+  const uint32_t half = 1u << (c - 1);
+  uint32_t carry = 0;
+  for (uint32_t w = 0; w < W; w++) {
+    if (have < (int)c) {
+      buf |= (uint64_t)nx[0] << have;
+      have += 32;
+#pragma unroll
+      for (int k = 0; k < 5; k++) nx[k] = nx[k + 1];
+      nx[5] = 0;
+    }
+    const uint32_t val = (uint32_t)(buf & mask) + carry;
+    buf >>= c;
+    have -= (int)c;
+    uint32_t out;
+    if (val >= half) {
+      const uint32_t mag = (1u << c) - val;
+      carry = 1;
+      out = mag ? (mag | SIGN) : 0u;
+    } else {
+      carry = 0;
+      out = val;
+    }
+    digits[row.off + (uint64_t)w * row.inner + i] = out;
+  }
+}
+
 // ---------------------------------------------------------------- host side
 
 // c = max(4, min(16, bitlen(n) - 2))   (runtime/metal/msm.py:191,
@@ -470,6 +575,145 @@ static int default_window_bits(uint64_t n) {
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Shape of one pipeline launch sequence over `nrows` rows of digits.
+struct MsmShape {
+  int c = 0;
+  uint32_t W = 0;       // windows per scalar
+  uint32_t B = 0;       // buckets per row (incl. the empty bucket 0)
+  uint32_t nrows = 0;
+  uint32_t nkeys = 0;   // nrows * B
+  uint32_t nbins = 0;   // B - 1
+  uint32_t nranges = 0;
+  uint32_t max_tiles = 0;
+  uint32_t tile_slots = 0;
+  uint64_t pairs = 0;   // digits over all rows
+  uint64_t Smax = 0;    // accumulation segments
+  uint32_t L = 0, K = 0, cpw = 0;
+  uint32_t nchunks = 0;
+};
+
+struct MsmBuffers {
+  uint32_t *digits, *hist, *counts, *row_ptr, *sums, *sorted, *hkey;
+  xyzz *buckets, *head, *partial, *rsum, *rexp;
+  MsmRow* rows;
+};
+
+static int make_shape(Ctx* c, int cb, uint32_t nrows, const std::vector<MsmRow>& rows, MsmShape& sh) {
+  if (cb < 2 || cb > 24) return set_err(c, ZK_ERR_INVALID, "window_bits must be in [2, 24]");
+  sh.c = cb;
+  sh.W = (255 + cb - 1) / cb;
+  sh.B = (1u << (cb - 1)) + 1;
+  sh.nrows = nrows;
+  sh.nkeys = nrows * sh.B;
+  sh.nbins = sh.B - 1;
+  sh.nranges = (sh.nbins + SORT_RANGE - 1) / SORT_RANGE;
+  sh.max_tiles = 0;
+  sh.tile_slots = 0;
+  sh.pairs = 0;
+  for (const MsmRow& r : rows) {
+    sh.max_tiles = r.ntiles > sh.max_tiles ? r.ntiles : sh.max_tiles;
+    sh.tile_slots += r.ntiles;
+    sh.pairs += r.len;
+  }
+  if (sh.pairs >= (1ull << 32)) return set_err(c, ZK_ERR_INVALID, "total digit count must be below 2^32");
+  sh.L = c->msm_segment;
+  sh.Smax = (sh.pairs + sh.L - 1) / sh.L;
+  if (sh.Smax == 0) sh.Smax = 1;
+  uint32_t K = c->msm_reduce_chunk;
+  if (K == 0) {
+    const uint64_t per = ((uint64_t)nrows * sh.nbins) / 65536;
+    K = per < 2 ? 2 : (uint32_t)per;
+  }
+  if (K > sh.nbins) K = sh.nbins;
+  sh.K = K;
+  sh.cpw = (sh.nbins + K - 1) / K;
+  sh.nchunks = (sh.nkeys + SCAN_CHUNK - 1) / SCAN_CHUNK;
+  if (sh.nchunks > SCAN_CHUNK) return set_err(c, ZK_ERR_INVALID, "too many buckets for the scan");
+  return ZK_OK;
+}
+
+static size_t layout(const MsmShape& sh, char* base, MsmBuffers* b) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off = align256(off + bytes);
+    return base ? (void*)(base + o) : nullptr;
+  };
+  MsmBuffers t;
+  t.digits = (uint32_t*)take(sh.pairs * 4);
+  t.hist = (uint32_t*)take((size_t)sh.tile_slots * sh.nbins * 4);
+  t.counts = (uint32_t*)take((size_t)(sh.nkeys + 1) * 4);
+  t.row_ptr = (uint32_t*)take((size_t)(sh.nkeys + 1) * 4);
+  t.sums = (uint32_t*)take((size_t)sh.nchunks * 4);
+  t.sorted = (uint32_t*)take(sh.pairs * 4);
+  t.buckets = (xyzz*)take((size_t)sh.nkeys * sizeof(xyzz));
+  t.hkey = (uint32_t*)take(sh.Smax * 4);
+  t.head = (xyzz*)take(sh.Smax * sizeof(xyzz));
+  t.partial = (xyzz*)take((size_t)sh.nrows * sh.cpw * sizeof(xyzz));
+  t.rsum = (xyzz*)take((size_t)sh.nrows * sizeof(xyzz));
+  t.rexp = (xyzz*)take((size_t)sh.nrows * sizeof(xyzz));
+  t.rows = (MsmRow*)take((size_t)sh.nrows * sizeof(MsmRow));
+  if (b) *b = t;
+  return off;
+}
+
+// sort -> accumulate -> fix-up -> per-row reduction -> export (R = 2^256 XYZZ
+// per row in b.rexp).  No allocation, no host synchronisation.
+static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* bases9, hipStream_t st) {
+  hipEvent_t ev;
+  phase_begin(c, "msm_sort", st, &ev);
+  {
+    dim3 g(sh.max_tiles, sh.nrows, sh.nranges);
+    msm_tile_count_kernel<<<g, SORT_THREADS, 0, st>>>(b.digits, b.rows, sh.nbins, b.hist);
+    msm_tile_prefix_kernel<<<grid_for((uint64_t)sh.nrows * sh.nbins, 256), 256, 0, st>>>(
+        b.hist, b.rows, sh.nrows, sh.nbins, sh.B, b.counts);
+    scan_chunks_kernel<<<sh.nchunks, SCAN_THREADS, 0, st>>>(b.counts, b.row_ptr, b.sums, sh.nkeys);
+    scan_sums_kernel<<<1, SCAN_THREADS, 0, st>>>(b.sums, sh.nchunks, b.row_ptr + sh.nkeys);
+    scan_add_kernel<<<sh.nchunks, SCAN_THREADS, 0, st>>>(b.row_ptr, b.sums, sh.nkeys);
+    msm_tile_scatter_kernel<<<g, SORT_THREADS, 0, st>>>(b.digits, b.rows, sh.nbins, sh.B, b.hist, b.row_ptr,
+                                                         b.sorted);
+  }
+  ZK_HIP(c, hipGetLastError());
+  phase_end(c, "msm_sort", st, ev);
+
+  phase_begin(c, "msm_accumulate", st, &ev);
+  msm_accumulate_kernel<<<grid_for(sh.Smax, 256), 256, 0, st>>>(bases9, b.sorted, b.row_ptr, sh.nkeys, sh.L,
+                                                               b.buckets, b.hkey, b.head);
+  ZK_HIP(c, hipGetLastError());
+  phase_end(c, "msm_accumulate", st, ev);
+
+  phase_begin(c, "msm_fixup", st, &ev);
+  for (uint64_t stride = 1; stride < sh.Smax; stride *= FIXUP_FAN)
+    msm_fixup_round_kernel<<<grid_for(sh.Smax, 256), 256, 0, st>>>(b.row_ptr, sh.nkeys, sh.L, stride, b.hkey,
+                                                                  b.head);
+  msm_fixup_final_kernel<<<grid_for(sh.Smax, 256), 256, 0, st>>>(b.row_ptr, sh.nkeys, sh.L, b.buckets, b.hkey,
+                                                                b.head);
+  ZK_HIP(c, hipGetLastError());
+  phase_end(c, "msm_fixup", st, ev);
+
+  phase_begin(c, "msm_reduce", st, &ev);
+  msm_reduce_chunks_kernel<<<grid_for((uint64_t)sh.nrows * sh.cpw, 128), 128, 0, st>>>(
+      b.buckets, b.row_ptr, sh.nrows, sh.B, sh.K, sh.cpw, b.partial);
+  msm_reduce_windows_kernel<<<sh.nrows, 128, 0, st>>>(b.partial, sh.cpw, b.rsum);
+  msm_wsum_export_kernel<<<grid_for(sh.nrows, 64), 64, 0, st>>>(b.rsum, sh.nrows, b.rexp);
+  ZK_HIP(c, hipGetLastError());
+  phase_end(c, "msm_reduce", st, ev);
+  return ZK_OK;
+}
+
+// XYZZ (R = 2^256) -> Jacobian (X ZZ^2, Y ZZZ^2, ZZZ) words; zeros for infinity
+static void export_jacobian(const xyzz& acc, uint64_t out[12]) {
+  for (int i = 0; i < 12; i++) out[i] = 0;
+  if (xyzz_is_inf(acc)) return;
+  fe zz2 = Fq::sqr(acc.zz), zzz2 = Fq::sqr(acc.zzz);
+  fe X = Fq::mul(acc.x, zz2), Y = Fq::mul(acc.y, zzz2), Z = acc.zzz;
+  for (int i = 0; i < 4; i++) {
+    out[i] = (uint64_t)X.w[2 * i] | ((uint64_t)X.w[2 * i + 1] << 32);
+    out[4 + i] = (uint64_t)Y.w[2 * i] | ((uint64_t)Y.w[2 * i + 1] << 32);
+    out[8 + i] = (uint64_t)Z.w[2 * i] | ((uint64_t)Z.w[2 * i + 1] << 32);
+  }
+}
+
 int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
                uint64_t out_jac[12], hipStream_t st) {
   for (int i = 0; i < 12; i++) out_jac[i] = 0;
@@ -478,119 +722,157 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   const int cb = window_bits > 0 ? window_bits : default_window_bits(n);
   if (cb < 2 || cb > 24) return set_err(c, ZK_ERR_INVALID, "window_bits must be in [2, 24]");
   const uint32_t W = (255 + cb - 1) / cb;
-  const uint32_t B = (1u << (cb - 1)) + 1;
-  const uint32_t nkeys = W * B;
-  const uint32_t L = c->msm_segment;
-  uint32_t K = c->msm_reduce_chunk;
-  if (K == 0) {
-    // enough reduction threads to fill the device (~64K), at least 2 buckets each
-    const uint64_t per = ((uint64_t)W * (B - 1)) / 65536;
-    K = per < 2 ? 2 : (uint32_t)per;
+  // rows = windows: row w holds digits[w n .. w n + n), base index = position
+  std::vector<MsmRow> rows(W);
+  const uint32_t nt = (uint32_t)((n + SORT_TILE - 1) / SORT_TILE);
+  for (uint32_t w = 0; w < W; w++) {
+    rows[w].off = (uint64_t)w * n;
+    rows[w].len = (uint32_t)n;
+    rows[w].inner = (uint32_t)n;
+    rows[w].stride = 0;
+    rows[w].ntiles = nt;
+    rows[w].tile0 = w * nt;
+    rows[w].pad = 0;
   }
-  if (K > B - 1) K = B - 1;
-  const uint32_t cpw = (B - 1 + K - 1) / K;
-  const uint64_t pairs = n * W;
-  if (pairs >= (1ull << 32)) return set_err(c, ZK_ERR_INVALID, "n * window count must be below 2^32");
-  const uint64_t Smax = (pairs + L - 1) / L;
-  const uint32_t nchunks = (nkeys + SCAN_CHUNK - 1) / SCAN_CHUNK;
-  if (nchunks > SCAN_CHUNK) return set_err(c, ZK_ERR_INVALID, "too many buckets for the scan");
-  const uint32_t nbins = B - 1;  // |d| in 1..2^(c-1)
-  const uint32_t ntiles = (uint32_t)((n + SORT_TILE - 1) / SORT_TILE);
-  const uint32_t nranges = (nbins + SORT_RANGE - 1) / SORT_RANGE;
-
-  // scratch layout
-  size_t off = 0;
-  const size_t o_digits = off; off = align256(off + pairs * 4);
-  const size_t o_hist = off; off = align256(off + (size_t)W * ntiles * nbins * 4);
-  const size_t o_counts = off; off = align256(off + (size_t)(nkeys + 1) * 4);
-  const size_t o_rowptr = off; off = align256(off + (size_t)(nkeys + 1) * 4);
-  const size_t o_sums = off; off = align256(off + (size_t)nchunks * 4);
-  const size_t o_sorted = off; off = align256(off + pairs * 4);
-  const size_t o_buckets = off; off = align256(off + (size_t)nkeys * sizeof(xyzz));
-  const size_t o_hkey = off; off = align256(off + Smax * 4);
-  const size_t o_head = off; off = align256(off + Smax * sizeof(xyzz));
-  const size_t o_partial = off; off = align256(off + (size_t)W * cpw * sizeof(xyzz));
-  const size_t o_wsum = off; off = align256(off + (size_t)W * sizeof(xyzz));
-  const size_t o_wexp = off; off = align256(off + (size_t)W * sizeof(xyzz));
-  const size_t o_bases = off; off = align256(off + (size_t)n * 64);
-  void* base;
-  int rc = ensure_scratch(c, 2, off, &base);
+  MsmShape sh;
+  int rc = make_shape(c, cb, W, rows, sh);
   if (rc) return rc;
-  char* b8 = (char*)base;
-  uint32_t* digits = (uint32_t*)(b8 + o_digits);
-  uint32_t* hist = (uint32_t*)(b8 + o_hist);
-  uint32_t* counts = (uint32_t*)(b8 + o_counts);
-  uint32_t* row_ptr = (uint32_t*)(b8 + o_rowptr);
-  uint32_t* sums = (uint32_t*)(b8 + o_sums);
-  uint32_t* sorted = (uint32_t*)(b8 + o_sorted);
-  xyzz* buckets = (xyzz*)(b8 + o_buckets);
-  uint32_t* hkey = (uint32_t*)(b8 + o_hkey);
-  xyzz* head = (xyzz*)(b8 + o_head);
-  xyzz* partial = (xyzz*)(b8 + o_partial);
-  xyzz* wsum = (xyzz*)(b8 + o_wsum);
-  xyzz* wexp = (xyzz*)(b8 + o_wexp);
-  fe* bases9 = (fe*)(b8 + o_bases);
+  const size_t bytes = layout(sh, nullptr, nullptr);
+  const size_t o_bases = align256(bytes);
+  void* base;
+  rc = ensure_scratch(c, 2, o_bases + (size_t)n * 64, &base);
+  if (rc) return rc;
+  MsmBuffers b;
+  layout(sh, (char*)base, &b);
+  fe* bases9 = (fe*)((char*)base + o_bases);
+  ZK_HIP(c, hipMemcpyAsync(b.rows, rows.data(), rows.size() * sizeof(MsmRow), hipMemcpyHostToDevice, st));
 
   hipEvent_t ev;
   phase_begin(c, "msm_digits", st, &ev);
   msm_bases_convert_kernel<<<grid_for(n, 256), 256, 0, st>>>(bases, n, bases9);
-  msm_digits_kernel<<<grid_for(n, 256), 256, 0, st>>>(scalars, n, (uint32_t)cb, W, digits);
+  msm_digits_kernel<<<grid_for(n, 256), 256, 0, st>>>(scalars, n, (uint32_t)cb, W, b.digits);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_digits", st, ev);
 
-  phase_begin(c, "msm_sort", st, &ev);
-  {
-    dim3 g(ntiles, W, nranges);
-    msm_tile_count_kernel<<<g, SORT_THREADS, 0, st>>>(digits, n, nbins, ntiles, hist);
-    msm_tile_prefix_kernel<<<grid_for((uint64_t)W * nbins, 256), 256, 0, st>>>(hist, W, nbins, ntiles, B,
-                                                                               counts);
-    scan_chunks_kernel<<<nchunks, SCAN_THREADS, 0, st>>>(counts, row_ptr, sums, nkeys);
-    scan_sums_kernel<<<1, SCAN_THREADS, 0, st>>>(sums, nchunks, row_ptr + nkeys);
-    scan_add_kernel<<<nchunks, SCAN_THREADS, 0, st>>>(row_ptr, sums, nkeys);
-    msm_tile_scatter_kernel<<<g, SORT_THREADS, 0, st>>>(digits, n, nbins, ntiles, B, hist, row_ptr, sorted);
-  }
-  ZK_HIP(c, hipGetLastError());
-  phase_end(c, "msm_sort", st, ev);
-
-  phase_begin(c, "msm_accumulate", st, &ev);
-  msm_accumulate_kernel<<<grid_for(Smax, 256), 256, 0, st>>>(bases9, sorted, row_ptr, nkeys, L, buckets,
-                                                            hkey, head);
-  ZK_HIP(c, hipGetLastError());
-  phase_end(c, "msm_accumulate", st, ev);
-
-  phase_begin(c, "msm_fixup", st, &ev);
-  for (uint64_t stride = 1; stride < Smax; stride *= FIXUP_FAN)
-    msm_fixup_round_kernel<<<grid_for(Smax, 256), 256, 0, st>>>(row_ptr, nkeys, L, stride, hkey, head);
-  msm_fixup_final_kernel<<<grid_for(Smax, 256), 256, 0, st>>>(row_ptr, nkeys, L, buckets, hkey, head);
-  ZK_HIP(c, hipGetLastError());
-  phase_end(c, "msm_fixup", st, ev);
-
-  phase_begin(c, "msm_reduce", st, &ev);
-  msm_reduce_chunks_kernel<<<grid_for((uint64_t)W * cpw, 128), 128, 0, st>>>(buckets, row_ptr, W, B, K,
-                                                                           cpw, partial);
-  msm_reduce_windows_kernel<<<W, 128, 0, st>>>(partial, cpw, wsum);
-  msm_wsum_export_kernel<<<grid_for(W, 64), 64, 0, st>>>(wsum, W, wexp);
-  ZK_HIP(c, hipGetLastError());
-  phase_end(c, "msm_reduce", st, ev);
+  rc = run_rows(c, sh, b, bases9, st);
+  if (rc) return rc;
 
   // window combination on the host: acc = Sum_w 2^(c w) S_w (Horner)
   std::vector<xyzz> hs(W);
-  ZK_HIP(c, hipMemcpyAsync(hs.data(), wexp, (size_t)W * sizeof(xyzz), hipMemcpyDeviceToHost, st));
+  ZK_HIP(c, hipMemcpyAsync(hs.data(), b.rexp, (size_t)W * sizeof(xyzz), hipMemcpyDeviceToHost, st));
   ZK_HIP(c, hipStreamSynchronize(st));
   xyzz acc = hs[W - 1];
   for (int w = (int)W - 2; w >= 0; w--) {
     for (int d = 0; d < cb; d++) acc = xyzz_dbl(acc);
     acc = xyzz_add(acc, hs[w]);
   }
-  if (!xyzz_is_inf(acc)) {
-    // Jacobian (X ZZ^2, Y ZZZ^2, ZZZ) represents (X/ZZ, Y/ZZZ)
-    fe zz2 = Fq::sqr(acc.zz), zzz2 = Fq::sqr(acc.zzz);
-    fe X = Fq::mul(acc.x, zz2), Y = Fq::mul(acc.y, zzz2), Z = acc.zzz;
-    for (int i = 0; i < 4; i++) {
-      out_jac[i] = (uint64_t)X.w[2 * i] | ((uint64_t)X.w[2 * i + 1] << 32);
-      out_jac[4 + i] = (uint64_t)Y.w[2 * i] | ((uint64_t)Y.w[2 * i + 1] << 32);
-      out_jac[8 + i] = (uint64_t)Z.w[2 * i] | ((uint64_t)Z.w[2 * i + 1] << 32);
-    }
+  export_jacobian(acc, out_jac);
+  return ZK_OK;
+}
+
+// ------------------------------------------------------------ fixed-base sets
+
+int fixed_bases_create(Ctx* c, const fe* bases, uint64_t n, int window_bits, FixedBases** out,
+                       hipStream_t st) {
+  *out = nullptr;
+  if (n == 0 || n >= (1ull << 31)) return set_err(c, ZK_ERR_INVALID, "fixed-base set size must be in [1, 2^31)");
+  const int cb = window_bits > 0 ? window_bits : default_window_bits(n);
+  if (cb < 2 || cb > 24) return set_err(c, ZK_ERR_INVALID, "window_bits must be in [2, 24]");
+  const uint32_t W = (255 + cb - 1) / cb;
+  if ((uint64_t)W * n >= (1ull << 31)) return set_err(c, ZK_ERR_INVALID, "fixed-base table too large");
+  FixedBases* fb = new FixedBases();
+  fb->n = n;
+  fb->c = cb;
+  fb->W = W;
+  fb->table.bytes = (size_t)W * n * 64;
+  hipError_t e = hipMalloc(&fb->table.ptr, fb->table.bytes);
+  if (e != hipSuccess) {
+    delete fb;
+    return hip_check(c, e, "hipMalloc(fixed-base table)");
+  }
+  void* tmp;
+  int rc = ensure_scratch(c, 3, (size_t)n * 64, &tmp);
+  if (rc) {
+    hipFree(fb->table.ptr);
+    delete fb;
+    return rc;
+  }
+  msm_bases_convert_kernel<<<grid_for(n, 256), 256, 0, st>>>(bases, n, (fe*)tmp);
+  msm_fixed_table_kernel<<<grid_for(n, 128), 128, 0, st>>>((const fe*)tmp, n, (uint32_t)cb, W,
+                                                          (fe*)fb->table.ptr);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    hipFree(fb->table.ptr);
+    delete fb;
+    return hip_check(c, e, "fixed-base table build");
+  }
+  *out = fb;
+  return ZK_OK;
+}
+
+void fixed_bases_destroy(FixedBases* fb) {
+  if (!fb) return;
+  if (fb->table.ptr) hipFree(fb->table.ptr);
+  delete fb;
+}
+
+// MSM j = sum_{i < sizes[j]} scalars[offsets[j] + i] * B_i over one fixed set.
+int msm_fixed_batch_run(Ctx* c, const FixedBases* fb, const fe* scalars, const uint64_t* offsets,
+                        const uint64_t* sizes, uint32_t count, uint64_t* out_jac, hipStream_t st) {
+  for (uint64_t i = 0; i < 12ull * count; i++) out_jac[i] = 0;
+  if (count == 0) return ZK_OK;
+  std::vector<MsmRow> rows(count);
+  std::vector<uint64_t> soff(count);
+  uint64_t off = 0;
+  uint32_t tiles = 0, max_inner = 0;
+  for (uint32_t j = 0; j < count; j++) {
+    if (sizes[j] > fb->n) return set_err(c, ZK_ERR_INVALID, "scalars length exceeds points length");
+    const uint64_t len = sizes[j] * fb->W;
+    rows[j].off = off;
+    rows[j].len = (uint32_t)len;
+    rows[j].inner = (uint32_t)sizes[j];
+    rows[j].stride = (uint32_t)fb->n;
+    rows[j].ntiles = (uint32_t)((len + SORT_TILE - 1) / SORT_TILE);
+    rows[j].tile0 = tiles;
+    rows[j].pad = 0;
+    tiles += rows[j].ntiles;
+    off += len;
+    soff[j] = offsets[j];
+    if (sizes[j] > max_inner) max_inner = (uint32_t)sizes[j];
+  }
+  if (off >= (1ull << 32)) return set_err(c, ZK_ERR_INVALID, "total digit count must be below 2^32");
+  MsmShape sh;
+  int rc = make_shape(c, fb->c, count, rows, sh);
+  if (rc) return rc;
+  const size_t bytes = layout(sh, nullptr, nullptr);
+  const size_t o_soff = align256(bytes);
+  void* base;
+  rc = ensure_scratch(c, 2, o_soff + (size_t)count * 8, &base);
+  if (rc) return rc;
+  MsmBuffers b;
+  layout(sh, (char*)base, &b);
+  uint64_t* d_soff = (uint64_t*)((char*)base + o_soff);
+  ZK_HIP(c, hipMemcpyAsync(b.rows, rows.data(), rows.size() * sizeof(MsmRow), hipMemcpyHostToDevice, st));
+  ZK_HIP(c, hipMemcpyAsync(d_soff, soff.data(), soff.size() * 8, hipMemcpyHostToDevice, st));
+
+  if (max_inner > 0) {
+    hipEvent_t ev;
+    phase_begin(c, "msm_digits", st, &ev);
+    dim3 g(grid_for(max_inner, 256), count);
+    msm_digits_fixed_kernel<<<g, 256, 0, st>>>(scalars, b.rows, d_soff, (uint32_t)fb->c, fb->W, b.digits);
+    ZK_HIP(c, hipGetLastError());
+    phase_end(c, "msm_digits", st, ev);
+    rc = run_rows(c, sh, b, (const fe*)fb->table.ptr, st);
+    if (rc) return rc;
+  }
+  std::vector<xyzz> hs(count);
+  if (max_inner > 0)
+    ZK_HIP(c, hipMemcpyAsync(hs.data(), b.rexp, (size_t)count * sizeof(xyzz), hipMemcpyDeviceToHost, st));
+  ZK_HIP(c, hipStreamSynchronize(st));
+  for (uint32_t j = 0; j < count; j++) {
+    if (sizes[j] == 0) continue;
+    export_jacobian(hs[j], out_jac + 12ull * j);
   }
   return ZK_OK;
 }

@@ -125,3 +125,44 @@ def test_msm_closed_form_large(zctx, log_n, skewed):
         assert _msm(zctx, bases, scal, n, c) == expect, c
     bases.free()
     scal.free()
+
+
+@pytest.mark.parametrize("c", [0, 8, 13])
+def test_fixed_base_batch(zctx, c):
+    n = 1 << 12
+    ks = workload.fr_uniform(0x4101, n)
+    bases = workload.device_bases(zctx, ks)
+    fb = zctx.fixed_bases_create(bases.ptr, n, c)
+    sizes = [1, 0, 5, 100, 1000, n, 777, 2]
+    tot = sum(sizes)
+    ss = workload.scalars_skewed(0x4102, tot)
+    scal = workload.device_fr_mont(zctx, ss)
+    offs = [sum(sizes[:j]) for j in range(len(sizes))]
+    got = zctx.msm_g1_fixed_batch(fb, scal.ptr, offs, sizes)
+    kk = words_to_ints(ks)
+    sv = words_to_ints(ss)
+    for j, (o, m) in enumerate(zip(offs, sizes)):
+        p = from_jacobian_mont(*got[j])
+        res = None if p[2] == 0 else (p[0], p[1])
+        assert res == M.msm_closed_form(kk[:m], sv[o:o + m]), (c, j, m)
+    fb.free()
+    bases.free()
+    scal.free()
+
+
+def test_fixed_base_batch_same_scalars(zctx):
+    # every scalar equal: one bucket holds everything (deep fix-up)
+    n = 3000
+    ks = workload.fr_uniform(0x4201, n)
+    bases = workload.device_bases(zctx, ks)
+    fb = zctx.fixed_bases_create(bases.ptr, n, 10)
+    s = workload.fr_uniform(0x4202, 1)
+    ss = np.repeat(s, n, axis=0)
+    scal = workload.device_fr_mont(zctx, ss)
+    got = zctx.msm_g1_fixed_batch(fb, scal.ptr, [0, 0], [n, n // 2])
+    kk = words_to_ints(ks)
+    sv = words_to_ints(ss)
+    for (o, m), g in zip(((0, n), (0, n // 2)), got):
+        p = from_jacobian_mont(*g)
+        assert (p[0], p[1]) == M.msm_closed_form(kk[:m], sv[:m])
+    fb.free()
