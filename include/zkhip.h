@@ -37,6 +37,22 @@ extern "C" {
 
 typedef struct zk_ctx zk_ctx;
 typedef struct zk_fixed_bases zk_fixed_bases;
+typedef struct zk_batch zk_batch;
+
+/* One NTT job of a batch: `batch` transforms of size 2^log_n laid out back
+ * to back, out of place (in != out), with the optional root / coset shift
+ * of zk_ntt. */
+typedef struct zk_ntt_job {
+  const void* in;
+  void* out;
+  uint32_t log_n;
+  uint32_t batch;
+  int inverse;
+  int has_omega;
+  int has_shift;
+  uint64_t omega_mont[4];
+  uint64_t shift_mont[4];
+} zk_ntt_job;
 
 /* Library version (major * 10000 + minor * 100 + patch). */
 int zk_version(void);
@@ -146,6 +162,22 @@ int zk_fixed_bases_info(const zk_fixed_bases* fb, uint64_t* n, int* window_bits)
 int zk_msm_g1_fixed_batch(zk_ctx* ctx, const zk_fixed_bases* fb, const void* scalars,
                           const uint64_t* offsets, const uint64_t* sizes, uint32_t count,
                           uint64_t* out_jac_mont, void* stream);
+
+/* Batch plan: n_msm fixed-base MSMs over `fb` (as zk_msm_g1_fixed_batch)
+ * plus n_ntt NTT jobs, every buffer and table allocated at creation.  The
+ * NTT jobs run on a forked stream concurrently with the MSM pipeline.  With
+ * use_graph the whole launch sequence is captured once into a hipGraph and
+ * zk_batch_run replays it (inputs are read from the pointers given at
+ * creation, so callers refill those buffers between runs).  zk_batch_run
+ * synchronises `stream` and writes the MSM results (12 words each).
+ * Counterpart of the reference's sequential re-use of one graph per proof
+ * (Executor.run_repeated, runtime/executor.py:59-81; prove_batch,
+ * protocols/groth16/prove.py:133-248). */
+int zk_batch_create(zk_ctx* ctx, const zk_fixed_bases* fb, const void* msm_scalars,
+                    const uint64_t* msm_offsets, const uint64_t* msm_sizes, uint32_t n_msm,
+                    const zk_ntt_job* ntts, uint32_t n_ntt, int use_graph, zk_batch** out, void* stream);
+int zk_batch_run(zk_ctx* ctx, zk_batch* b, uint64_t* msm_out_jac_mont, void* stream);
+int zk_batch_destroy(zk_ctx* ctx, zk_batch* b);
 
 /* Tuning knobs: "ntt_max_pass_bits" (stages per NTT pass, 1..11; 0 = auto),
  * "msm_segment" (pairs per accumulation thread), "msm_reduce_chunk"

@@ -53,12 +53,23 @@ SIGNATURES = {
     "zk_fixed_bases_destroy": [_vp, _vp],
     "zk_fixed_bases_info": [_vp, ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_int)],
     "zk_msm_g1_fixed_batch": [_vp, _vp, _vp, _u64p, _u64p, ct.c_uint32, _u64p, _vp],
+    "zk_batch_create": [_vp, _vp, _vp, _u64p, _u64p, ct.c_uint32, _vp, ct.c_uint32, ct.c_int,
+                        ct.POINTER(_vp), _vp],
+    "zk_batch_run": [_vp, _vp, _u64p, _vp],
+    "zk_batch_destroy": [_vp, _vp],
     "zk_set_tuning": [_vp, ct.c_char_p, ct.c_int64],
     "zk_probe_mad_rate": [_vp, ct.POINTER(ct.c_double)],
     "zk_profile_enable": [_vp, ct.c_int],
     "zk_profile_get": [_vp, ct.c_char_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_uint64)],
     "zk_profile_reset": [_vp],
 }
+
+class NttJob(ct.Structure):
+    """zk_ntt_job of include/zkhip.h."""
+    _fields_ = [("in_", ct.c_void_p), ("out", ct.c_void_p), ("log_n", ct.c_uint32), ("batch", ct.c_uint32),
+                ("inverse", ct.c_int), ("has_omega", ct.c_int), ("has_shift", ct.c_int),
+                ("omega_mont", ct.c_uint64 * 4), ("shift_mont", ct.c_uint64 * 4)]
+
 
 _lib = None
 _lib_lock = threading.Lock()
@@ -133,6 +144,27 @@ class FixedBases:
     def free(self) -> None:
         if self.handle:
             self.ctx._check(self.ctx.lib.zk_fixed_bases_destroy(self.ctx.handle, self.handle))
+            self.handle = None
+
+
+class BatchPlan:
+    """Fixed-base MSMs + NTT jobs, optionally replayed from one hipGraph."""
+
+    def __init__(self, ctx: "Context", handle, n_msm: int) -> None:
+        self.ctx = ctx
+        self.handle = handle
+        self.n_msm = n_msm
+
+    def run(self, stream=None):
+        out = (ct.c_uint64 * (12 * max(self.n_msm, 1)))()
+        self.ctx._check(self.ctx.lib.zk_batch_run(self.ctx.handle, self.handle, out, stream))
+        w = list(out)
+        f = lambda a: w[a] | (w[a + 1] << 64) | (w[a + 2] << 128) | (w[a + 3] << 192)
+        return [(f(12 * j), f(12 * j + 4), f(12 * j + 8)) for j in range(self.n_msm)]
+
+    def free(self) -> None:
+        if self.handle:
+            self.ctx._check(self.ctx.lib.zk_batch_destroy(self.ctx.handle, self.handle))
             self.handle = None
 
 
@@ -277,6 +309,30 @@ class Context:
         w = list(out)
         f = lambda a: w[a] | (w[a + 1] << 64) | (w[a + 2] << 128) | (w[a + 3] << 192)
         return [(f(12 * j), f(12 * j + 4), f(12 * j + 8)) for j in range(count)]
+
+    def batch_create(self, fb: Optional["FixedBases"], scalars: int, offsets, sizes, ntt_jobs,
+                     use_graph: bool = True, stream=None) -> "BatchPlan":
+        """ntt_jobs: list of dicts {in, out, log_n, batch, inverse, omega_mont?, shift_mont?}."""
+        n_msm = len(sizes)
+        off = (ct.c_uint64 * max(n_msm, 1))(*[int(x) for x in offsets])
+        sz = (ct.c_uint64 * max(n_msm, 1))(*[int(x) for x in sizes])
+        jobs = (NttJob * max(len(ntt_jobs), 1))()
+        for k, j in enumerate(ntt_jobs):
+            jobs[k].in_ = j["in"]
+            jobs[k].out = j["out"]
+            jobs[k].log_n = j["log_n"]
+            jobs[k].batch = j.get("batch", 1)
+            jobs[k].inverse = int(bool(j.get("inverse", False)))
+            if j.get("omega_mont") is not None:
+                jobs[k].has_omega = 1
+                jobs[k].omega_mont = _words(j["omega_mont"])
+            if j.get("shift_mont") is not None:
+                jobs[k].has_shift = 1
+                jobs[k].shift_mont = _words(j["shift_mont"])
+        h = _vp()
+        self._check(self.lib.zk_batch_create(self.handle, fb.handle if fb else None, scalars, off, sz, n_msm,
+                                             jobs, len(ntt_jobs), int(bool(use_graph)), ct.byref(h), stream))
+        return BatchPlan(self, h, n_msm)
 
     # ---- tuning / profiling
     def set_tuning(self, key: str, value: int) -> None:

@@ -352,6 +352,46 @@ int zk_msm_g1_fixed_batch(zk_ctx* ctx, const zk_fixed_bases* fb, const void* sca
                              out_jac_mont, pick(ctx, stream));
 }
 
+int zk_batch_create(zk_ctx* ctx, const zk_fixed_bases* fb, const void* msm_scalars,
+                    const uint64_t* msm_offsets, const uint64_t* msm_sizes, uint32_t n_msm,
+                    const zk_ntt_job* ntts, uint32_t n_ntt, int use_graph, zk_batch** out, void* stream) {
+  CTX_CHECK(ctx);
+  if (!out) return set_err(&ctx->c, ZK_ERR_INVALID, "null output");
+  *out = nullptr;
+  if (n_msm && (!msm_scalars || !msm_offsets || !msm_sizes)) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  if (n_ntt && !ntts) return set_err(&ctx->c, ZK_ERR_INVALID, "null NTT job list");
+  std::vector<NttJob> jobs(n_ntt);
+  for (uint32_t k = 0; k < n_ntt; k++) {
+    if (!ntts[k].in || !ntts[k].out) return set_err(&ctx->c, ZK_ERR_INVALID, "null NTT buffer");
+    jobs[k].in = (const fe*)ntts[k].in;
+    jobs[k].out = (fe*)ntts[k].out;
+    jobs[k].log_n = ntts[k].log_n;
+    jobs[k].batch = ntts[k].batch;
+    jobs[k].inverse = ntts[k].inverse != 0;
+    jobs[k].has_omega = ntts[k].has_omega != 0;
+    jobs[k].has_shift = ntts[k].has_shift != 0;
+    jobs[k].omega = words_to_fe(ntts[k].omega_mont);
+    jobs[k].shift = words_to_fe(ntts[k].shift_mont);
+  }
+  BatchPlan* p = nullptr;
+  int rc = batch_create(&ctx->c, (const FixedBases*)fb, (const fe*)msm_scalars, msm_offsets, msm_sizes, n_msm,
+                        jobs.data(), n_ntt, use_graph != 0, &p, pick(ctx, stream));
+  *out = (zk_batch*)p;
+  return rc;
+}
+
+int zk_batch_run(zk_ctx* ctx, zk_batch* b, uint64_t* msm_out_jac_mont, void* stream) {
+  CTX_CHECK(ctx);
+  if (!b) return set_err(&ctx->c, ZK_ERR_INVALID, "null batch");
+  return batch_run(&ctx->c, (BatchPlan*)b, msm_out_jac_mont, pick(ctx, stream));
+}
+
+int zk_batch_destroy(zk_ctx* ctx, zk_batch* b) {
+  CTX_CHECK(ctx);
+  batch_destroy((BatchPlan*)b);
+  return ZK_OK;
+}
+
 int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value) {
   CTX_CHECK(ctx);
   if (!key) return set_err(&ctx->c, ZK_ERR_INVALID, "null key");

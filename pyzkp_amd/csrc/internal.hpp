@@ -71,7 +71,21 @@ void phase_end(Ctx* c, const char* name, hipStream_t st, hipEvent_t ev);
 // NTT ----------------------------------------------------------------------
 int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint32_t batch,
             bool inverse, const zkh::Fe* omega, const zkh::Fe* shift, hipStream_t st,
-            int max_pass_bits);
+            int max_pass_bits, fe* tmp_buf = nullptr);
+int ntt_prepare(Ctx* c, uint32_t log_n, bool inverse, const zkh::Fe* omega, const zkh::Fe* shift,
+                hipStream_t st);
+
+// one NTT job of a batch plan (out of place, `batch` transforms back to back)
+struct NttJob {
+  const fe* in = nullptr;
+  fe* out = nullptr;
+  uint32_t log_n = 0;
+  uint32_t batch = 1;
+  bool inverse = false;
+  bool has_omega = false;
+  bool has_shift = false;
+  zkh::Fe omega, shift;
+};
 
 // element-wise -----------------------------------------------------------------
 enum EltOp { ELT_FR_TO_MONT = 0, ELT_FR_FROM_MONT, ELT_FQ_TO_MONT, ELT_FQ_FROM_MONT,
@@ -92,6 +106,12 @@ int fixed_bases_create(Ctx* c, const fe* bases, uint64_t n, int window_bits, Fix
 void fixed_bases_destroy(FixedBases* fb);
 int msm_fixed_batch_run(Ctx* c, const FixedBases* fb, const fe* scalars, const uint64_t* offsets,
                         const uint64_t* sizes, uint32_t count, uint64_t* out_jac, hipStream_t st);
+struct BatchPlan;
+int batch_create(Ctx* c, const FixedBases* fb, const fe* scalars, const uint64_t* offsets,
+                 const uint64_t* sizes, uint32_t n_msm, const NttJob* jobs, uint32_t n_ntt, bool use_graph,
+                 BatchPlan** out, hipStream_t st);
+int batch_run(Ctx* c, BatchPlan* p, uint64_t* out_jac, hipStream_t st);
+void batch_destroy(BatchPlan* p);
 int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
                uint64_t out_jac[12], hipStream_t st);
 

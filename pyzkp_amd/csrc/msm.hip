@@ -877,4 +877,178 @@ int msm_fixed_batch_run(Ctx* c, const FixedBases* fb, const fe* scalars, const u
   return ZK_OK;
 }
 
+// ------------------------------------------------------------------ batches
+// A batch plan: many fixed-base MSMs (uneven sizes, one base set) plus many
+// NTT jobs, with every buffer and table allocated up front so the whole
+// launch sequence can be captured once into a hipGraph and replayed.  The
+// NTT jobs run on a forked stream, concurrent with the MSM pipeline.
+
+struct BatchPlan {
+  const FixedBases* fb = nullptr;
+  uint32_t n_msm = 0;
+  std::vector<uint64_t> sizes;
+  const fe* scalars = nullptr;
+  uint32_t max_inner = 0;
+  MsmShape sh;
+  MsmBuffers b;
+  uint64_t* d_soff = nullptr;
+  DevBuf msm_scratch;
+  std::vector<NttJob> ntts;
+  std::vector<DevBuf> ntt_tmp;
+  hipStream_t aux = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  bool use_graph = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  xyzz* host_res = nullptr;  // pinned
+};
+
+static int batch_enqueue(Ctx* c, BatchPlan* p, hipStream_t st) {
+  ZK_HIP(c, hipEventRecord(p->fork, st));
+  ZK_HIP(c, hipStreamWaitEvent(p->aux, p->fork, 0));
+  for (size_t k = 0; k < p->ntts.size(); k++) {
+    const NttJob& j = p->ntts[k];
+    int rc = ntt_run(c, j.in, 1ull << j.log_n, j.out, j.log_n, j.batch, j.inverse,
+                     j.has_omega ? &j.omega : nullptr, j.has_shift ? &j.shift : nullptr, p->aux,
+                     c->ntt_max_pass_bits, (fe*)p->ntt_tmp[k].ptr);
+    if (rc) return rc;
+  }
+  ZK_HIP(c, hipEventRecord(p->join, p->aux));
+  if (p->n_msm && p->max_inner) {
+    dim3 g(grid_for(p->max_inner, 256), p->n_msm);
+    msm_digits_fixed_kernel<<<g, 256, 0, st>>>(p->scalars, p->b.rows, p->d_soff, (uint32_t)p->fb->c, p->fb->W,
+                                               p->b.digits);
+    ZK_HIP(c, hipGetLastError());
+    int rc = run_rows(c, p->sh, p->b, (const fe*)p->fb->table.ptr, st);
+    if (rc) return rc;
+    ZK_HIP(c, hipMemcpyAsync(p->host_res, p->b.rexp, (size_t)p->n_msm * sizeof(xyzz), hipMemcpyDeviceToHost, st));
+  }
+  ZK_HIP(c, hipStreamWaitEvent(st, p->join, 0));
+  return ZK_OK;
+}
+
+void batch_destroy(BatchPlan* p) {
+  if (!p) return;
+  hipDeviceSynchronize();
+  if (p->exec) hipGraphExecDestroy(p->exec);
+  if (p->graph) hipGraphDestroy(p->graph);
+  if (p->msm_scratch.ptr) hipFree(p->msm_scratch.ptr);
+  for (auto& t : p->ntt_tmp)
+    if (t.ptr) hipFree(t.ptr);
+  if (p->host_res) hipHostFree(p->host_res);
+  if (p->fork) hipEventDestroy(p->fork);
+  if (p->join) hipEventDestroy(p->join);
+  if (p->aux) hipStreamDestroy(p->aux);
+  delete p;
+}
+
+int batch_create(Ctx* c, const FixedBases* fb, const fe* scalars, const uint64_t* offsets,
+                 const uint64_t* sizes, uint32_t n_msm, const NttJob* jobs, uint32_t n_ntt, bool use_graph,
+                 BatchPlan** out, hipStream_t st) {
+  *out = nullptr;
+  if (n_msm && !fb) return set_err(c, ZK_ERR_INVALID, "MSMs need a fixed-base set");
+  BatchPlan* p = new BatchPlan();
+  auto fail = [&](int rc) {
+    batch_destroy(p);
+    return rc;
+  };
+  hipError_t e = hipStreamCreateWithFlags(&p->aux, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&p->join, hipEventDisableTiming);
+  if (e != hipSuccess) return fail(hip_check(c, e, "batch streams"));
+  p->fb = fb;
+  p->n_msm = n_msm;
+  p->scalars = scalars;
+  p->use_graph = use_graph;
+  if (n_msm) {
+    std::vector<MsmRow> rows(n_msm);
+    std::vector<uint64_t> soff(n_msm);
+    uint64_t off = 0;
+    uint32_t tiles = 0;
+    for (uint32_t j = 0; j < n_msm; j++) {
+      if (sizes[j] > fb->n) return fail(set_err(c, ZK_ERR_INVALID, "scalars length exceeds points length"));
+      const uint64_t len = sizes[j] * fb->W;
+      rows[j] = MsmRow{off, (uint32_t)len, (uint32_t)sizes[j], (uint32_t)fb->n,
+                       (uint32_t)((len + SORT_TILE - 1) / SORT_TILE), tiles, 0};
+      tiles += rows[j].ntiles;
+      off += len;
+      soff[j] = offsets[j];
+      p->sizes.push_back(sizes[j]);
+      if (sizes[j] > p->max_inner) p->max_inner = (uint32_t)sizes[j];
+    }
+    if (off >= (1ull << 32)) return fail(set_err(c, ZK_ERR_INVALID, "total digit count must be below 2^32"));
+    int rc = make_shape(c, fb->c, n_msm, rows, p->sh);
+    if (rc) return fail(rc);
+    const size_t bytes = layout(p->sh, nullptr, nullptr);
+    const size_t o_soff = align256(bytes);
+    p->msm_scratch.bytes = o_soff + (size_t)n_msm * 8;
+    e = hipMalloc(&p->msm_scratch.ptr, p->msm_scratch.bytes);
+    if (e != hipSuccess) return fail(hip_check(c, e, "hipMalloc(batch MSM scratch)"));
+    layout(p->sh, (char*)p->msm_scratch.ptr, &p->b);
+    p->d_soff = (uint64_t*)((char*)p->msm_scratch.ptr + o_soff);
+    e = hipMemcpy(p->b.rows, rows.data(), rows.size() * sizeof(MsmRow), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_soff, soff.data(), soff.size() * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&p->host_res, (size_t)n_msm * sizeof(xyzz), hipHostMallocDefault);
+    if (e != hipSuccess) return fail(hip_check(c, e, "batch MSM setup"));
+  }
+  for (uint32_t k = 0; k < n_ntt; k++) {
+    const NttJob& j = jobs[k];
+    if (j.log_n > 28 || j.batch == 0) return fail(set_err(c, ZK_ERR_INVALID, "bad NTT job"));
+    if ((const void*)j.in == (const void*)j.out)
+      return fail(set_err(c, ZK_ERR_INVALID, "batched NTT jobs must be out of place"));
+    int rc = ntt_prepare(c, j.log_n, j.inverse, j.has_omega ? &j.omega : nullptr, j.has_shift ? &j.shift : nullptr,
+                         st);
+    if (rc) return fail(rc);
+    DevBuf t;
+    t.bytes = ((size_t)j.batch << j.log_n) * 32;
+    e = hipMalloc(&t.ptr, t.bytes);
+    if (e != hipSuccess) return fail(hip_check(c, e, "hipMalloc(batch NTT scratch)"));
+    p->ntt_tmp.push_back(t);
+    p->ntts.push_back(j);
+  }
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return fail(hip_check(c, e, "batch setup"));
+  if (use_graph) {
+    const bool prof = c->profiling;
+    c->profiling = false;
+    e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      c->profiling = prof;
+      return fail(hip_check(c, e, "hipStreamBeginCapture"));
+    }
+    int rc = batch_enqueue(c, p, st);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(st, &g);
+    c->profiling = prof;
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return fail(rc);
+    }
+    if (e != hipSuccess) return fail(hip_check(c, e, "hipStreamEndCapture"));
+    p->graph = g;
+    e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) return fail(hip_check(c, e, "hipGraphInstantiate"));
+  }
+  *out = p;
+  return ZK_OK;
+}
+
+int batch_run(Ctx* c, BatchPlan* p, uint64_t* out_jac, hipStream_t st) {
+  if (p->use_graph) {
+    ZK_HIP(c, hipGraphLaunch(p->exec, st));
+  } else {
+    int rc = batch_enqueue(c, p, st);
+    if (rc) return rc;
+  }
+  ZK_HIP(c, hipStreamSynchronize(st));
+  for (uint32_t j = 0; j < p->n_msm; j++) {
+    if (p->sizes[j] == 0) {
+      for (int i = 0; i < 12; i++) out_jac[12ull * j + i] = 0;
+      continue;
+    }
+    export_jacobian(p->host_res[j], out_jac + 12ull * j);
+  }
+  return ZK_OK;
+}
+
 }  // namespace zk

@@ -257,7 +257,7 @@ static void plan_passes(uint32_t log_n, int max_pass_bits, std::vector<uint32_t>
 
 int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint32_t batch,
             bool inverse, const zkh::Fe* omega, const zkh::Fe* shift, hipStream_t st,
-            int max_pass_bits) {
+            int max_pass_bits, fe* tmp_buf) {
   if (log_n > 28) return set_err(c, ZK_ERR_INVALID, "domain too large for BN254 root of unity");
   if (batch == 0) return ZK_OK;
   const uint64_t n = 1ull << log_n;
@@ -310,8 +310,8 @@ int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint
   const size_t P = ks.size();
 
   // buffer sequence: in -> ... -> out, intermediates never alias their source
-  fe* tmp = nullptr;
-  if (P > 1) {
+  fe* tmp = tmp_buf;
+  if (P > 1 && !tmp) {
     void* p;
     rc = ensure_scratch(c, 0, (size_t)n * batch * 32, &p);
     if (rc) return rc;
@@ -324,6 +324,7 @@ int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint
     dst[i] = (((P - 1 - i) & 1) == 0) ? out : tmp;
   }
   if (P > 1 && (const void*)in == (const void*)out && dst[0] == out) {
+    if (tmp_buf) return set_err(c, ZK_ERR_INVALID, "in-place NTT with a caller scratch buffer is not supported");
     // in-place with an odd pass count: copy the input aside first
     void* p2;
     rc = ensure_scratch(c, 1, (size_t)n * batch * 32, &p2);
@@ -354,4 +355,33 @@ int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint
   return ZK_OK;
 }
 
+}  // namespace zk
+
+namespace zk {
+// Build the twiddle / scaling tables an NTT of this shape needs (so a later
+// launch sequence can run without allocating, e.g. under graph capture).
+int ntt_prepare(Ctx* c, uint32_t log_n, bool inverse, const zkh::Fe* omega, const zkh::Fe* shift,
+                hipStream_t st) {
+  if (log_n == 0) return ZK_OK;
+  if (log_n > 28) return set_err(c, ZK_ERR_INVALID, "domain too large for BN254 root of unity");
+  const uint64_t n = 1ull << log_n;
+  zkh::Fe w = omega ? *omega : standard_root(log_n);
+  if (inverse) w = zkh::inv(zkh::FR, w);
+  const fe* tw;
+  int rc = get_twiddles(c, log_n, w, &tw, st);
+  if (rc) return rc;
+  const fe *h, *l;
+  if (!inverse) {
+    if (shift) rc = get_pow_tables(c, log_n, *shift, zkh::one(zkh::FR), &h, &l, st);
+  } else {
+    zkh::Fe ninv = zkh::inv(zkh::FR, zkh::from_u64(zkh::FR, n));
+    if (shift) {
+      zkh::Fe ginv = zkh::is_zero(*shift) ? zkh::zero() : zkh::inv(zkh::FR, *shift);
+      rc = get_pow_tables(c, log_n, ginv, ninv, &h, &l, st);
+    } else {
+      rc = get_pow_tables(c, 0, zkh::zero(), ninv, &h, &l, st);
+    }
+  }
+  return rc;
+}
 }  // namespace zk
