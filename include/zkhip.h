@@ -153,15 +153,16 @@ int zk_fixed_bases_destroy(zk_ctx* ctx, zk_fixed_bases* fb);
 int zk_fixed_bases_info(const zk_fixed_bases* fb, uint64_t* n, int* window_bits);
 
 /* count MSMs over one fixed-base set: MSM j sums scalars[offsets[j] + i] *
- * B_i for i < sizes[j] (sizes[j] <= n; Montgomery scalars, offsets in
- * elements).  Results: out_jac_mont[12 j .. 12 j + 12), Jacobian Montgomery.
+ * B_(base_offsets[j] + i) for i < sizes[j] (base_offsets NULL: all zero;
+ * base_offsets[j] + sizes[j] <= n; Montgomery scalars, offsets in
+ * elements), so one set can hold several query vectors back to back.  Results: out_jac_mont[12 j .. 12 j + 12), Jacobian Montgomery.
  * One launch sequence for the whole batch; counterpart of
  * msm_fixed_base_batch / msm_pippenger_batch (crypto/msm/pippenger.py:124-214,
  * 312-342) behind MSM_G1_BATCH and KZG_BATCH_COMMIT
  * (runtime/kernels/cpu/kernels.py:441-496). */
 int zk_msm_g1_fixed_batch(zk_ctx* ctx, const zk_fixed_bases* fb, const void* scalars,
-                          const uint64_t* offsets, const uint64_t* sizes, uint32_t count,
-                          uint64_t* out_jac_mont, void* stream);
+                          const uint64_t* offsets, const uint64_t* sizes, const uint64_t* base_offsets,
+                          uint32_t count, uint64_t* out_jac_mont, void* stream);
 
 /* Batch plan: n_msm fixed-base MSMs over `fb` (as zk_msm_g1_fixed_batch)
  * plus n_ntt NTT jobs, every buffer and table allocated at creation.  The
@@ -174,10 +175,37 @@ int zk_msm_g1_fixed_batch(zk_ctx* ctx, const zk_fixed_bases* fb, const void* sca
  * (Executor.run_repeated, runtime/executor.py:59-81; prove_batch,
  * protocols/groth16/prove.py:133-248). */
 int zk_batch_create(zk_ctx* ctx, const zk_fixed_bases* fb, const void* msm_scalars,
-                    const uint64_t* msm_offsets, const uint64_t* msm_sizes, uint32_t n_msm,
-                    const zk_ntt_job* ntts, uint32_t n_ntt, int use_graph, zk_batch** out, void* stream);
+                    const uint64_t* msm_offsets, const uint64_t* msm_sizes, const uint64_t* msm_base_offsets,
+                    uint32_t n_msm, const zk_ntt_job* ntts, uint32_t n_ntt, int use_graph, zk_batch** out,
+                    void* stream);
 int zk_batch_run(zk_ctx* ctx, zk_batch* b, uint64_t* msm_out_jac_mont, void* stream);
 int zk_batch_destroy(zk_ctx* ctx, zk_batch* b);
+
+/* Groth16 helpers on the device (Montgomery Fr):
+ * zk_r1cs_eval: out[j] = sum_{k in [row_ptr[j], row_ptr[j+1])} coeffs[k] * w[cols[k]]
+ *   for j < rows (CSR; replaces eval_r1cs_vectors, protocols/groth16/r1cs.py:77-92);
+ * zk_fr_gather: out[k] = src[idx[k]] (aux slice, protocols/groth16/prove.py:105);
+ * zk_fr_inv: out[i] = a[i]^-1, 0 -> 0 (BATCH_INV, runtime/kernels/cpu/kernels.py:253-269);
+ * zk_fr_count_nonzero: number of non-zero elements (synchronises `stream`;
+ *   the R1CS-satisfaction check of protocols/groth16/qap.py:98-99). */
+int zk_r1cs_eval(zk_ctx* ctx, const uint32_t* row_ptr, const uint32_t* cols, const void* coeffs, const void* w,
+                 uint64_t rows, void* out, void* stream);
+int zk_fr_gather(zk_ctx* ctx, const void* src, const uint32_t* idx, uint64_t n, void* out, void* stream);
+int zk_fr_inv(zk_ctx* ctx, const void* a, void* out, uint64_t n, void* stream);
+int zk_fr_count_nonzero(zk_ctx* ctx, const void* a, uint64_t n, uint64_t* result, void* stream);
+
+/* G2 on the host (CPU threads).  Points are affine over Fq2 = Fq[u]/(u^2+1)
+ * on the twist y^2 = x^3 + 3/(9+u), CANONICAL little-endian words
+ * (x0, x1, y0, y1; 16 x u64, all zero = infinity); scalars canonical.
+ * zk_g2_msm_host replaces msm_pippenger_g2 (crypto/msm/pippenger.py:67-121,
+ * the CPU MSM_G2 op runtime/kernels/cpu/kernels.py:338-348); window_bits 0
+ * follows the reference rule, threads 0 = all hardware threads.
+ * zk_g2_mul_fixed_host computes out[i] = [k_i] Q (setup-time g2_mul loops,
+ * protocols/groth16/setup.py:63-83); zk_g2_add_host adds two points. */
+int zk_g2_msm_host(const uint64_t* bases, const uint64_t* scalars, uint64_t n, int window_bits, int threads,
+                   uint64_t out[16]);
+int zk_g2_mul_fixed_host(const uint64_t* q, const uint64_t* scalars, uint64_t n, int threads, uint64_t* out);
+int zk_g2_add_host(const uint64_t* p, const uint64_t* q, uint64_t out[16]);
 
 /* Tuning knobs: "ntt_max_pass_bits" (stages per NTT pass, 1..11; 0 = auto),
  * "msm_segment" (pairs per accumulation thread), "msm_reduce_chunk"

@@ -52,11 +52,18 @@ SIGNATURES = {
     "zk_fixed_bases_create": [_vp, _vp, ct.c_uint64, ct.c_int, ct.POINTER(_vp), _vp],
     "zk_fixed_bases_destroy": [_vp, _vp],
     "zk_fixed_bases_info": [_vp, ct.POINTER(ct.c_uint64), ct.POINTER(ct.c_int)],
-    "zk_msm_g1_fixed_batch": [_vp, _vp, _vp, _u64p, _u64p, ct.c_uint32, _u64p, _vp],
-    "zk_batch_create": [_vp, _vp, _vp, _u64p, _u64p, ct.c_uint32, _vp, ct.c_uint32, ct.c_int,
+    "zk_msm_g1_fixed_batch": [_vp, _vp, _vp, _u64p, _u64p, _u64p, ct.c_uint32, _u64p, _vp],
+    "zk_batch_create": [_vp, _vp, _vp, _u64p, _u64p, _u64p, ct.c_uint32, _vp, ct.c_uint32, ct.c_int,
                         ct.POINTER(_vp), _vp],
     "zk_batch_run": [_vp, _vp, _u64p, _vp],
     "zk_batch_destroy": [_vp, _vp],
+    "zk_r1cs_eval": [_vp, _vp, _vp, _vp, _vp, ct.c_uint64, _vp, _vp],
+    "zk_fr_gather": [_vp, _vp, _vp, ct.c_uint64, _vp, _vp],
+    "zk_fr_inv": [_vp, _vp, _vp, ct.c_uint64, _vp],
+    "zk_fr_count_nonzero": [_vp, _vp, ct.c_uint64, ct.POINTER(ct.c_uint64), _vp],
+    "zk_g2_msm_host": [_vp, _vp, ct.c_uint64, ct.c_int, ct.c_int, _vp],
+    "zk_g2_mul_fixed_host": [_vp, _vp, ct.c_uint64, ct.c_int, _vp],
+    "zk_g2_add_host": [_vp, _vp, _vp],
     "zk_set_tuning": [_vp, ct.c_char_p, ct.c_int64],
     "zk_probe_mad_rate": [_vp, ct.POINTER(ct.c_double)],
     "zk_profile_enable": [_vp, ct.c_int],
@@ -287,6 +294,20 @@ class Context:
         f = lambda a: w[a] | (w[a + 1] << 64) | (w[a + 2] << 128) | (w[a + 3] << 192)
         return f(0), f(4), f(8)
 
+    def r1cs_eval(self, row_ptr: int, cols: int, coeffs: int, w: int, rows: int, out: int, stream=None) -> None:
+        self._check(self.lib.zk_r1cs_eval(self.handle, row_ptr, cols, coeffs, w, int(rows), out, stream))
+
+    def fr_gather(self, src: int, idx: int, n: int, out: int, stream=None) -> None:
+        self._check(self.lib.zk_fr_gather(self.handle, src, idx, int(n), out, stream))
+
+    def fr_inv(self, a: int, out: int, n: int, stream=None) -> None:
+        self._check(self.lib.zk_fr_inv(self.handle, a, out, int(n), stream))
+
+    def fr_count_nonzero(self, a: int, n: int, stream=None) -> int:
+        r = ct.c_uint64(0)
+        self._check(self.lib.zk_fr_count_nonzero(self.handle, a, int(n), ct.byref(r), stream))
+        return r.value
+
     def g1_fixed_base(self, table: int, scalars: int, n: int, out: int, stream=None) -> None:
         self._check(self.lib.zk_g1_fixed_base(self.handle, table, scalars, int(n), out, stream))
 
@@ -299,19 +320,22 @@ class Context:
         self.lib.zk_fixed_bases_info(h, ct.byref(nn), ct.byref(cb))
         return FixedBases(self, h, nn.value, cb.value)
 
-    def msm_g1_fixed_batch(self, fb: "FixedBases", scalars: int, offsets, sizes, stream=None):
+    def msm_g1_fixed_batch(self, fb: "FixedBases", scalars: int, offsets, sizes, base_offsets=None,
+                           stream=None):
         """Returns a list of Jacobian Montgomery (X, Y, Z) int triples."""
         count = len(sizes)
         off = (ct.c_uint64 * max(count, 1))(*[int(x) for x in offsets])
         sz = (ct.c_uint64 * max(count, 1))(*[int(x) for x in sizes])
+        bo = (ct.c_uint64 * max(count, 1))(*[int(x) for x in base_offsets]) if base_offsets is not None else None
         out = (ct.c_uint64 * (12 * max(count, 1)))()
-        self._check(self.lib.zk_msm_g1_fixed_batch(self.handle, fb.handle, scalars, off, sz, count, out, stream))
+        self._check(self.lib.zk_msm_g1_fixed_batch(self.handle, fb.handle, scalars, off, sz, bo, count, out,
+                                                   stream))
         w = list(out)
         f = lambda a: w[a] | (w[a + 1] << 64) | (w[a + 2] << 128) | (w[a + 3] << 192)
         return [(f(12 * j), f(12 * j + 4), f(12 * j + 8)) for j in range(count)]
 
     def batch_create(self, fb: Optional["FixedBases"], scalars: int, offsets, sizes, ntt_jobs,
-                     use_graph: bool = True, stream=None) -> "BatchPlan":
+                     use_graph: bool = True, base_offsets=None, stream=None) -> "BatchPlan":
         """ntt_jobs: list of dicts {in, out, log_n, batch, inverse, omega_mont?, shift_mont?}."""
         n_msm = len(sizes)
         off = (ct.c_uint64 * max(n_msm, 1))(*[int(x) for x in offsets])
@@ -329,8 +353,9 @@ class Context:
             if j.get("shift_mont") is not None:
                 jobs[k].has_shift = 1
                 jobs[k].shift_mont = _words(j["shift_mont"])
+        bo = (ct.c_uint64 * max(n_msm, 1))(*[int(x) for x in base_offsets]) if base_offsets is not None else None
         h = _vp()
-        self._check(self.lib.zk_batch_create(self.handle, fb.handle if fb else None, scalars, off, sz, n_msm,
+        self._check(self.lib.zk_batch_create(self.handle, fb.handle if fb else None, scalars, off, sz, bo, n_msm,
                                              jobs, len(ntt_jobs), int(bool(use_graph)), ct.byref(h), stream))
         return BatchPlan(self, h, n_msm)
 

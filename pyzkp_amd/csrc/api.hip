@@ -342,19 +342,20 @@ int zk_fixed_bases_info(const zk_fixed_bases* fb, uint64_t* n, int* window_bits)
 }
 
 int zk_msm_g1_fixed_batch(zk_ctx* ctx, const zk_fixed_bases* fb, const void* scalars,
-                          const uint64_t* offsets, const uint64_t* sizes, uint32_t count,
-                          uint64_t* out_jac_mont, void* stream) {
+                          const uint64_t* offsets, const uint64_t* sizes, const uint64_t* base_offsets,
+                          uint32_t count, uint64_t* out_jac_mont, void* stream) {
   CTX_CHECK(ctx);
   if (!fb) return set_err(&ctx->c, ZK_ERR_INVALID, "null fixed-base set");
   if (count && (!scalars || !offsets || !sizes || !out_jac_mont))
     return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
-  return msm_fixed_batch_run(&ctx->c, (const FixedBases*)fb, (const fe*)scalars, offsets, sizes, count,
-                             out_jac_mont, pick(ctx, stream));
+  return msm_fixed_batch_run(&ctx->c, (const FixedBases*)fb, (const fe*)scalars, offsets, sizes, base_offsets,
+                             count, out_jac_mont, pick(ctx, stream));
 }
 
 int zk_batch_create(zk_ctx* ctx, const zk_fixed_bases* fb, const void* msm_scalars,
-                    const uint64_t* msm_offsets, const uint64_t* msm_sizes, uint32_t n_msm,
-                    const zk_ntt_job* ntts, uint32_t n_ntt, int use_graph, zk_batch** out, void* stream) {
+                    const uint64_t* msm_offsets, const uint64_t* msm_sizes, const uint64_t* msm_base_offsets,
+                    uint32_t n_msm, const zk_ntt_job* ntts, uint32_t n_ntt, int use_graph, zk_batch** out,
+                    void* stream) {
   CTX_CHECK(ctx);
   if (!out) return set_err(&ctx->c, ZK_ERR_INVALID, "null output");
   *out = nullptr;
@@ -374,8 +375,8 @@ int zk_batch_create(zk_ctx* ctx, const zk_fixed_bases* fb, const void* msm_scala
     jobs[k].shift = words_to_fe(ntts[k].shift_mont);
   }
   BatchPlan* p = nullptr;
-  int rc = batch_create(&ctx->c, (const FixedBases*)fb, (const fe*)msm_scalars, msm_offsets, msm_sizes, n_msm,
-                        jobs.data(), n_ntt, use_graph != 0, &p, pick(ctx, stream));
+  int rc = batch_create(&ctx->c, (const FixedBases*)fb, (const fe*)msm_scalars, msm_offsets, msm_sizes,
+                        msm_base_offsets, n_msm, jobs.data(), n_ntt, use_graph != 0, &p, pick(ctx, stream));
   *out = (zk_batch*)p;
   return rc;
 }
@@ -390,6 +391,31 @@ int zk_batch_destroy(zk_ctx* ctx, zk_batch* b) {
   CTX_CHECK(ctx);
   batch_destroy((BatchPlan*)b);
   return ZK_OK;
+}
+
+int zk_r1cs_eval(zk_ctx* ctx, const uint32_t* row_ptr, const uint32_t* cols, const void* coeffs, const void* w,
+                 uint64_t rows, void* out, void* stream) {
+  CTX_CHECK(ctx);
+  if (rows && (!row_ptr || !w || !out)) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  return r1cs_eval(&ctx->c, row_ptr, cols, (const fe*)coeffs, (const fe*)w, rows, (fe*)out, pick(ctx, stream));
+}
+
+int zk_fr_gather(zk_ctx* ctx, const void* src, const uint32_t* idx, uint64_t n, void* out, void* stream) {
+  CTX_CHECK(ctx);
+  if (n && (!src || !idx || !out)) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  return fr_gather(&ctx->c, (const fe*)src, idx, n, (fe*)out, pick(ctx, stream));
+}
+
+int zk_fr_inv(zk_ctx* ctx, const void* a, void* out, uint64_t n, void* stream) {
+  CTX_CHECK(ctx);
+  if (n && (!a || !out)) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  return fr_inv(&ctx->c, (const fe*)a, n, (fe*)out, pick(ctx, stream));
+}
+
+int zk_fr_count_nonzero(zk_ctx* ctx, const void* a, uint64_t n, uint64_t* result, void* stream) {
+  CTX_CHECK(ctx);
+  if (!result || (n && !a)) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  return fr_count_nonzero(&ctx->c, (const fe*)a, n, result, pick(ctx, stream));
 }
 
 int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value) {

@@ -105,11 +105,12 @@ int fixed_bases_create(Ctx* c, const fe* bases, uint64_t n, int window_bits, Fix
                        hipStream_t st);
 void fixed_bases_destroy(FixedBases* fb);
 int msm_fixed_batch_run(Ctx* c, const FixedBases* fb, const fe* scalars, const uint64_t* offsets,
-                        const uint64_t* sizes, uint32_t count, uint64_t* out_jac, hipStream_t st);
+                        const uint64_t* sizes, const uint64_t* base_offsets, uint32_t count, uint64_t* out_jac,
+                        hipStream_t st);
 struct BatchPlan;
 int batch_create(Ctx* c, const FixedBases* fb, const fe* scalars, const uint64_t* offsets,
-                 const uint64_t* sizes, uint32_t n_msm, const NttJob* jobs, uint32_t n_ntt, bool use_graph,
-                 BatchPlan** out, hipStream_t st);
+                 const uint64_t* sizes, const uint64_t* base_offsets, uint32_t n_msm, const NttJob* jobs,
+                 uint32_t n_ntt, bool use_graph, BatchPlan** out, hipStream_t st);
 int batch_run(Ctx* c, BatchPlan* p, uint64_t* out_jac, hipStream_t st);
 void batch_destroy(BatchPlan* p);
 int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
@@ -117,6 +118,13 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
 
 // fixed-base scalar multiplication --------------------------------------------
 int g1_fixed_base_run(Ctx* c, const fe* table, const fe* scalars, uint64_t n, fe* out, hipStream_t st);
+
+// Groth16 helpers -------------------------------------------------------------
+int r1cs_eval(Ctx* c, const uint32_t* row_ptr, const uint32_t* cols, const fe* coeffs, const fe* w, uint64_t rows,
+              fe* out, hipStream_t st);
+int fr_gather(Ctx* c, const fe* src, const uint32_t* idx, uint64_t n, fe* out, hipStream_t st);
+int fr_inv(Ctx* c, const fe* a, uint64_t n, fe* out, hipStream_t st);
+int fr_count_nonzero(Ctx* c, const fe* a, uint64_t n, uint64_t* result, hipStream_t st);
 
 // measurement probe ---------------------------------------------------------
 int probe_mad_rate(Ctx* c, double* mads_per_s);

@@ -159,7 +159,7 @@ struct MsmRow {
   uint32_t stride;  // base-index stride between windows
   uint32_t ntiles;  // ceil(len / SORT_TILE)
   uint32_t tile0;   // first tile slot of the row in the tile histogram
-  uint32_t pad;
+  uint32_t boff;    // base-index offset of the row (fixed-base ranges)
 };
 
 __global__ __launch_bounds__(SORT_THREADS) void msm_tile_count_kernel(const uint32_t* __restrict__ digits,
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_tile_scatter_kernel(
     const uint32_t sh = (b & 1) * 16;
     const uint32_t rank = (atomicAdd(&cnt[b >> 1], 1u << sh) >> sh) & 0xffffu;
     const uint32_t wi = p / row.inner;
-    const uint32_t base = wi * row.stride + (p - wi * row.inner);
+    const uint32_t base = wi * row.stride + row.boff + (p - wi * row.inner);
     sorted[rp[bin] + pre[bin] + rank] = base | (d & SIGN);
   }
 }
@@ -732,7 +732,7 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
     rows[w].stride = 0;
     rows[w].ntiles = nt;
     rows[w].tile0 = w * nt;
-    rows[w].pad = 0;
+    rows[w].boff = 0;
   }
   MsmShape sh;
   int rc = make_shape(c, cb, W, rows, sh);
@@ -819,7 +819,8 @@ void fixed_bases_destroy(FixedBases* fb) {
 
 // MSM j = sum_{i < sizes[j]} scalars[offsets[j] + i] * B_i over one fixed set.
 int msm_fixed_batch_run(Ctx* c, const FixedBases* fb, const fe* scalars, const uint64_t* offsets,
-                        const uint64_t* sizes, uint32_t count, uint64_t* out_jac, hipStream_t st) {
+                        const uint64_t* sizes, const uint64_t* base_offsets, uint32_t count, uint64_t* out_jac,
+                        hipStream_t st) {
   for (uint64_t i = 0; i < 12ull * count; i++) out_jac[i] = 0;
   if (count == 0) return ZK_OK;
   std::vector<MsmRow> rows(count);
@@ -827,7 +828,8 @@ int msm_fixed_batch_run(Ctx* c, const FixedBases* fb, const fe* scalars, const u
   uint64_t off = 0;
   uint32_t tiles = 0, max_inner = 0;
   for (uint32_t j = 0; j < count; j++) {
-    if (sizes[j] > fb->n) return set_err(c, ZK_ERR_INVALID, "scalars length exceeds points length");
+    const uint64_t bo = base_offsets ? base_offsets[j] : 0;
+    if (bo + sizes[j] > fb->n) return set_err(c, ZK_ERR_INVALID, "scalars length exceeds points length");
     const uint64_t len = sizes[j] * fb->W;
     rows[j].off = off;
     rows[j].len = (uint32_t)len;
@@ -835,7 +837,7 @@ int msm_fixed_batch_run(Ctx* c, const FixedBases* fb, const fe* scalars, const u
     rows[j].stride = (uint32_t)fb->n;
     rows[j].ntiles = (uint32_t)((len + SORT_TILE - 1) / SORT_TILE);
     rows[j].tile0 = tiles;
-    rows[j].pad = 0;
+    rows[j].boff = (uint32_t)bo;
     tiles += rows[j].ntiles;
     off += len;
     soff[j] = offsets[j];
@@ -943,8 +945,8 @@ void batch_destroy(BatchPlan* p) {
 }
 
 int batch_create(Ctx* c, const FixedBases* fb, const fe* scalars, const uint64_t* offsets,
-                 const uint64_t* sizes, uint32_t n_msm, const NttJob* jobs, uint32_t n_ntt, bool use_graph,
-                 BatchPlan** out, hipStream_t st) {
+                 const uint64_t* sizes, const uint64_t* base_offsets, uint32_t n_msm, const NttJob* jobs,
+                 uint32_t n_ntt, bool use_graph, BatchPlan** out, hipStream_t st) {
   *out = nullptr;
   if (n_msm && !fb) return set_err(c, ZK_ERR_INVALID, "MSMs need a fixed-base set");
   BatchPlan* p = new BatchPlan();
@@ -966,10 +968,11 @@ int batch_create(Ctx* c, const FixedBases* fb, const fe* scalars, const uint64_t
     uint64_t off = 0;
     uint32_t tiles = 0;
     for (uint32_t j = 0; j < n_msm; j++) {
-      if (sizes[j] > fb->n) return fail(set_err(c, ZK_ERR_INVALID, "scalars length exceeds points length"));
+      const uint64_t bo = base_offsets ? base_offsets[j] : 0;
+      if (bo + sizes[j] > fb->n) return fail(set_err(c, ZK_ERR_INVALID, "scalars length exceeds points length"));
       const uint64_t len = sizes[j] * fb->W;
       rows[j] = MsmRow{off, (uint32_t)len, (uint32_t)sizes[j], (uint32_t)fb->n,
-                       (uint32_t)((len + SORT_TILE - 1) / SORT_TILE), tiles, 0};
+                       (uint32_t)((len + SORT_TILE - 1) / SORT_TILE), tiles, (uint32_t)bo};
       tiles += rows[j].ntiles;
       off += len;
       soff[j] = offsets[j];

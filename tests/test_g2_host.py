@@ -1,0 +1,72 @@
+"""Host G2 (libzkhip CPU code) against the oracle.  CPU only (no device)."""
+import json
+import os
+import random
+
+import pytest
+
+from oracle import curve2 as O
+from oracle.field import R_MOD
+
+LIB = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "pyzkp_amd", "libzkhip.so")
+pytestmark = pytest.mark.skipif(not os.path.exists(LIB), reason="libzkhip.so not built")
+
+
+def _aff(p):
+    from pyzkp_amd.g2 import normalize
+    return normalize(p)
+
+
+def test_oracle_g2_pins():
+    assert O.is_on_twist(O.G2_GENERATOR)
+    assert O.mul(O.G2_GENERATOR, R_MOD) is None
+    with open(os.path.join(os.path.dirname(__file__), "golden", "plonk_pk_n16.json")) as f:
+        pk = json.load(f)
+    g0, g1 = (tuple(tuple(c) for c in p) for p in pk["srs_g2"])
+    assert g0 == O.G2_GENERATOR
+    assert O.is_on_twist(g1)
+    a, b = 12345, 67890
+    assert O.add(O.mul(g1, a), O.mul(g1, b)) == O.mul(g1, a + b)
+
+
+def test_g2_mul_add_match_oracle():
+    from pyzkp_amd import g2 as G
+    rng = random.Random(1)
+    G0 = (O.G2_GENERATOR[0], O.G2_GENERATOR[1], (1, 0))
+    for _ in range(5):
+        k1, k2 = rng.randrange(R_MOD), rng.randrange(R_MOD)
+        p1, p2 = G.g2_mul(G0, k1), G.g2_mul(G0, k2)
+        assert _aff(p1) == O.mul(O.G2_GENERATOR, k1)
+        assert _aff(G.g2_add(p1, p2)) == O.add(O.mul(O.G2_GENERATOR, k1), O.mul(O.G2_GENERATOR, k2))
+        assert _aff(G.g2_add(p1, p1)) == O.mul(O.G2_GENERATOR, 2 * k1)
+        neg = (p1[0], ((-p1[1][0]) % O.Q, (-p1[1][1]) % O.Q), (1, 0))
+        assert _aff(G.g2_add(p1, neg)) is None
+    assert _aff(G.g2_mul(G0, 0)) is None
+    assert _aff(G.g2_mul(G0, R_MOD)) is None
+
+
+@pytest.mark.parametrize("n,c", [(1, 0), (7, 3), (64, 0), (200, 6)])
+def test_g2_msm_matches_oracle(n, c):
+    from pyzkp_amd import g2 as G
+    rng = random.Random(n)
+    ks = [rng.randrange(1, R_MOD) for _ in range(n)]
+    ss = [rng.randrange(R_MOD) for _ in range(n)]
+    G0 = (O.G2_GENERATOR[0], O.G2_GENERATOR[1], (1, 0))
+    words = G.g2_mul_many(G0, ks)
+    bases = G.from_words(words)
+    got = _aff(G.g2_msm(bases, ss, window_bits=c, threads=4))
+    expect = O.mul(O.G2_GENERATOR, sum(k * s for k, s in zip(ks, ss)) % R_MOD)
+    assert got == expect
+    if n <= 7:
+        pts = [O.mul(O.G2_GENERATOR, k) for k in ks]
+        assert O.msm_naive(pts, ss) == expect
+
+
+def test_g2_msm_edge_cases():
+    from pyzkp_amd import g2 as G
+    G0 = (O.G2_GENERATOR[0], O.G2_GENERATOR[1], (1, 0))
+    P = G.g2_mul(G0, 77)
+    pts = [P, P, P, G.G2_ZERO, G0]
+    ss = [1, 2, R_MOD - 1, 9, 0]
+    assert _aff(G.g2_msm(pts, ss, window_bits=4)) == O.mul(O.G2_GENERATOR, 77 * 2 % R_MOD)
+    assert _aff(G.g2_msm(pts, [0] * 5)) is None
