@@ -181,6 +181,20 @@ int zk_batch_create(zk_ctx* ctx, const zk_fixed_bases* fb, const void* msm_scala
 int zk_batch_run(zk_ctx* ctx, zk_batch* b, uint64_t* msm_out_jac_mont, void* stream);
 int zk_batch_destroy(zk_ctx* ctx, zk_batch* b);
 
+/* out[i] = scale * base^i for i < n (Montgomery Fr, host words in): the
+ * geometric vectors of setup (tau^k for h_query, protocols/groth16/setup.py:85-89;
+ * the domain roots, crypto/poly/ntt.py:27-38).  Synchronises `stream`. */
+int zk_fr_powers(zk_ctx* ctx, const uint64_t base_mont[4], const uint64_t scale_mont[4], uint64_t n, void* out,
+                 void* stream);
+
+/* Synthetic benchmark witness (no reference counterpart): `chains` chains of
+ * `rounds` rounds of x^5 S-boxes on 3 lanes plus a 3-term mix y = M x5;
+ * layout documented in csrc/synth.hip.  mix: 9 canonical elements, init:
+ * 3 chains canonical elements; w: 3 chains + 12 chains rounds + 1 elements
+ * out (canonical, last = 1).  Host only. */
+int zk_synth_witness(const uint64_t* mix, const uint64_t* init, uint32_t chains, uint32_t rounds, int threads,
+                     uint64_t* w);
+
 /* Groth16 helpers on the device (Montgomery Fr):
  * zk_r1cs_eval: out[j] = sum_{k in [row_ptr[j], row_ptr[j+1])} coeffs[k] * w[cols[k]]
  *   for j < rows (CSR; replaces eval_r1cs_vectors, protocols/groth16/r1cs.py:77-92);
@@ -206,6 +220,16 @@ int zk_g2_msm_host(const uint64_t* bases, const uint64_t* scalars, uint64_t n, i
                    uint64_t out[16]);
 int zk_g2_mul_fixed_host(const uint64_t* q, const uint64_t* scalars, uint64_t n, int threads, uint64_t* out);
 int zk_g2_add_host(const uint64_t* p, const uint64_t* q, uint64_t out[16]);
+
+/* BN254 optimal-ate pairing on the host.  G1 points affine canonical (8
+ * words, zeros = infinity), G2 points affine canonical (16 words).
+ * zk_pairing_product writes prod_i e(P_i, Q_i) as 48 canonical words (Fq12
+ * = Fq6[w]/(w^2 - v), Fq6 = Fq2[v]/(v^3 - (9 + i))); zk_pairing_check sets
+ * *ok = 1 iff the product is 1.  Replaces pairing_g1_g2
+ * (crypto/pairing/bn254.py:10-11) in the Groth16 verifier
+ * (protocols/groth16/verify.py:11-23). */
+int zk_pairing_product(const uint64_t* g1, const uint64_t* g2, uint32_t n, uint64_t* out48);
+int zk_pairing_check(const uint64_t* g1, const uint64_t* g2, uint32_t n, int* ok);
 
 /* Tuning knobs: "ntt_max_pass_bits" (stages per NTT pass, 1..11; 0 = auto),
  * "msm_segment" (pairs per accumulation thread), "msm_reduce_chunk"

@@ -57,6 +57,8 @@ SIGNATURES = {
                         ct.POINTER(_vp), _vp],
     "zk_batch_run": [_vp, _vp, _u64p, _vp],
     "zk_batch_destroy": [_vp, _vp],
+    "zk_fr_powers": [_vp, _u64p, _u64p, ct.c_uint64, _vp, _vp],
+    "zk_synth_witness": [_vp, _vp, ct.c_uint32, ct.c_uint32, ct.c_int, _vp],
     "zk_r1cs_eval": [_vp, _vp, _vp, _vp, _vp, ct.c_uint64, _vp, _vp],
     "zk_fr_gather": [_vp, _vp, _vp, ct.c_uint64, _vp, _vp],
     "zk_fr_inv": [_vp, _vp, _vp, ct.c_uint64, _vp],
@@ -64,6 +66,8 @@ SIGNATURES = {
     "zk_g2_msm_host": [_vp, _vp, ct.c_uint64, ct.c_int, ct.c_int, _vp],
     "zk_g2_mul_fixed_host": [_vp, _vp, ct.c_uint64, ct.c_int, _vp],
     "zk_g2_add_host": [_vp, _vp, _vp],
+    "zk_pairing_product": [_vp, _vp, ct.c_uint32, _vp],
+    "zk_pairing_check": [_vp, _vp, ct.c_uint32, ct.POINTER(ct.c_int)],
     "zk_set_tuning": [_vp, ct.c_char_p, ct.c_int64],
     "zk_probe_mad_rate": [_vp, ct.POINTER(ct.c_double)],
     "zk_profile_enable": [_vp, ct.c_int],
@@ -293,6 +297,9 @@ class Context:
         w = list(out)
         f = lambda a: w[a] | (w[a + 1] << 64) | (w[a + 2] << 128) | (w[a + 3] << 192)
         return f(0), f(4), f(8)
+
+    def fr_powers(self, base_mont: int, scale_mont: int, n: int, out: int, stream=None) -> None:
+        self._check(self.lib.zk_fr_powers(self.handle, _words(base_mont), _words(scale_mont), int(n), out, stream))
 
     def r1cs_eval(self, row_ptr: int, cols: int, coeffs: int, w: int, rows: int, out: int, stream=None) -> None:
         self._check(self.lib.zk_r1cs_eval(self.handle, row_ptr, cols, coeffs, w, int(rows), out, stream))

@@ -393,6 +393,13 @@ int zk_batch_destroy(zk_ctx* ctx, zk_batch* b) {
   return ZK_OK;
 }
 
+int zk_fr_powers(zk_ctx* ctx, const uint64_t base_mont[4], const uint64_t scale_mont[4], uint64_t n, void* out,
+                 void* stream) {
+  CTX_CHECK(ctx);
+  if (!base_mont || !scale_mont || (n && !out)) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  return fr_powers_run(&ctx->c, base_mont, scale_mont, n, (fe*)out, pick(ctx, stream));
+}
+
 int zk_r1cs_eval(zk_ctx* ctx, const uint32_t* row_ptr, const uint32_t* cols, const void* coeffs, const void* w,
                  uint64_t rows, void* out, void* stream) {
   CTX_CHECK(ctx);

@@ -3,6 +3,7 @@
 // pointwise_mul_fr_mont / poly_sub_fr_mont (runtime/metal/runtime.py:204-224)
 // and of its host-side Montgomery packing (runtime/kernels/cpu/kernels.py:98-172),
 // which this library does on the device instead of in a host loop.
+#include <string.h>
 #include "internal.hpp"
 
 namespace zk {
@@ -73,6 +74,45 @@ int elementwise_scale(Ctx* c, const fe* a, const uint64_t c_mont[4], fe* out, ui
   unsigned grid = (unsigned)(want < 65536 ? want : 65536);
   scale_kernel<<<grid, 256, 0, st>>>(a, cst, out, n);
   ZK_HIP(c, hipGetLastError());
+  return ZK_OK;
+}
+
+}  // namespace zk
+
+namespace zk {
+
+// out[i] = scale * base^i (Montgomery Fr), two-level tables built on the host
+__global__ __launch_bounds__(256) void fr_powers_kernel(const fe* __restrict__ hi, const fe* __restrict__ lo,
+                                                        uint64_t n, fe* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) store_fe(out + i, Fr::mul(load_fe(hi + (i >> 10)), load_fe(lo + (i & 1023))));
+}
+
+int fr_powers_run(Ctx* c, const uint64_t base_mont[4], const uint64_t scale_mont[4], uint64_t n, fe* out,
+                  hipStream_t st) {
+  if (n == 0) return ZK_OK;
+  zkh::Fe b, s;
+  memcpy(b.v, base_mont, 32);
+  memcpy(s.v, scale_mont, 32);
+  std::vector<zkh::Fe> lo(1024), hi((n + 1023) / 1024);
+  zkh::Fe x = zkh::one(zkh::FR);
+  for (int i = 0; i < 1024; i++) {
+    lo[i] = x;
+    x = zkh::mul(zkh::FR, x, b);
+  }
+  zkh::Fe y = s;
+  for (size_t h = 0; h < hi.size(); h++) {
+    hi[h] = y;
+    y = zkh::mul(zkh::FR, y, x);
+  }
+  void* p;
+  int rc = ensure_scratch(c, 5, (hi.size() + 1024) * 32, &p);
+  if (rc) return rc;
+  ZK_HIP(c, hipMemcpyAsync(p, lo.data(), 1024 * 32, hipMemcpyHostToDevice, st));
+  ZK_HIP(c, hipMemcpyAsync((char*)p + 1024 * 32, hi.data(), hi.size() * 32, hipMemcpyHostToDevice, st));
+  fr_powers_kernel<<<grid_for(n, 256), 256, 0, st>>>((const fe*)p + 1024, (const fe*)p, n, out);
+  ZK_HIP(c, hipGetLastError());
+  ZK_HIP(c, hipStreamSynchronize(st));  // host tables die here
   return ZK_OK;
 }
 

@@ -92,6 +92,8 @@ enum EltOp { ELT_FR_TO_MONT = 0, ELT_FR_FROM_MONT, ELT_FQ_TO_MONT, ELT_FQ_FROM_M
              ELT_FR_MUL, ELT_FR_ADD, ELT_FR_SUB };
 int elementwise(Ctx* c, int op, const fe* a, const fe* b, fe* out, uint64_t n, hipStream_t st);
 int elementwise_scale(Ctx* c, const fe* a, const uint64_t c_mont[4], fe* out, uint64_t n, hipStream_t st);
+int fr_powers_run(Ctx* c, const uint64_t base_mont[4], const uint64_t scale_mont[4], uint64_t n, fe* out,
+                  hipStream_t st);
 
 // MSM ------------------------------------------------------------------------
 // A base set prepared for fixed-base MSMs: table[w][i] = 2^(c w) B_i.
