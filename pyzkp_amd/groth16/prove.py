@@ -1,0 +1,159 @@
+"""Groth16 prover on the device (reference: protocols/groth16/prove.py:30-128,
+prove_batch :133-248, QAP: protocols/groth16/qap.py:53-101).
+
+Per proof, on the device: the witness (canonical words) is converted to
+Montgomery form; A w, B w, C w are evaluated over the padded domain; a
+witness that does not satisfy the R1CS raises ValueError (the reference's
+non-zero remainder); H comes from 3 iNTTs, 3 coset NTTs (shift g = 5),
+(a b - c) / (g^n - 1) point-wise and one coset iNTT — the same quotient
+coefficients as the reference's poly_mul_ntt + div_by_xn_minus_1, since
+Z_H(g w^i) = g^n - 1 on the coset.  The four G1 MSMs (a_query . w,
+b_query_g1 . w, h_query . h[:n-1], l_query . w[aux]) run as one fixed-base
+batch over the key's query set; the G2 MSM (b_query_g2 . w) runs on host
+threads concurrently.  r and s are arguments so proofs are reproducible
+(the reference draws them with fr_rand, prove.py:84-85).
+"""
+from __future__ import annotations
+
+import threading
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+from .. import g2 as G2
+from ..curve import from_jacobian_mont, g1_add, g1_mul
+from ..field import FR_GENERATOR, FR_MODULUS, fr_inv, fr_to_mont
+from .types import Proof, ProvingKey
+
+
+class _Work:
+    """Per-key device buffers for k in-flight proofs."""
+
+    def __init__(self, pk: ProvingKey, k: int) -> None:
+        ctx = pk.ctx
+        n, nv, na = pk.n, pk.n_vars, len(pk.aux_ids)
+        self.seg = nv + n + na              # scalars of one proof: [w | h | l]
+        self.k = k
+        self.S = ctx.alloc(32 * self.seg * k)
+        self.abc = ctx.alloc(32 * 3 * n)
+        self.tmp = ctx.alloc(32 * n)
+
+    def free(self) -> None:
+        for b in (self.S, self.abc, self.tmp):
+            b.free()
+
+
+_WORK: dict = {}
+
+
+def _work(pk: ProvingKey, k: int) -> _Work:
+    key = (id(pk), k)
+    w = _WORK.get(key)
+    if w is None or w.S.ptr == 0:
+        w = _Work(pk, k)
+        _WORK[key] = w
+    return w
+
+
+def release_work(pk: ProvingKey) -> None:
+    for key in [k for k in _WORK if k[0] == id(pk)]:
+        _WORK.pop(key).free()
+
+
+def _device_part(pk: ProvingKey, wk: _Work, slot: int, w_words: np.ndarray) -> None:
+    """Fills scalars segment `slot` with [w | h | w[aux]] (Montgomery)."""
+    ctx = pk.ctx
+    n, nv = pk.n, pk.n_vars
+    if w_words.shape[0] != nv:
+        raise ValueError("witness length mismatch")
+    S = wk.S.ptr + 32 * wk.seg * slot
+    wbuf = np.ascontiguousarray(w_words, dtype="<u8")   # alive until the synchronising check below
+    ctx.h2d(S, wbuf.ctypes.data, 32 * nv)
+    ctx.fr_to_mont(S, S, nv)
+    a, b, c = wk.abc.ptr, wk.abc.ptr + 32 * n, wk.abc.ptr + 64 * n
+    for name, out in (("A", a), ("B", b), ("C", c)):
+        rp, cols, co = pk.csr[name]
+        ctx.r1cs_eval(rp.ptr, cols.ptr, co.ptr, S, n, out)
+    # R1CS satisfied on the domain <=> a b - c == 0 everywhere (qap.py:98-99)
+    ctx.fr_mul(a, b, wk.tmp.ptr, n)
+    ctx.fr_sub(wk.tmp.ptr, c, wk.tmp.ptr, n)
+    bad = ctx.fr_count_nonzero(wk.tmp.ptr, n)
+    del wbuf
+    if bad:
+        raise ValueError("witness does not satisfy R1CS (non-zero remainder)")
+    log_n = n.bit_length() - 1
+    g = fr_to_mont(FR_GENERATOR)
+    for x in (a, b, c):
+        ctx.ntt(x, n, x, log_n, True)                       # evals -> coefficients
+        ctx.ntt(x, n, x, log_n, False, shift_mont=g)        # coefficients -> coset evals
+    ctx.fr_mul(a, b, wk.tmp.ptr, n)
+    ctx.fr_sub(wk.tmp.ptr, c, wk.tmp.ptr, n)
+    z_inv = fr_inv((pow(FR_GENERATOR, n, FR_MODULUS) - 1) % FR_MODULUS)
+    ctx.fr_scale(wk.tmp.ptr, fr_to_mont(z_inv), wk.tmp.ptr, n)
+    ctx.ntt(wk.tmp.ptr, n, S + 32 * nv, log_n, True, shift_mont=g)   # h coefficients (h[n-1] = 0)
+    na = len(pk.aux_ids)
+    if na:
+        ctx.fr_gather(S, pk.aux_dev.ptr, na, S + 32 * (nv + n))
+
+
+def _msm_entries(pk: ProvingKey, wk: _Work, slots: Sequence[int]):
+    n, nv, na = pk.n, pk.n_vars, len(pk.aux_ids)
+    qa, qb, qh, ql = pk.query_offsets
+    offs, sizes, boffs = [], [], []
+    for sl in slots:
+        base = wk.seg * sl
+        offs += [base, base, base + nv, base + nv + n]
+        sizes += [nv, nv, n - 1, na]
+        boffs += [qa, qb, qh, ql]
+    return offs, sizes, boffs
+
+
+def _assemble(pk: ProvingKey, a_lin, b_lin_g1, h_acc, l_acc, b_lin_g2, r: int, s: int) -> Proof:
+    vk = pk.vk
+    a = g1_add(vk.alpha_g1, g1_add(a_lin, g1_mul(pk.delta_g1, r)))
+    b = G2.g2_add(vk.beta_g2, G2.g2_add(b_lin_g2, G2.g2_mul(vk.delta_g2, s)))
+    sa = g1_mul(g1_add(vk.alpha_g1, a_lin), s)
+    rb = g1_mul(g1_add(pk.beta_g1, b_lin_g1), r)
+    rs_delta = g1_mul(pk.delta_g1, r * s % FR_MODULUS)
+    c = g1_add(g1_add(l_acc, h_acc), g1_add(g1_add(sa, rb), rs_delta))
+    return Proof(a=_norm(a), b=b, c=_norm(c))
+
+
+def _norm(p):
+    from ..curve import from_affine, normalize
+    return from_affine(normalize(p))
+
+
+def prove_many(pk: ProvingKey, witnesses: Sequence[np.ndarray], rs: Sequence[Tuple[int, int]],
+               g2_threads: int = 0) -> List[Proof]:
+    """Proofs for several witnesses of one key; their G1 MSMs share one
+    fixed-base batch launch and their G2 MSMs run on host threads while the
+    device works."""
+    k = len(witnesses)
+    if k == 0:
+        return []
+    wk = _work(pk, k)
+    g2_res: List = [None] * k
+
+    def g2_job():
+        for i, w in enumerate(witnesses):
+            g2_res[i] = G2.g2_msm_words(pk.b_query_g2, w, pk.n_vars, threads=g2_threads)
+
+    t = threading.Thread(target=g2_job)
+    t.start()
+    try:
+        for i, w in enumerate(witnesses):
+            _device_part(pk, wk, i, w)
+        offs, sizes, boffs = _msm_entries(pk, wk, range(k))
+        res = pk.ctx.msm_g1_fixed_batch(pk.fixed, wk.S.ptr, offs, sizes, boffs)
+    finally:
+        t.join()
+    proofs = []
+    for i in range(k):
+        a_lin, b_lin_g1, h_acc, l_acc = (from_jacobian_mont(*res[4 * i + j]) for j in range(4))
+        proofs.append(_assemble(pk, a_lin, b_lin_g1, h_acc, l_acc, g2_res[i], *rs[i]))
+    return proofs
+
+
+def prove(pk: ProvingKey, w_words: np.ndarray, r: int, s: int, g2_threads: int = 0) -> Proof:
+    return prove_many(pk, [w_words], [(r, s)], g2_threads=g2_threads)[0]
