@@ -221,6 +221,11 @@ int zk_g2_msm_host(const uint64_t* bases, const uint64_t* scalars, uint64_t n, i
 int zk_g2_mul_fixed_host(const uint64_t* q, const uint64_t* scalars, uint64_t n, int threads, uint64_t* out);
 int zk_g2_add_host(const uint64_t* p, const uint64_t* q, uint64_t out[16]);
 
+/* out = [k] P on the host for one G1 point (affine canonical, 8 words, zeros
+ * = infinity; k canonical).  Replaces the g1_mul calls of proof assembly
+ * (protocols/groth16/prove.py:119-126, crypto/ecc/bn254.py:15-16). */
+int zk_g1_mul_host(const uint64_t p[8], const uint64_t k[4], uint64_t out[8]);
+
 /* BN254 optimal-ate pairing on the host.  G1 points affine canonical (8
  * words, zeros = infinity), G2 points affine canonical (16 words).
  * zk_pairing_product writes prod_i e(P_i, Q_i) as 48 canonical words (Fq12

@@ -268,6 +268,25 @@ int zk_g2_mul_fixed_host(const uint64_t* q, const uint64_t* scalars, uint64_t n,
   if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
   if (threads < 1) threads = 1;
   g2::A2 base = g2::rd_aff(q);
+  if (n < 64) {
+    // few scalars: 4-bit fixed windows over 16 multiples, no table set-up
+    g2::J2 m[16];
+    m[0] = g2::j_inf();
+    for (int d = 1; d < 16; d++) m[d] = base.inf ? g2::j_inf() : g2::j_madd(m[d - 1], base.x, base.y);
+    for (uint64_t i = 0; i < n; i++) {
+      zkh::Fe k;
+      memcpy(k.v, scalars + 4 * i, 32);
+      while (zkh::geq(k.v, zkh::FR.p)) zkh::sub_raw(k.v, k.v, zkh::FR.p);
+      g2::J2 acc = g2::j_inf();
+      for (int j = 63; j >= 0; j--) {
+        for (int t = 0; t < 4; t++) acc = g2::j_dbl(acc);
+        uint32_t d = (uint32_t)((k.v[j >> 4] >> (4 * (j & 15))) & 0xf);
+        if (d) acc = g2::j_add(acc, m[d]);
+      }
+      g2::wr_jac_affine(out + 16 * i, acc);
+    }
+    return ZK_OK;
+  }
   // table[j][d] = d 2^(8 j) Q as Jacobian, j < 32, d < 256
   std::vector<g2::J2> tab(32 * 256, g2::j_inf());
   if (!base.inf) {

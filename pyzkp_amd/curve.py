@@ -133,6 +133,25 @@ def g1_mul(p: G1Point, k: int) -> G1Point:
     return acc
 
 
+def g1_mul_native(p: G1Point, k: int) -> G1Point:
+    """[k] P through the library's host multiplier (one call, no Python
+    double-and-add); same result as g1_mul."""
+    import numpy as np
+    from ._native import load_library
+    from .field import FR_MODULUS, ints_to_words, words_to_ints
+    a = normalize(p)
+    src = np.zeros(8, dtype="<u8")
+    if a is not None:
+        src[:] = ints_to_words([a[0], a[1]]).reshape(-1)
+    kw = ints_to_words([int(k) % FR_MODULUS]).reshape(-1).astype("<u8")
+    out = np.zeros(8, dtype="<u8")
+    rc = load_library().zk_g1_mul_host(src.ctypes.data, kw.ctypes.data, out.ctypes.data)
+    if rc:
+        raise ValueError("zk_g1_mul_host failed")
+    x, y = words_to_ints(out.reshape(2, 4))
+    return from_affine(None if x == 0 and y == 0 else (x, y))
+
+
 def fixed_base_table(p: G1Point = G1_GENERATOR) -> bytes:
     """T[j][d] = [d * 2^(8j)] p for j < 32, d < 256, packed as 64-byte affine
     Montgomery points (the table zk_g1_fixed_base expects)."""

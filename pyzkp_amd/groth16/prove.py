@@ -21,7 +21,7 @@ from typing import List, Sequence, Tuple
 import numpy as np
 
 from .. import g2 as G2
-from ..curve import from_jacobian_mont, g1_add, g1_mul
+from ..curve import from_jacobian_mont, g1_add, g1_mul_native as g1_mul
 from ..field import FR_GENERATOR, FR_MODULUS, fr_inv, fr_to_mont
 from .types import Proof, ProvingKey
 
@@ -47,17 +47,26 @@ _WORK: dict = {}
 
 
 def _work(pk: ProvingKey, k: int) -> _Work:
-    key = (id(pk), k)
-    w = _WORK.get(key)
-    if w is None or w.S.ptr == 0:
+    """Per-key workspace with room for at least k proofs (grown, never shrunk)."""
+    w = _WORK.get(id(pk))
+    if w is None or w.S.ptr == 0 or w.k < k:
+        if w is not None:
+            pk.ctx.sync()
+            w.free()
         w = _Work(pk, k)
-        _WORK[key] = w
+        _WORK[id(pk)] = w
     return w
 
 
+def segment_bytes(pk: ProvingKey) -> int:
+    """Device bytes one in-flight proof of this key occupies ([w | h | l])."""
+    return 32 * (pk.n_vars + pk.n + len(pk.aux_ids))
+
+
 def release_work(pk: ProvingKey) -> None:
-    for key in [k for k in _WORK if k[0] == id(pk)]:
-        _WORK.pop(key).free()
+    w = _WORK.pop(id(pk), None)
+    if w is not None:
+        w.free()
 
 
 def _device_part(pk: ProvingKey, wk: _Work, slot: int, w_words: np.ndarray) -> None:
@@ -124,6 +133,31 @@ def _norm(p):
     return from_affine(normalize(p))
 
 
+def g2_part(pk: ProvingKey, w_words: np.ndarray, threads: int = 0):
+    """b_query_g2 . w on host threads (the reference's MSM_G2 on CPU)."""
+    return G2.g2_msm_words(pk.b_query_g2, w_words, pk.n_vars, threads=threads)
+
+
+def device_batch(pk: ProvingKey, witnesses: Sequence[np.ndarray]) -> List[tuple]:
+    """Device part of several proofs of one key: per proof the H pipeline,
+    then one fixed-base batch for all their G1 MSMs.  Returns per proof
+    (a_lin, b_lin_g1, h_acc, l_acc)."""
+    k = len(witnesses)
+    if k == 0:
+        return []
+    wk = _work(pk, k)
+    for i, w in enumerate(witnesses):
+        _device_part(pk, wk, i, w)
+    offs, sizes, boffs = _msm_entries(pk, wk, range(k))
+    res = pk.ctx.msm_g1_fixed_batch(pk.fixed, wk.S.ptr, offs, sizes, boffs)
+    return [tuple(from_jacobian_mont(*res[4 * i + j]) for j in range(4)) for i in range(k)]
+
+
+def assemble(pk: ProvingKey, g1_parts: tuple, b_lin_g2, r: int, s: int) -> Proof:
+    a_lin, b_lin_g1, h_acc, l_acc = g1_parts
+    return _assemble(pk, a_lin, b_lin_g1, h_acc, l_acc, b_lin_g2, r, s)
+
+
 def prove_many(pk: ProvingKey, witnesses: Sequence[np.ndarray], rs: Sequence[Tuple[int, int]],
                g2_threads: int = 0) -> List[Proof]:
     """Proofs for several witnesses of one key; their G1 MSMs share one
@@ -132,27 +166,19 @@ def prove_many(pk: ProvingKey, witnesses: Sequence[np.ndarray], rs: Sequence[Tup
     k = len(witnesses)
     if k == 0:
         return []
-    wk = _work(pk, k)
     g2_res: List = [None] * k
 
     def g2_job():
         for i, w in enumerate(witnesses):
-            g2_res[i] = G2.g2_msm_words(pk.b_query_g2, w, pk.n_vars, threads=g2_threads)
+            g2_res[i] = g2_part(pk, w, g2_threads)
 
     t = threading.Thread(target=g2_job)
     t.start()
     try:
-        for i, w in enumerate(witnesses):
-            _device_part(pk, wk, i, w)
-        offs, sizes, boffs = _msm_entries(pk, wk, range(k))
-        res = pk.ctx.msm_g1_fixed_batch(pk.fixed, wk.S.ptr, offs, sizes, boffs)
+        parts = device_batch(pk, witnesses)
     finally:
         t.join()
-    proofs = []
-    for i in range(k):
-        a_lin, b_lin_g1, h_acc, l_acc = (from_jacobian_mont(*res[4 * i + j]) for j in range(4))
-        proofs.append(_assemble(pk, a_lin, b_lin_g1, h_acc, l_acc, g2_res[i], *rs[i]))
-    return proofs
+    return [assemble(pk, parts[i], g2_res[i], *rs[i]) for i in range(k)]
 
 
 def prove(pk: ProvingKey, w_words: np.ndarray, r: int, s: int, g2_threads: int = 0) -> Proof:

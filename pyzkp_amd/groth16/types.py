@@ -12,6 +12,21 @@ class Proof:
     b: Any  # G2 ((c0, c1) x3)
     c: Any  # G1
 
+    def to_bytes(self) -> bytes:
+        """A || B || C as affine big-endian coordinates, 32 bytes each: G1 as
+        x || y (the transcript encoding, protocols/plonk/transcript.py:13-19),
+        G2 as x.c0 || x.c1 || y.c0 || y.c1; the identity is all zeros."""
+        from ..curve import normalize
+        from ..g2 import normalize as normalize2
+
+        def be(v: int) -> bytes:
+            return int(v).to_bytes(32, "big")
+
+        a, c, b = normalize(self.a), normalize(self.c), normalize2(self.b)
+        g1 = lambda p: bytes(64) if p is None else be(p[0]) + be(p[1])
+        g2 = bytes(128) if b is None else be(b[0][0]) + be(b[0][1]) + be(b[1][0]) + be(b[1][1])
+        return g1(a) + g2 + g1(c)
+
 
 @dataclass(frozen=True)
 class VerifyingKey:

@@ -5,7 +5,7 @@ from __future__ import annotations
 
 from typing import Sequence
 
-from ..curve import G1_ZERO, g1_add, g1_mul, g1_neg
+from ..curve import G1_ZERO, g1_add, g1_mul_native as g1_mul, g1_neg
 from ..field import FR_MODULUS
 from ..pairing import pairing_check
 from .types import Proof, VerifyingKey
