@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["msm", "ntt", "batch", "serve"], default="msm")
+    ap.add_argument("--workload", choices=["msm", "ntt", "batch", "serve", "copy"], default="msm")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of proof requests")
     ap.add_argument("--serve-log", type=str, default="10,20", help="serve: constraint range 2^lo..2^hi")
     ap.add_argument("--cpu-threads", type=int, default=16, help="serve: host threads for G2 MSMs")
@@ -55,6 +55,20 @@ def parse():
     ap.add_argument("--cpu-sample-log", type=int, default=18)
     ap.add_argument("--no-ntt", action="store_true")
     return ap.parse_args()
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` for this workload, read from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a
+    streaming copy kernel); None when that workload was not profiled."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)[workload][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return {"bytes": t["read_bytes"] + t["write_bytes"], "read_bytes": t["read_bytes"],
+            "write_bytes": t["write_bytes"], "source": "profiles/hbm_traffic.json (rocprofv3 --pmc)"}
 
 
 def ntt_block(ctx, log_n, steps, warmup):
@@ -177,6 +191,27 @@ def run_batch(args, ctx, peak_mads):
         b.free()
 
 
+def run_copy(args):
+    """Counter calibration: a plain streaming kernel of known size (out = ~in
+    over 2^28 int32 = 1 GiB read + 1 GiB written), so that the FETCH_SIZE /
+    WRITE_SIZE counters of the hot kernels can be scaled to bytes."""
+    import torch
+    n = 1 << 28
+    a = torch.randint(0, 1 << 30, (n,), dtype=torch.int32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(args.warmup):
+        torch.bitwise_not(a, out=b)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        torch.bitwise_not(a, out=b)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    print(json.dumps({"metric": "streaming copy GB/s (counter calibration)", "value": round(2 * 4 * n / dt / 1e9, 1),
+                      "unit": "GB/s", "bytes_read_per_launch": 4 * n, "bytes_written_per_launch": 4 * n,
+                      "ms_per_step": round(dt * 1e3, 4), "steps": args.steps, "warmup": args.warmup}))
+
+
 def run_serve(args, ctx):
     """Served Groth16: Poisson arrivals (seed 0x5001) of requests whose circuit
     has floor(2^U) constraints, U ~ U[lo, hi] (seed 0x5002), padded to 2^k;
@@ -292,6 +327,10 @@ def main():
     from pyzkp_amd.curve import G1_GENERATOR, from_jacobian_mont, g1_eq, g1_mul
     from pyzkp_amd.field import FR_MODULUS, words_to_ints
 
+    if args.workload == "copy":
+        run_copy(args)
+        return
+
     ctx = get_context(0)
     log("probing v_mad_u64_u32 issue rate")
     peak_mads = ctx.probe_mad_rate()
@@ -321,7 +360,9 @@ def main():
                        "variants": r},
             "roofline": {"bound": "valu", "achieved": round(mads / (fwd["device_ms"] / 1e3) / 1e12, 3),
                          "peak": round(peak_mads / 1e12, 3), "unit": "T lane-mads/s",
-                         "frac": round(mads / (fwd["device_ms"] / 1e3) / peak_mads, 4), "traffic": None,
+                         "frac": round(mads / (fwd["device_ms"] / 1e3) / peak_mads, 4),
+                         "traffic": measured_traffic(f"ntt_2^{log_n}", "ntt_pass_kernel"),
+                         "kernel": "ntt_pass_kernel",
                          "hbm_GBps_algorithmic": fwd["GBps"], "hbm_frac": round(fwd["GBps"] / 8000.0, 4)},
             "cpu_baseline": None,
         }
@@ -381,7 +422,8 @@ def main():
     acc_s = phases["msm_accumulate"] / 1e3
     achieved = algo_mads / acc_s
     roofline = {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": round(peak_mads / 1e12, 3),
-                "unit": "T lane-mads/s", "frac": round(achieved / peak_mads, 4), "traffic": None,
+                "unit": "T lane-mads/s", "frac": round(achieved / peak_mads, 4),
+                "traffic": measured_traffic(f"msm_2^{log_n}_c{c}", "msm_accumulate_kernel"),
                 "kernel": "msm_accumulate_kernel",
                 "work_per_launch": f"n*W*10 Fq-mul*136 = {algo_mads:.3e} v_mad_u64_u32"}
 
