@@ -137,20 +137,36 @@ __global__ __launch_bounds__(256) void msm_digits_kernel(const fe* __restrict__ 
   }
 }
 
-// 2. counting sort of the (row, |d|) pairs.  A row is one window of a plain
-//    MSM, or one whole MSM of a fixed-base batch (all its windows share one
-//    bucket set).  MsmRow describes where a row's digits live and how a
-//    position p inside the row maps to a base index:
-//      base = (p / inner) * stride + p % inner.
-//    A tile of SORT_TILE digits of one row is histogrammed in LDS with
-//    16-bit counters (two per word; a tile has at most 2^15 entries, so a
-//    counter cannot overflow).  Bins are |d| - 1 (zero digits are dropped);
-//    rows with more than SORT_RANGE bins are covered by several bin ranges
-//    (grid z), each re-reading the tile.
-//    tile_hist[(row.tile0 + tile) * nbins + bin] = count
-static constexpr uint32_t SORT_TILE = 1u << 15;
-static constexpr uint32_t SORT_RANGE = 1u << 15;  // bins per LDS pass (64 KiB)
-static constexpr uint32_t SORT_THREADS = 1024;
+// 2. bucket sort of the (row, |d|) pairs in two levels.  A row is one window
+//    of a plain MSM, or one whole MSM of a fixed-base batch (all its windows
+//    share one bucket set).  MsmRow describes where a row's digits live and
+//    how a position p inside the row maps to a base index:
+//      base = (p / inner) * stride + boff + p % inner.
+//    Bins are |d| - 1 (zero digits are dropped).  A bin splits into a coarse
+//    part k = bin >> S (at most SORT_MAX_COARSE values) and a fine part
+//    f = bin & (2^S - 1), so that every level works with few counters:
+//      a. count: each tile of SORT_TILE digits of a row histograms its coarse
+//         keys in LDS into hist[col(r, k) + tile], col(r, k) =
+//         tile0(r) NC + k ntiles(r), i.e. columns in (row, coarse key) order
+//         with the row's tiles contiguous;
+//      b. one exclusive scan of the whole histogram gives every (row, coarse
+//         key, tile) its output offset (the sorted order is (row, k, f));
+//      c. scatter: a tile ranks its digits per coarse key in LDS, stages them
+//         in coarse-key order and writes runs of consecutive slots, so global
+//         writes are contiguous per (tile, k) instead of one 4-byte store per
+//         bucket;
+//      d. fine: each (row, k) segment is cut into chunks; the chunks count
+//         their fine keys (count / scan / scatter again, so one segment that
+//         holds a huge bucket is spread over many workgroups), write the CSR
+//         row pointers of the segment's 2^S buckets and place each entry at
+//         its bucket position.
+static constexpr uint32_t SORT_TILE = 1u << 13;
+static constexpr uint32_t SORT_THREADS = 512;
+static constexpr uint32_t SORT_ITEMS = SORT_TILE / SORT_THREADS;
+static constexpr uint32_t SORT_MAX_COARSE = 2 * SORT_THREADS;
+static constexpr uint32_t SORT_MAX_FINE_BITS = 13;   // 2^13 u32 LDS counters = 32 KiB
+static constexpr uint32_t FINE_THREADS = 256;
+static constexpr uint32_t FINE_CHUNK = 1u << 14;   // entries per fine-level workgroup (at least)
 
 struct MsmRow {
   uint64_t off;     // first digit of the row
@@ -158,62 +174,32 @@ struct MsmRow {
   uint32_t inner;   // digits per window inside the row
   uint32_t stride;  // base-index stride between windows
   uint32_t ntiles;  // ceil(len / SORT_TILE)
-  uint32_t tile0;   // first tile slot of the row in the tile histogram
+  uint32_t tile0;   // first tile slot of the row
   uint32_t boff;    // base-index offset of the row (fixed-base ranges)
 };
 
-__global__ __launch_bounds__(SORT_THREADS) void msm_tile_count_kernel(const uint32_t* __restrict__ digits,
+__global__ __launch_bounds__(SORT_THREADS) void msm_sort_count_kernel(const uint32_t* __restrict__ digits,
                                                                       const MsmRow* __restrict__ rows,
-                                                                      uint32_t nbins,
-                                                                      uint32_t* __restrict__ tile_hist) {
-  __shared__ uint32_t cnt[SORT_RANGE / 2];
+                                                                      uint32_t S, uint32_t NC,
+                                                                      uint32_t* __restrict__ hist) {
+  __shared__ uint32_t cnt[SORT_MAX_COARSE];
   const MsmRow row = rows[blockIdx.y];
   const uint32_t tile = blockIdx.x;
   if (tile >= row.ntiles) return;
-  const uint32_t lo = blockIdx.z * SORT_RANGE;
-  const uint32_t hi = (lo + SORT_RANGE < nbins) ? lo + SORT_RANGE : nbins;
-  for (uint32_t k = threadIdx.x; k < SORT_RANGE / 2; k += SORT_THREADS) cnt[k] = 0;
+  for (uint32_t k = threadIdx.x; k < NC; k += SORT_THREADS) cnt[k] = 0;
   __syncthreads();
   const uint32_t p0 = tile * SORT_TILE;
   const uint32_t p1 = (p0 + SORT_TILE < row.len) ? p0 + SORT_TILE : row.len;
   for (uint32_t p = p0 + threadIdx.x; p < p1; p += SORT_THREADS) {
     const uint32_t mag = digits[row.off + p] & ~SIGN;
-    if (mag == 0) continue;
-    const uint32_t bin = mag - 1;
-    if (bin < lo || bin >= hi) continue;
-    const uint32_t b = bin - lo;
-    atomicAdd(&cnt[b >> 1], 1u << ((b & 1) * 16));
+    if (mag) atomicAdd(&cnt[(mag - 1) >> S], 1u);
   }
   __syncthreads();
-  uint32_t* out = tile_hist + (uint64_t)(row.tile0 + tile) * nbins;
-  for (uint32_t b = lo + threadIdx.x; b < hi; b += SORT_THREADS) {
-    const uint32_t k = b - lo;
-    out[b] = (cnt[k >> 1] >> ((k & 1) * 16)) & 0xffffu;
-  }
+  for (uint32_t k = threadIdx.x; k < NC; k += SORT_THREADS)
+    hist[(uint64_t)row.tile0 * NC + (uint64_t)k * row.ntiles + tile] = cnt[k];
 }
 
-// exclusive prefix of each (row, bin) column over the row's tiles, in place;
-// the column total becomes counts[row * B + bin + 1] (bucket |d| = bin + 1)
-__global__ __launch_bounds__(256) void msm_tile_prefix_kernel(uint32_t* __restrict__ tile_hist,
-                                                              const MsmRow* __restrict__ rows,
-                                                              uint32_t nrows, uint32_t nbins, uint32_t B,
-                                                              uint32_t* __restrict__ counts) {
-  const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (uint64_t)nrows * nbins) return;
-  const uint32_t r = (uint32_t)(gid / nbins), bin = (uint32_t)(gid - (uint64_t)r * nbins);
-  const MsmRow row = rows[r];
-  uint32_t run = 0;
-  uint32_t* col = tile_hist + (uint64_t)row.tile0 * nbins + bin;
-  for (uint32_t t = 0; t < row.ntiles; t++) {
-    const uint32_t v = col[(uint64_t)t * nbins];
-    col[(uint64_t)t * nbins] = run;
-    run += v;
-  }
-  counts[r * B + bin + 1] = run;
-  if (bin == 0) counts[r * B] = 0;
-}
-
-// 2. exclusive scan of the bucket counts -> row pointers (3 small kernels)
+// exclusive scan building blocks (a hierarchical scan over any length)
 static constexpr int SCAN_THREADS = 256;
 static constexpr int SCAN_PER_THREAD = 8;
 static constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_PER_THREAD;
@@ -234,12 +220,13 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
   return incl - v;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void scan_chunks_kernel(const uint32_t* __restrict__ in,
-                                                                   uint32_t* __restrict__ out,
-                                                                   uint32_t* __restrict__ sums,
-                                                                   uint32_t len) {
+// out[i] = exclusive prefix of in[] within its chunk of SCAN_CHUNK; the
+// chunk total goes to sums[chunk]
+// (in and out may be the same buffer: a block loads its chunk before it stores)
+__global__ __launch_bounds__(SCAN_THREADS) void scan_chunks_kernel(const uint32_t* in, uint32_t* out,
+                                                                   uint32_t* sums, uint64_t len) {
   __shared__ uint32_t sh[SCAN_THREADS];
-  const uint32_t base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER_THREAD;
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER_THREAD;
   uint32_t v[SCAN_PER_THREAD];
   uint32_t local = 0;
 #pragma unroll
@@ -257,71 +244,179 @@ __global__ __launch_bounds__(SCAN_THREADS) void scan_chunks_kernel(const uint32_
   if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// one block scans all chunk sums (at most SCAN_CHUNK of them) in place and
-// writes the grand total to out_total
-__global__ __launch_bounds__(SCAN_THREADS) void scan_sums_kernel(uint32_t* __restrict__ sums,
-                                                                 uint32_t nsums,
-                                                                 uint32_t* __restrict__ out_total) {
-  __shared__ uint32_t sh[SCAN_THREADS];
-  const uint32_t base = threadIdx.x * SCAN_PER_THREAD;
-  uint32_t v[SCAN_PER_THREAD];
-  uint32_t local = 0;
-#pragma unroll
-  for (int k = 0; k < SCAN_PER_THREAD; k++) {
-    v[k] = (base + k < nsums) ? sums[base + k] : 0;
-    local += v[k];
-  }
-  uint32_t total;
-  uint32_t run = block_exclusive_scan(local, sh, total);
-#pragma unroll
-  for (int k = 0; k < SCAN_PER_THREAD; k++) {
-    if (base + k < nsums) sums[base + k] = run;
-    run += v[k];
-  }
-  if (threadIdx.x == 0) *out_total = total;
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void scan_add_kernel(uint32_t* __restrict__ out,
-                                                                const uint32_t* __restrict__ sums,
-                                                                uint32_t len) {
-  const uint32_t base = blockIdx.x * SCAN_CHUNK;
+// out[i] += sums[chunk of i]; out[len] = sums[number of chunks] (the total)
+__global__ __launch_bounds__(SCAN_THREADS) void scan_add_kernel(uint32_t* out, const uint32_t* sums,
+                                                                uint64_t len) {
+  const uint64_t base = (uint64_t)blockIdx.x * SCAN_CHUNK;
   const uint32_t add = sums[blockIdx.x];
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[len] = sums[gridDim.x];
   for (uint32_t k = threadIdx.x; k < SCAN_CHUNK; k += SCAN_THREADS) {
-    const uint32_t idx = base + k;
+    const uint64_t idx = base + k;
     if (idx < len) out[idx] += add;
   }
 }
 
-// 3. scatter: sorted[row_ptr[row B + |d|] + tile prefix + rank in tile] =
-//    base index | sign; ranks come from 16-bit LDS counters as in the count pass
-__global__ __launch_bounds__(SORT_THREADS) void msm_tile_scatter_kernel(
-    const uint32_t* __restrict__ digits, const MsmRow* __restrict__ rows, uint32_t nbins, uint32_t B,
-    const uint32_t* __restrict__ tile_hist, const uint32_t* __restrict__ row_ptr,
-    uint32_t* __restrict__ sorted) {
-  __shared__ uint32_t cnt[SORT_RANGE / 2];
+// c. scatter of one tile into coarse-key order
+__global__ __launch_bounds__(SORT_THREADS) void msm_sort_scatter_kernel(
+    const uint32_t* __restrict__ digits, const MsmRow* __restrict__ rows, uint32_t S, uint32_t NC,
+    const uint32_t* __restrict__ hoff, uint32_t* __restrict__ items_v, uint16_t* __restrict__ items_f) {
+  __shared__ uint32_t cnt[SORT_MAX_COARSE];
+  __shared__ uint32_t cstart[SORT_MAX_COARSE];
+  __shared__ uint32_t gbase[SORT_MAX_COARSE];
+  __shared__ uint32_t stage_v[SORT_TILE];
+  __shared__ uint16_t stage_f[SORT_TILE];
+  __shared__ uint16_t stage_k[SORT_TILE];
+  __shared__ uint32_t scan_sh[SORT_THREADS];
   const MsmRow row = rows[blockIdx.y];
   const uint32_t tile = blockIdx.x;
   if (tile >= row.ntiles) return;
-  const uint32_t lo = blockIdx.z * SORT_RANGE;
-  const uint32_t hi = (lo + SORT_RANGE < nbins) ? lo + SORT_RANGE : nbins;
-  for (uint32_t k = threadIdx.x; k < SORT_RANGE / 2; k += SORT_THREADS) cnt[k] = 0;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t k = tid; k < NC; k += SORT_THREADS) cnt[k] = 0;
   __syncthreads();
-  const uint32_t* pre = tile_hist + (uint64_t)(row.tile0 + tile) * nbins;
-  const uint32_t* rp = row_ptr + (uint64_t)blockIdx.y * B + 1;
   const uint32_t p0 = tile * SORT_TILE;
-  const uint32_t p1 = (p0 + SORT_TILE < row.len) ? p0 + SORT_TILE : row.len;
-  for (uint32_t p = p0 + threadIdx.x; p < p1; p += SORT_THREADS) {
-    const uint32_t d = digits[row.off + p];
-    const uint32_t mag = d & ~SIGN;
-    if (mag == 0) continue;
-    const uint32_t bin = mag - 1;
-    if (bin < lo || bin >= hi) continue;
-    const uint32_t b = bin - lo;
-    const uint32_t sh = (b & 1) * 16;
-    const uint32_t rank = (atomicAdd(&cnt[b >> 1], 1u << sh) >> sh) & 0xffffu;
+  const uint32_t fmask = (1u << S) - 1;
+  uint32_t d[SORT_ITEMS], rank[SORT_ITEMS];
+#pragma unroll
+  for (uint32_t i = 0; i < SORT_ITEMS; i++) {
+    const uint32_t p = p0 + i * SORT_THREADS + tid;
+    d[i] = (p < row.len) ? digits[row.off + p] : 0u;
+    const uint32_t mag = d[i] & ~SIGN;
+    rank[i] = mag ? atomicAdd(&cnt[(mag - 1) >> S], 1u) : 0u;
+  }
+  __syncthreads();
+  // exclusive scan of the NC coarse counts, two per thread (NC <= 2 SORT_THREADS)
+  uint32_t total;
+  const uint32_t k0 = 2 * tid, k1 = 2 * tid + 1;
+  const uint32_t a = (k0 < NC) ? cnt[k0] : 0u, b = (k1 < NC) ? cnt[k1] : 0u;
+  const uint32_t ex = block_exclusive_scan(a + b, scan_sh, total);
+  const uint64_t hrow = (uint64_t)row.tile0 * NC + tile;
+  if (k0 < NC) {
+    cstart[k0] = ex;
+    gbase[k0] = hoff[hrow + (uint64_t)k0 * row.ntiles] - ex;
+  }
+  if (k1 < NC) {
+    cstart[k1] = ex + a;
+    gbase[k1] = hoff[hrow + (uint64_t)k1 * row.ntiles] - (ex + a);
+  }
+  __syncthreads();
+#pragma unroll
+  for (uint32_t i = 0; i < SORT_ITEMS; i++) {
+    const uint32_t mag = d[i] & ~SIGN;
+    if (!mag) continue;
+    const uint32_t p = p0 + i * SORT_THREADS + tid;
+    const uint32_t k = (mag - 1) >> S;
+    const uint32_t slot = cstart[k] + rank[i];
     const uint32_t wi = p / row.inner;
-    const uint32_t base = wi * row.stride + row.boff + (p - wi * row.inner);
-    sorted[rp[bin] + pre[bin] + rank] = base | (d & SIGN);
+    stage_v[slot] = (wi * row.stride + row.boff + (p - wi * row.inner)) | (d[i] & SIGN);
+    stage_f[slot] = (uint16_t)((mag - 1) & fmask);
+    stage_k[slot] = (uint16_t)k;
+  }
+  __syncthreads();
+  for (uint32_t slot = tid; slot < total; slot += SORT_THREADS) {
+    const uint32_t g = gbase[stage_k[slot]] + slot;
+    items_v[g] = stage_v[slot];
+    items_f[g] = stage_f[slot];
+  }
+}
+
+// d. fine level.  Segment g = (row r, coarse key k) holds the entries
+//    [s0, s1) of the coarse order.  It is cut into chunks of FINE_CHUNK
+//    entries (at least one chunk, so empty segments still write their row
+//    pointers); chunk j of g is one workgroup, which counts the fine keys
+//    in LDS into fhist[fc0(g) NF + f nch(g) + j] (columns in (segment, f)
+//    order, chunks contiguous).  One exclusive scan of fhist then gives the
+//    output offset of every (segment, f, chunk), and the bucket start of
+//    (r, bin = k 2^S + f) is the value at chunk 0.  A segment holding one
+//    huge bucket is thus spread over many workgroups.
+__device__ __forceinline__ void fine_segment(const uint32_t* __restrict__ hoff, const MsmRow* __restrict__ rows,
+                                             uint32_t NC, uint32_t g, uint32_t& r, uint32_t& k, uint32_t& s0,
+                                             uint32_t& s1) {
+  r = g / NC;
+  k = g - r * NC;
+  const MsmRow row = rows[r];
+  const uint64_t col = (uint64_t)row.tile0 * NC + (uint64_t)k * row.ntiles;
+  s0 = hoff[col];
+  s1 = hoff[col + row.ntiles];  // the next column's start (the grand total after the last)
+}
+
+__global__ __launch_bounds__(256) void msm_fine_chunks_kernel(const uint32_t* __restrict__ hoff,
+                                                              const MsmRow* __restrict__ rows, uint32_t NC,
+                                                              uint32_t nseg, uint32_t fch,
+                                                              uint32_t* __restrict__ nch) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nseg) return;
+  uint32_t r, k, s0, s1;
+  fine_segment(hoff, rows, NC, g, r, k, s0, s1);
+  const uint32_t m = (s1 - s0 + fch - 1) / fch;
+  nch[g] = m ? m : 1u;
+}
+
+// workgroup b -> (segment g, chunk j) with fc0[g] <= b < fc0[g + 1]; false
+// for the spare workgroups of the upper-bound grid
+__device__ __forceinline__ bool fine_chunk(const uint32_t* __restrict__ fc0, uint32_t nseg, uint32_t b,
+                                           uint32_t& g, uint32_t& j, uint32_t& nc) {
+  if (b >= fc0[nseg]) return false;
+  uint32_t lo = 0, hi = nseg;  // fc0[lo] <= b < fc0[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (fc0[mid] <= b) lo = mid; else hi = mid;
+  }
+  g = lo;
+  j = b - fc0[g];
+  nc = fc0[g + 1] - fc0[g];
+  return true;
+}
+
+__global__ __launch_bounds__(FINE_THREADS) void msm_fine_count_kernel(
+    const uint16_t* __restrict__ items_f, const uint32_t* __restrict__ hoff, const MsmRow* __restrict__ rows,
+    uint32_t S, uint32_t NC, uint32_t nseg, uint32_t fch, const uint32_t* __restrict__ fc0,
+    uint32_t* __restrict__ fhist) {
+  extern __shared__ uint32_t fcnt[];
+  uint32_t g, j, nc;
+  if (!fine_chunk(fc0, nseg, blockIdx.x, g, j, nc)) return;
+  uint32_t r, k, s0, s1;
+  fine_segment(hoff, rows, NC, g, r, k, s0, s1);
+  const uint32_t NF = 1u << S;
+  for (uint32_t f = threadIdx.x; f < NF; f += FINE_THREADS) fcnt[f] = 0;
+  __syncthreads();
+  const uint32_t a = s0 + j * fch;
+  const uint32_t e = (s1 - a > fch) ? a + fch : s1;
+  for (uint32_t p = a + threadIdx.x; p < e; p += FINE_THREADS) atomicAdd(&fcnt[items_f[p]], 1u);
+  __syncthreads();
+  uint32_t* out = fhist + (uint64_t)fc0[g] * NF + j;
+  for (uint32_t f = threadIdx.x; f < NF; f += FINE_THREADS) out[(uint64_t)f * nc] = fcnt[f];
+}
+
+// places the chunk's entries at their bucket positions; chunk 0 of a
+// segment writes the CSR row pointers of its buckets (bucket |d| = bin + 1,
+// key = row B + |d|; bucket 0 of each row is empty)
+__global__ __launch_bounds__(FINE_THREADS) void msm_fine_scatter_kernel(
+    const uint32_t* __restrict__ items_v, const uint16_t* __restrict__ items_f,
+    const uint32_t* __restrict__ hoff, const MsmRow* __restrict__ rows, uint32_t S, uint32_t NC,
+    uint32_t nseg, uint32_t fch, const uint32_t* __restrict__ fc0, const uint32_t* __restrict__ foff,
+    uint32_t nbins, uint32_t B, uint32_t* __restrict__ row_ptr, uint32_t* __restrict__ sorted) {
+  extern __shared__ uint32_t fpos[];
+  uint32_t g, j, nc;
+  if (!fine_chunk(fc0, nseg, blockIdx.x, g, j, nc)) return;
+  uint32_t r, k, s0, s1;
+  fine_segment(hoff, rows, NC, g, r, k, s0, s1);
+  const uint32_t NF = 1u << S;
+  const uint32_t* in = foff + (uint64_t)fc0[g] * NF + j;
+  for (uint32_t f = threadIdx.x; f < NF; f += FINE_THREADS) fpos[f] = in[(uint64_t)f * nc];
+  __syncthreads();
+  if (j == 0) {
+    const uint64_t key0 = (uint64_t)r * B + 1 + ((uint64_t)k << S);
+    for (uint32_t f = threadIdx.x; f < NF; f += FINE_THREADS)
+      if (((k << S) + f) < nbins) row_ptr[key0 + f] = fpos[f];
+    if (k == 0 && threadIdx.x == 0) row_ptr[(uint64_t)r * B] = s0;
+    if (g == nseg - 1 && threadIdx.x == 0) row_ptr[(uint64_t)(nseg / NC) * B] = s1;  // the total
+  }
+  __syncthreads();  // the bucket starts are read before the placement below advances them
+  const uint32_t a = s0 + j * fch;
+  const uint32_t e = (s1 - a > fch) ? a + fch : s1;
+  for (uint32_t p = a + threadIdx.x; p < e; p += FINE_THREADS) {
+    const uint32_t pos = atomicAdd(&fpos[items_f[p]], 1u);
+    sorted[pos] = items_v[p];
   }
 }
 
@@ -583,9 +678,13 @@ struct MsmShape {
   uint32_t nrows = 0;
   uint32_t nkeys = 0;   // nrows * B
   uint32_t nbins = 0;   // B - 1
-  uint32_t nranges = 0;
+  uint32_t S = 0;       // fine-key bits
+  uint32_t NC = 0;      // coarse keys per row
   uint32_t max_tiles = 0;
   uint32_t tile_slots = 0;
+  uint64_t hist_len = 0;  // tile_slots * NC
+  uint32_t fch = 0;       // entries per fine-level chunk
+  uint32_t fchunks = 0;   // upper bound of the fine-level chunks
   uint64_t pairs = 0;   // digits over all rows
   uint64_t Smax = 0;    // accumulation segments
   uint32_t L = 0, K = 0, cpw = 0;
@@ -593,7 +692,8 @@ struct MsmShape {
 };
 
 struct MsmBuffers {
-  uint32_t *digits, *hist, *counts, *row_ptr, *sums, *sorted, *hkey;
+  uint32_t *digits, *hist, *items_v, *row_ptr, *sums, *sorted, *hkey, *fc0, *fhist;
+  uint16_t* items_f;
   xyzz *buckets, *head, *partial, *rsum, *rexp;
   MsmRow* rows;
 };
@@ -606,7 +706,14 @@ static int make_shape(Ctx* c, int cb, uint32_t nrows, const std::vector<MsmRow>&
   sh.nrows = nrows;
   sh.nkeys = nrows * sh.B;
   sh.nbins = sh.B - 1;
-  sh.nranges = (sh.nbins + SORT_RANGE - 1) / SORT_RANGE;
+  {
+    uint32_t lb = 0;
+    while ((1u << lb) < sh.nbins) lb++;
+    sh.S = lb > 8 ? lb - 8 : 0;
+    if (sh.S > SORT_MAX_FINE_BITS) sh.S = SORT_MAX_FINE_BITS;
+    sh.NC = (sh.nbins + (1u << sh.S) - 1) >> sh.S;
+    if (sh.NC > SORT_MAX_COARSE) return set_err(c, ZK_ERR_INVALID, "window too large for the bucket sort");
+  }
   sh.max_tiles = 0;
   sh.tile_slots = 0;
   sh.pairs = 0;
@@ -616,6 +723,14 @@ static int make_shape(Ctx* c, int cb, uint32_t nrows, const std::vector<MsmRow>&
     sh.pairs += r.len;
   }
   if (sh.pairs >= (1ull << 32)) return set_err(c, ZK_ERR_INVALID, "total digit count must be below 2^32");
+  if (nrows > 65535) return set_err(c, ZK_ERR_INVALID, "too many MSM rows in one launch");
+  sh.hist_len = (uint64_t)sh.tile_slots * sh.NC;
+  sh.fch = FINE_CHUNK > 16u * (1u << sh.S) ? FINE_CHUNK : 16u * (1u << sh.S);
+  {
+    const uint64_t fc = sh.pairs / sh.fch + (uint64_t)nrows * sh.NC;
+    if (fc * (1ull << sh.S) >= (1ull << 32)) return set_err(c, ZK_ERR_INVALID, "MSM too large for the bucket sort");
+    sh.fchunks = (uint32_t)fc;
+  }
   sh.L = c->msm_segment;
   sh.Smax = (sh.pairs + sh.L - 1) / sh.L;
   if (sh.Smax == 0) sh.Smax = 1;
@@ -627,8 +742,11 @@ static int make_shape(Ctx* c, int cb, uint32_t nrows, const std::vector<MsmRow>&
   if (K > sh.nbins) K = sh.nbins;
   sh.K = K;
   sh.cpw = (sh.nbins + K - 1) / K;
-  sh.nchunks = (sh.nkeys + SCAN_CHUNK - 1) / SCAN_CHUNK;
-  if (sh.nchunks > SCAN_CHUNK) return set_err(c, ZK_ERR_INVALID, "too many buckets for the scan");
+  {
+    uint64_t longest = sh.hist_len;  // scan partial sums, all levels, of the longest scan
+    if ((uint64_t)sh.fchunks << sh.S > longest) longest = (uint64_t)sh.fchunks << sh.S;
+    sh.nchunks = (uint32_t)(longest / (SCAN_CHUNK - 1) + 16);
+  }
   return ZK_OK;
 }
 
@@ -641,11 +759,14 @@ static size_t layout(const MsmShape& sh, char* base, MsmBuffers* b) {
   };
   MsmBuffers t;
   t.digits = (uint32_t*)take(sh.pairs * 4);
-  t.hist = (uint32_t*)take((size_t)sh.tile_slots * sh.nbins * 4);
-  t.counts = (uint32_t*)take((size_t)(sh.nkeys + 1) * 4);
+  t.sorted = t.digits;  // the digits are dead once the scatter pass has run
+  t.hist = (uint32_t*)take((size_t)(sh.hist_len + 1) * 4);
+  t.items_v = (uint32_t*)take(sh.pairs * 4);
+  t.items_f = (uint16_t*)take(sh.pairs * 2);
   t.row_ptr = (uint32_t*)take((size_t)(sh.nkeys + 1) * 4);
   t.sums = (uint32_t*)take((size_t)sh.nchunks * 4);
-  t.sorted = (uint32_t*)take(sh.pairs * 4);
+  t.fc0 = (uint32_t*)take(((size_t)sh.nrows * sh.NC + 1) * 4);
+  t.fhist = (uint32_t*)take(((size_t)sh.fchunks << sh.S) * 4 + 4);
   t.buckets = (xyzz*)take((size_t)sh.nkeys * sizeof(xyzz));
   t.hkey = (uint32_t*)take(sh.Smax * 4);
   t.head = (xyzz*)take(sh.Smax * sizeof(xyzz));
@@ -657,21 +778,43 @@ static size_t layout(const MsmShape& sh, char* base, MsmBuffers* b) {
   return off;
 }
 
+// exclusive scan of in[0, len) into out[0, len) (in place allowed) with the
+// total in out[len]; tmp holds the partial sums of every level
+// (len / (SCAN_CHUNK - 1) + 16 words suffice)
+static void scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t len, uint32_t* tmp, hipStream_t st) {
+  if (len <= (uint64_t)SCAN_CHUNK) {
+    scan_chunks_kernel<<<1, SCAN_THREADS, 0, st>>>(in, out, out + len, len);
+    return;
+  }
+  const uint64_t nch = (len + SCAN_CHUNK - 1) / SCAN_CHUNK;
+  scan_chunks_kernel<<<(unsigned)nch, SCAN_THREADS, 0, st>>>(in, out, tmp, len);
+  scan_exclusive(tmp, tmp, nch, tmp + nch + 1, st);  // tmp[nch] = total
+  scan_add_kernel<<<(unsigned)nch, SCAN_THREADS, 0, st>>>(out, tmp, len);
+}
+
 // sort -> accumulate -> fix-up -> per-row reduction -> export (R = 2^256 XYZZ
 // per row in b.rexp).  No allocation, no host synchronisation.
 static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* bases9, hipStream_t st) {
   hipEvent_t ev;
   phase_begin(c, "msm_sort", st, &ev);
+  const dim3 g(sh.max_tiles, sh.nrows);
+  if (sh.max_tiles > 0)
+    msm_sort_count_kernel<<<g, SORT_THREADS, 0, st>>>(b.digits, b.rows, sh.S, sh.NC, b.hist);
+  scan_exclusive(b.hist, b.hist, sh.hist_len, b.sums, st);  // in place; hist[hist_len] = total
+  if (sh.max_tiles > 0)
+    msm_sort_scatter_kernel<<<g, SORT_THREADS, 0, st>>>(b.digits, b.rows, sh.S, sh.NC, b.hist, b.items_v,
+                                                         b.items_f);
   {
-    dim3 g(sh.max_tiles, sh.nrows, sh.nranges);
-    msm_tile_count_kernel<<<g, SORT_THREADS, 0, st>>>(b.digits, b.rows, sh.nbins, b.hist);
-    msm_tile_prefix_kernel<<<grid_for((uint64_t)sh.nrows * sh.nbins, 256), 256, 0, st>>>(
-        b.hist, b.rows, sh.nrows, sh.nbins, sh.B, b.counts);
-    scan_chunks_kernel<<<sh.nchunks, SCAN_THREADS, 0, st>>>(b.counts, b.row_ptr, b.sums, sh.nkeys);
-    scan_sums_kernel<<<1, SCAN_THREADS, 0, st>>>(b.sums, sh.nchunks, b.row_ptr + sh.nkeys);
-    scan_add_kernel<<<sh.nchunks, SCAN_THREADS, 0, st>>>(b.row_ptr, b.sums, sh.nkeys);
-    msm_tile_scatter_kernel<<<g, SORT_THREADS, 0, st>>>(b.digits, b.rows, sh.nbins, sh.B, b.hist, b.row_ptr,
-                                                         b.sorted);
+    const uint32_t nseg = sh.nrows * sh.NC, NF = 1u << sh.S;
+    msm_fine_chunks_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(b.hist, b.rows, sh.NC, nseg, sh.fch, b.fc0);
+    scan_exclusive(b.fc0, b.fc0, nseg, b.sums, st);  // fc0[nseg] = chunks in use (<= sh.fchunks)
+    msm_fine_count_kernel<<<sh.fchunks, FINE_THREADS, NF * 4, st>>>(b.items_f, b.hist, b.rows, sh.S, sh.NC, nseg,
+                                                                    sh.fch, b.fc0, b.fhist);
+    // entries past the chunks in use are never read; scanning them is harmless
+    scan_exclusive(b.fhist, b.fhist, (uint64_t)sh.fchunks * NF, b.sums, st);
+    msm_fine_scatter_kernel<<<sh.fchunks, FINE_THREADS, NF * 4, st>>>(b.items_v, b.items_f, b.hist, b.rows, sh.S,
+                                                                      sh.NC, nseg, sh.fch, b.fc0, b.fhist,
+                                                                      sh.nbins, sh.B, b.row_ptr, b.sorted);
   }
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_sort", st, ev);

@@ -137,6 +137,33 @@ def test_msm_same_scalar_everywhere(zctx):
     zctx.set_tuning("msm_segment", 64)
 
 
+def test_msm_wide_windows(zctx):
+    # c >= 20: up to 2^13 fine keys per coarse key in the bucket sort
+    n = 100
+    rng = random.Random(20)
+    ks = [rng.randrange(1, R_MOD) for _ in range(n)]
+    ss = [rng.randrange(R_MOD) for _ in range(n)]
+    bases = upload_bases(zctx, _bases(ks))
+    sc = upload_fr_mont(zctx, ss)
+    for c in (20, 22, 24):
+        assert msm_affine(zctx, bases, sc, n, c) == M.msm_closed_form(ks, ss), c
+
+
+def test_msm_one_huge_bucket(zctx):
+    # every point in one bucket of every window: the fine level of the
+    # bucket sort spreads that segment over several workgroups
+    from pyzkp_amd import workload
+    from pyzkp_amd.field import words_to_ints
+    n = 1 << 17
+    kw = workload.fr_uniform(0x51, n)
+    bases = workload.device_bases(zctx, kw)
+    s = random.Random(5).randrange(R_MOD)
+    sc = upload_fr_mont(zctx, [s] * n)
+    expect = M.msm_closed_form(words_to_ints(kw), [s] * n)
+    for c in (14, 16, 18):
+        assert msm_affine(zctx, bases, sc, n, c) == expect, c
+
+
 def test_msm_golden_commitments(zctx):
     import json, os
     with open(os.path.join(os.path.dirname(__file__), "golden", "plonk_pk_n16.json")) as f:
