@@ -420,6 +420,83 @@ __global__ __launch_bounds__(FINE_THREADS) void msm_fine_scatter_kernel(
   }
 }
 
+// Staged form of the placement for up to FINE_STAGE_KEYS fine keys: the
+// chunk (at most FINE_STAGE entries) is ranked in LDS, staged in fine-key
+// order and written out by consecutive threads, so the global writes are
+// contiguous runs per fine key instead of one scattered 4-byte store per
+// entry.
+static constexpr uint32_t FINE_STAGE = 8192;
+static constexpr uint32_t FINE_STAGE_THREADS = 512;
+static constexpr uint32_t FINE_STAGE_ITEMS = FINE_STAGE / FINE_STAGE_THREADS;
+static constexpr uint32_t FINE_STAGE_KEYS = 2 * FINE_STAGE_THREADS;
+
+__global__ __launch_bounds__(FINE_STAGE_THREADS) void msm_fine_scatter_staged_kernel(
+    const uint32_t* __restrict__ items_v, const uint16_t* __restrict__ items_f,
+    const uint32_t* __restrict__ hoff, const MsmRow* __restrict__ rows, uint32_t S, uint32_t NC,
+    uint32_t nseg, uint32_t fch, const uint32_t* __restrict__ fc0, const uint32_t* __restrict__ foff,
+    uint32_t nbins, uint32_t B, uint32_t* __restrict__ row_ptr, uint32_t* __restrict__ sorted) {
+  __shared__ uint32_t gpos[FINE_STAGE_KEYS];
+  __shared__ uint32_t lcnt[FINE_STAGE_KEYS];
+  __shared__ uint32_t stage_v[FINE_STAGE];
+  __shared__ uint16_t stage_f[FINE_STAGE];
+  __shared__ uint32_t scan_sh[FINE_STAGE_THREADS];
+  uint32_t g, j, nc;
+  if (!fine_chunk(fc0, nseg, blockIdx.x, g, j, nc)) return;
+  uint32_t r, k, s0, s1;
+  fine_segment(hoff, rows, NC, g, r, k, s0, s1);
+  const uint32_t NF = 1u << S;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t* in = foff + (uint64_t)fc0[g] * NF + j;
+  for (uint32_t f = tid; f < NF; f += FINE_STAGE_THREADS) {
+    gpos[f] = in[(uint64_t)f * nc];
+    lcnt[f] = 0;
+  }
+  __syncthreads();
+  if (j == 0) {
+    const uint64_t key0 = (uint64_t)r * B + 1 + ((uint64_t)k << S);
+    for (uint32_t f = tid; f < NF; f += FINE_STAGE_THREADS)
+      if (((k << S) + f) < nbins) row_ptr[key0 + f] = gpos[f];
+    if (k == 0 && tid == 0) row_ptr[(uint64_t)r * B] = s0;
+    if (g == nseg - 1 && tid == 0) row_ptr[(uint64_t)(nseg / NC) * B] = s1;  // the total
+  }
+  const uint32_t a = s0 + j * fch;
+  const uint32_t cnt = (s1 - a > fch) ? fch : s1 - a;
+  uint32_t v[FINE_STAGE_ITEMS], f[FINE_STAGE_ITEMS], rank[FINE_STAGE_ITEMS];
+#pragma unroll
+  for (uint32_t i = 0; i < FINE_STAGE_ITEMS; i++) {
+    const uint32_t q = i * FINE_STAGE_THREADS + tid;
+    if (q < cnt) {
+      v[i] = items_v[a + q];
+      f[i] = items_f[a + q];
+      rank[i] = atomicAdd(&lcnt[f[i]], 1u);
+    }
+  }
+  __syncthreads();
+  // exclusive scan of the local counts, two keys per thread
+  uint32_t total;
+  const uint32_t k0 = 2 * tid, k1 = 2 * tid + 1;
+  const uint32_t c0 = (k0 < NF) ? lcnt[k0] : 0u, c1 = (k1 < NF) ? lcnt[k1] : 0u;
+  const uint32_t ex = block_exclusive_scan(c0 + c1, scan_sh, total);
+  __syncthreads();
+  if (k0 < NF) lcnt[k0] = ex;
+  if (k1 < NF) lcnt[k1] = ex + c0;
+  __syncthreads();
+#pragma unroll
+  for (uint32_t i = 0; i < FINE_STAGE_ITEMS; i++) {
+    const uint32_t q = i * FINE_STAGE_THREADS + tid;
+    if (q < cnt) {
+      const uint32_t slot = lcnt[f[i]] + rank[i];
+      stage_v[slot] = v[i];
+      stage_f[slot] = (uint16_t)f[i];
+    }
+  }
+  __syncthreads();
+  for (uint32_t slot = tid; slot < cnt; slot += FINE_STAGE_THREADS) {
+    const uint32_t ff = stage_f[slot];
+    sorted[gpos[ff] + (slot - lcnt[ff])] = stage_v[slot];
+  }
+}
+
 // 4. bucket accumulation over fixed-length segments of the sorted list.
 //    A run (bucket) that starts inside the segment is owned by it and its
 //    partial sum goes straight to buckets[key]; the run the segment starts
@@ -555,21 +632,23 @@ __global__ __launch_bounds__(128) void msm_reduce_chunks_kernel(const xyzz* __re
   store_xyzz9(partial + gid, sum);
 }
 
-// 7. per-window tree sum of the chunk partials
-__global__ __launch_bounds__(128) void msm_reduce_windows_kernel(const xyzz* __restrict__ partial,
-                                                                 uint32_t cpw,
-                                                                 xyzz* __restrict__ window_sums) {
-  __shared__ xyzz9 sh[128];
-  const uint32_t w = blockIdx.x, tid = threadIdx.x;
-  xyzz9 acc = xyzz9_inf();
-  for (uint32_t t = tid; t < cpw; t += blockDim.x) acc = xyzz9_add(acc, load_xyzz9(partial + (uint64_t)w * cpw + t));
-  sh[tid] = acc;
+// 7. per-window tree sum of the chunk partials, in levels: group g of window
+//    w sums in[w m + 64 g .. + 64) into out[w G + g], G = ceil(m / 64);
+//    the host repeats until one value per window is left
+static constexpr uint32_t RED_GROUP = 64;
+
+__global__ __launch_bounds__(RED_GROUP) void msm_reduce_groups_kernel(const xyzz* __restrict__ in, uint32_t m,
+                                                                     uint32_t G, xyzz* __restrict__ out) {
+  __shared__ xyzz9 sh[RED_GROUP];
+  const uint32_t w = blockIdx.y, g = blockIdx.x, tid = threadIdx.x;
+  const uint32_t t = g * RED_GROUP + tid;
+  sh[tid] = (t < m) ? load_xyzz9(in + (uint64_t)w * m + t) : xyzz9_inf();
   __syncthreads();
-  for (uint32_t step = blockDim.x >> 1; step > 0; step >>= 1) {
+  for (uint32_t step = RED_GROUP >> 1; step > 0; step >>= 1) {
     if (tid < step) sh[tid] = xyzz9_add(sh[tid], sh[tid + step]);
     __syncthreads();
   }
-  if (tid == 0) store_xyzz9(window_sums + w, sh[0]);
+  if (tid == 0) store_xyzz9(out + (uint64_t)w * G + g, sh[0]);
 }
 
 // 8. fixed-base batches: the tables and the digit split.
@@ -694,7 +773,7 @@ struct MsmShape {
 struct MsmBuffers {
   uint32_t *digits, *hist, *items_v, *row_ptr, *sums, *sorted, *hkey, *fc0, *fhist;
   uint16_t* items_f;
-  xyzz *buckets, *head, *partial, *rsum, *rexp;
+  xyzz *buckets, *head, *partial, *partial2, *rsum, *rexp;
   MsmRow* rows;
 };
 
@@ -725,7 +804,8 @@ static int make_shape(Ctx* c, int cb, uint32_t nrows, const std::vector<MsmRow>&
   if (sh.pairs >= (1ull << 32)) return set_err(c, ZK_ERR_INVALID, "total digit count must be below 2^32");
   if (nrows > 65535) return set_err(c, ZK_ERR_INVALID, "too many MSM rows in one launch");
   sh.hist_len = (uint64_t)sh.tile_slots * sh.NC;
-  sh.fch = FINE_CHUNK > 16u * (1u << sh.S) ? FINE_CHUNK : 16u * (1u << sh.S);
+  if ((1u << sh.S) <= FINE_STAGE_KEYS) sh.fch = FINE_STAGE;  // staged placement
+  else sh.fch = FINE_CHUNK > 16u * (1u << sh.S) ? FINE_CHUNK : 16u * (1u << sh.S);
   {
     const uint64_t fc = sh.pairs / sh.fch + (uint64_t)nrows * sh.NC;
     if (fc * (1ull << sh.S) >= (1ull << 32)) return set_err(c, ZK_ERR_INVALID, "MSM too large for the bucket sort");
@@ -771,6 +851,7 @@ static size_t layout(const MsmShape& sh, char* base, MsmBuffers* b) {
   t.hkey = (uint32_t*)take(sh.Smax * 4);
   t.head = (xyzz*)take(sh.Smax * sizeof(xyzz));
   t.partial = (xyzz*)take((size_t)sh.nrows * sh.cpw * sizeof(xyzz));
+  t.partial2 = (xyzz*)take((size_t)sh.nrows * ((sh.cpw + RED_GROUP - 1) / RED_GROUP) * sizeof(xyzz));
   t.rsum = (xyzz*)take((size_t)sh.nrows * sizeof(xyzz));
   t.rexp = (xyzz*)take((size_t)sh.nrows * sizeof(xyzz));
   t.rows = (MsmRow*)take((size_t)sh.nrows * sizeof(MsmRow));
@@ -812,9 +893,14 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
                                                                     sh.fch, b.fc0, b.fhist);
     // entries past the chunks in use are never read; scanning them is harmless
     scan_exclusive(b.fhist, b.fhist, (uint64_t)sh.fchunks * NF, b.sums, st);
-    msm_fine_scatter_kernel<<<sh.fchunks, FINE_THREADS, NF * 4, st>>>(b.items_v, b.items_f, b.hist, b.rows, sh.S,
-                                                                      sh.NC, nseg, sh.fch, b.fc0, b.fhist,
-                                                                      sh.nbins, sh.B, b.row_ptr, b.sorted);
+    if (NF <= FINE_STAGE_KEYS)
+      msm_fine_scatter_staged_kernel<<<sh.fchunks, FINE_STAGE_THREADS, 0, st>>>(
+          b.items_v, b.items_f, b.hist, b.rows, sh.S, sh.NC, nseg, sh.fch, b.fc0, b.fhist, sh.nbins, sh.B,
+          b.row_ptr, b.sorted);
+    else
+      msm_fine_scatter_kernel<<<sh.fchunks, FINE_THREADS, NF * 4, st>>>(b.items_v, b.items_f, b.hist, b.rows, sh.S,
+                                                                        sh.NC, nseg, sh.fch, b.fc0, b.fhist,
+                                                                        sh.nbins, sh.B, b.row_ptr, b.sorted);
   }
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_sort", st, ev);
@@ -837,7 +923,20 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
   phase_begin(c, "msm_reduce", st, &ev);
   msm_reduce_chunks_kernel<<<grid_for((uint64_t)sh.nrows * sh.cpw, 128), 128, 0, st>>>(
       b.buckets, b.row_ptr, sh.nrows, sh.B, sh.K, sh.cpw, b.partial);
-  msm_reduce_windows_kernel<<<sh.nrows, 128, 0, st>>>(b.partial, sh.cpw, b.rsum);
+  {
+    // chunk partials -> one sum per window, 64 to 1 per level
+    xyzz* src = b.partial;
+    xyzz* spare[2] = {b.partial2, b.partial};
+    uint32_t m = sh.cpw;
+    for (int lvl = 0;; lvl++) {
+      const uint32_t G = (m + RED_GROUP - 1) / RED_GROUP;
+      xyzz* dst = (G == 1) ? b.rsum : spare[lvl & 1];
+      msm_reduce_groups_kernel<<<dim3(G, sh.nrows), RED_GROUP, 0, st>>>(src, m, G, dst);
+      if (G == 1) break;
+      src = dst;
+      m = G;
+    }
+  }
   msm_wsum_export_kernel<<<grid_for(sh.nrows, 64), 64, 0, st>>>(b.rsum, sh.nrows, b.rexp);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_reduce", st, ev);
