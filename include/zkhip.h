@@ -212,8 +212,9 @@ int zk_fr_count_nonzero(zk_ctx* ctx, const void* a, uint64_t n, uint64_t* result
  * on the twist y^2 = x^3 + 3/(9+u), CANONICAL little-endian words
  * (x0, x1, y0, y1; 16 x u64, all zero = infinity); scalars canonical.
  * zk_g2_msm_host replaces msm_pippenger_g2 (crypto/msm/pippenger.py:67-121,
- * the CPU MSM_G2 op runtime/kernels/cpu/kernels.py:338-348); window_bits 0
- * follows the reference rule, threads 0 = all hardware threads.
+ * the CPU MSM_G2 op runtime/kernels/cpu/kernels.py:338-348) with signed
+ * c-bit windows (window_bits in [2, 20]; 0 = sized from n and threads) and
+ * batch-affine bucket sums; threads 0 = all hardware threads.
  * zk_g2_mul_fixed_host computes out[i] = [k_i] Q (setup-time g2_mul loops,
  * protocols/groth16/setup.py:63-83); zk_g2_add_host adds two points. */
 int zk_g2_msm_host(const uint64_t* bases, const uint64_t* scalars, uint64_t n, int window_bits, int threads,

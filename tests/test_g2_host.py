@@ -70,3 +70,25 @@ def test_g2_msm_edge_cases():
     ss = [1, 2, R_MOD - 1, 9, 0]
     assert _aff(G.g2_msm(pts, ss, window_bits=4)) == O.mul(O.G2_GENERATOR, 77 * 2 % R_MOD)
     assert _aff(G.g2_msm(pts, [0] * 5)) is None
+
+
+@pytest.mark.parametrize("c", [3, 5, 8])
+def test_g2_msm_batches_and_collisions(c):
+    # more points than one batch of affine additions, few buckets: many
+    # additions wait for a later batch; repeated and negated bases put
+    # P + P and P + (-P) into one bucket
+    from pyzkp_amd import g2 as G
+    rng = random.Random(c)
+    n = 2500
+    base_ks = [rng.randrange(1, R_MOD) for _ in range(40)]
+    ks = [base_ks[rng.randrange(40)] * (1 if rng.random() < 0.7 else -1) % R_MOD for _ in range(n)]
+    ss = [rng.randrange(R_MOD) if rng.random() < 0.8 else rng.randrange(1 << 10) for _ in range(n)]
+    G0 = (O.G2_GENERATOR[0], O.G2_GENERATOR[1], (1, 0))
+    bases = G.from_words(G.g2_mul_many(G0, ks))
+    expect = O.mul(O.G2_GENERATOR, sum(k * s for k, s in zip(ks, ss)) % R_MOD)
+    for threads in (1, 3, 8):
+        assert _aff(G.g2_msm(bases, ss, window_bits=c, threads=threads)) == expect
+    # every scalar equal: all points of a window land in one bucket
+    s = rng.randrange(R_MOD)
+    expect2 = O.mul(O.G2_GENERATOR, sum(ks) * s % R_MOD)
+    assert _aff(G.g2_msm(bases, [s] * n, window_bits=c, threads=4)) == expect2
