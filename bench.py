@@ -48,6 +48,7 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16, help="serve: host threads for G2 MSMs")
     ap.add_argument("--hbm-budget-gb", type=float, default=16.0, help="serve: device budget for working sets")
     ap.add_argument("--load", type=float, default=0.8, help="serve: arrival rate / plain-loop capacity")
+    ap.add_argument("--g2-workers", type=int, default=4, help="serve: concurrent host G2 MSMs")
     ap.add_argument("--log-n", type=int, default=None, help="MSM: 22, NTT: 24")
     ap.add_argument("--window-bits", type=int, default=0, help="0: autotune in 13..18")
     ap.add_argument("--skewed", action="store_true", help="skewed scalars + repeated bases")
@@ -251,7 +252,8 @@ def run_serve(args, ctx):
     budget = int(args.hbm_budget_gb * 2**30)
 
     def server_run(reqs):
-        srv = ProveServer(keys, hbm_budget=budget, max_batch=16, g2_workers=4, cpu_threads=args.cpu_threads)
+        srv = ProveServer(keys, hbm_budget=budget, max_batch=16, g2_workers=args.g2_workers,
+                          cpu_threads=args.cpu_threads)
         try:
             return srv.run(reqs), dict(srv.stats)
         finally:
@@ -304,7 +306,8 @@ def run_serve(args, ctx):
                    "verified": n_ok, "identical_to_plain": n_same,
                    "key_setup_s": round(t_setup, 2), "keys_hbm_gib": round(key_bytes / 2**30, 3),
                    "working_set_max_mib": round(max(working_set_bytes(pk) for pk in keys.values()) / 2**20, 2),
-                   "cpu_threads": args.cpu_threads, "hbm_budget_gb": args.hbm_budget_gb},
+                   "cpu_threads": args.cpu_threads, "g2_workers": args.g2_workers,
+                   "hbm_budget_gb": args.hbm_budget_gb},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(line))

@@ -227,7 +227,7 @@ static A2 a_dbl(const A2& p) {
   return r;
 }
 
-static J2 bucket_task(const A2* pts, const int32_t* digits, uint32_t W, const BucketTask& t, int c) {
+static J2 bucket_task(const A2* pts, const int32_t* digits, uint64_t n_all, const BucketTask& t, int c) {
   const uint32_t nb = 1u << (c - 1);
   std::vector<A2> bk(nb);
   for (auto& b : bk) b.inf = true;
@@ -260,7 +260,7 @@ static J2 bucket_task(const A2* pts, const int32_t* digits, uint32_t W, const Bu
   };
 
   for (uint64_t i = t.i0; i < t.i1; i++) {
-    const int32_t d = digits[i * W + t.w];
+    const int32_t d = digits[(uint64_t)t.w * n_all + i];
     if (d == 0 || pts[i].inf) continue;
     A2 P = pts[i];
     if (d < 0) P.y = f2_neg(P.y);
@@ -323,7 +323,7 @@ static void msm(const uint64_t* bases, const uint64_t* scalars, uint64_t n, int 
         } else {
           carry = 0;
         }
-        digits[i * W + w] = (int32_t)dv;
+        digits[(uint64_t)w * n + i] = (int32_t)dv;
       }
     }
   });
@@ -337,7 +337,7 @@ static void msm(const uint64_t* bases, const uint64_t* scalars, uint64_t n, int 
     for (uint64_t k = 0; k < S; k++) tasks.push_back({w, n * k / S, n * (k + 1) / S});
   std::vector<J2> part(tasks.size());
   parallel_for(tasks.size(), threads,
-               [&](uint64_t t) { part[t] = bucket_task(pts.data(), digits.data(), W, tasks[t], c); });
+               [&](uint64_t t) { part[t] = bucket_task(pts.data(), digits.data(), n, tasks[t], c); });
   J2 acc = j_inf();
   for (int w = (int)W - 1; w >= 0; w--) {
     for (int k = 0; k < c; k++) acc = j_dbl(acc);
