@@ -174,6 +174,31 @@ struct Mont29 {
     return reduce2p(s);
   }
 
+  // a + b and a - b + 2p without the final reduction (normalised limbs):
+  // for a, b < 2p the results are below 4p, which mul() accepts
+  static ZK_HD f9 add_nr(const f9& a, const f9& b) {
+    f9 s;
+    u32 c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const u32 t = a.l[i] + b.l[i] + c;
+      s.l[i] = t & M29;
+      c = t >> 29;
+    }
+    return s;
+  }
+  static ZK_HD f9 sub_nr(const f9& a, const f9& b) {
+    f9 s;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const int32_t t = (int32_t)a.l[i] - (int32_t)b.l[i] + (int32_t)F::P2[i] + c;
+      s.l[i] = (u32)t & M29;
+      c = t >> 29;
+    }
+    return s;
+  }
+
   static ZK_HD f9 dbl(const f9& a) { return add(a, a); }
   static ZK_HD f9 neg(const f9& a) { return sub(zero(), a); }
 

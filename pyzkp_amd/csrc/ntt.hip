@@ -26,13 +26,16 @@
 // reading them as R = 2^261 values scales every element by the same 2^-5,
 // which the output carries back out.  Twiddles and scaling tables are kept
 // in R = 2^261 form (packed 32 bytes per entry), intermediate passes store
-// lazily reduced values (< 2p), and the last pass stores canonical values.
+// lazily reduced values (< 4p: a butterfly reduces only its first input,
+// then forms u + w v and u - w v + 2p without a final subtraction), and the
+// last pass stores canonical values.
 #include "internal.hpp"
 #include "field29.hpp"
 
 namespace zk {
 
-static constexpr int NTT_TILE = 2048;   // elements per LDS tile (64 KiB)
+static constexpr int NTT_TILE = 1024;   // elements per LDS tile (36 KiB as limbs: four workgroups per CU)
+static constexpr int NTT_TILE_BITS = 10;
 static constexpr int NTT_THREADS = 256;
 static constexpr int NTT_BFLY_PER_THREAD = NTT_TILE / 2 / NTT_THREADS;
 static constexpr int POW_LO_BITS = 10;  // two-level power tables: lo has 2^10 entries
@@ -72,7 +75,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     f9 v;
     if (idx < in_len) {
       v = unpack9(load_fe(in + idx));
-      if (s0 == 0) v = Fr9::reduce2p(Fr9::reduce2p(v));  // caller data: any value < 2^256
+      if (s0 == 0) v = Fr9::reduce2p(v);  // caller data: any value < 2^256 (< 5.3 p) -> < 4p
       if (pre_mode == SCALE_TABLE) v = Fr9::mul(v, pow_lookup(pre_hi, pre_lo, idx));
     } else {
       v = Fr9::zero();
@@ -102,15 +105,18 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
       if (b < halfRT) {
         const uint32_t c = b & (T - 1), q = b >> log_t;
         const uint32_t d = q & ((1u << t) - 1);
-        f9 vv = v[i];
+        f9 vv;
         if (s != 0) {
           const uint64_t low = (j0 + c) & (Ns - 1);
           const uint64_t e = (low + (uint64_t)d * Ns) << (log_n - 1 - s);
-          vv = Fr9::mul(vv, tab9(tw, e));
+          vv = Fr9::mul(v[i], tab9(tw, e));  // v < 4p is a valid input: the product is < 2p
+        } else {
+          vv = Fr9::reduce2p(v[i]);
         }
+        const f9 uu = Fr9::reduce2p(u[i]);
         const uint32_t o = ((q - d) << 1) + d;
-        lds[(o << log_t) + c] = Fr9::add(u[i], vv);
-        lds[((o + (1u << t)) << log_t) + c] = Fr9::sub(u[i], vv);
+        lds[(o << log_t) + c] = Fr9::add_nr(uu, vv);
+        lds[((o + (1u << t)) << log_t) + c] = Fr9::sub_nr(uu, vv);
       }
     }
     __syncthreads();
@@ -135,7 +141,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     } else if (post_mode == SCALE_TABLE) {
       x = Fr9::mul(x, pow_lookup(post_hi, post_lo, pos));
     }
-    if (last) x = Fr9::canonical(x);
+    if (last) x = Fr9::canonical(Fr9::reduce2p(x));
     store_fe(out + pos, pack9(x));
   }
 }
@@ -243,7 +249,7 @@ static zkh::Fe standard_root(uint32_t log_n) {
 static void plan_passes(uint32_t log_n, int max_pass_bits, std::vector<uint32_t>& ks) {
   ks.clear();
   if (log_n == 0) return;
-  const uint32_t tile_bits = 11;  // log2(NTT_TILE)
+  const uint32_t tile_bits = NTT_TILE_BITS;
   uint32_t kmax = (max_pass_bits > 0) ? (uint32_t)max_pass_bits : 9;
   if (kmax > tile_bits) kmax = tile_bits;
   if (log_n <= tile_bits && (max_pass_bits <= 0 || log_n <= kmax)) {
@@ -339,7 +345,7 @@ int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint
   for (size_t i = 0; i < P; i++) {
     const uint32_t k = ks[i];
     const uint64_t cols = n >> k;
-    uint32_t log_t = 11 - k;
+    uint32_t log_t = NTT_TILE_BITS - k;
     while ((1ull << log_t) > cols) log_t--;
     const uint64_t tiles = cols >> log_t;
     const fe* s = (i == 0) ? in : dst[i - 1];
