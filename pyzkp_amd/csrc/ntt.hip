@@ -42,6 +42,13 @@ static constexpr int POW_LO_BITS = 10;  // two-level power tables: lo has 2^10 e
 
 // tables: R = 2^261 values packed in 32 bytes
 __device__ __forceinline__ f9 tab9(const fe* __restrict__ t, uint64_t i) { return unpack9(load_fe(t + i)); }
+// twiddles: 9 limbs per entry
+__device__ __forceinline__ f9 tw9(const uint32_t* __restrict__ t, uint64_t i) {
+  f9 r;
+#pragma unroll
+  for (int k = 0; k < 9; k++) r.l[k] = t[9 * i + k];
+  return r;
+}
 
 __device__ __forceinline__ f9 pow_lookup(const fe* __restrict__ hi, const fe* __restrict__ lo, uint64_t idx) {
   return Fr9::mul(tab9(hi, idx >> POW_LO_BITS), tab9(lo, idx & ((1u << POW_LO_BITS) - 1)));
@@ -51,7 +58,7 @@ __device__ __forceinline__ f9 pow_lookup(const fe* __restrict__ hi, const fe* __
 enum { SCALE_NONE = 0, SCALE_CONST = 1, SCALE_TABLE = 2 };
 
 __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
-    const fe* __restrict__ in, fe* __restrict__ out, const fe* __restrict__ tw, uint32_t log_n,
+    const fe* __restrict__ in, fe* __restrict__ out, const uint32_t* __restrict__ tw, uint32_t log_n,
     uint32_t s0, uint32_t k, uint32_t log_t, uint64_t in_len, int pre_mode,
     const fe* __restrict__ pre_hi, const fe* __restrict__ pre_lo, int post_mode,
     const fe* __restrict__ post_hi, const fe* __restrict__ post_lo, int last) {
@@ -109,7 +116,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         if (s != 0) {
           const uint64_t low = (j0 + c) & (Ns - 1);
           const uint64_t e = (low + (uint64_t)d * Ns) << (log_n - 1 - s);
-          vv = Fr9::mul(v[i], tab9(tw, e));  // v < 4p is a valid input: the product is < 2p
+          vv = Fr9::mul(v[i], tw9(tw, e));  // v < 4p is a valid input: the product is < 2p
         } else {
           vv = Fr9::reduce2p(v[i]);
         }
@@ -146,12 +153,16 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
   }
 }
 
-// twiddle table tw[e] = hi[e >> 10] * lo[e & 1023]
+// twiddle table tw[e] = hi[e >> 10] * lo[e & 1023], stored as 9 limbs per
+// entry so the butterflies load it without unpacking
 __global__ __launch_bounds__(256) void pow_expand_kernel(const fe* __restrict__ hi,
                                                          const fe* __restrict__ lo,
-                                                         fe* __restrict__ out, uint64_t count) {
+                                                         uint32_t* __restrict__ out, uint64_t count) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) store_fe(out + i, pack9(Fr9::canonical(pow_lookup(hi, lo, i))));
+  if (i >= count) return;
+  const f9 t = Fr9::canonical(pow_lookup(hi, lo, i));
+#pragma unroll
+  for (int k = 0; k < 9; k++) out[9 * i + k] = t.l[k];
 }
 
 // host-built R = 2^256 tables -> R = 2^261 packed form, in place
@@ -215,7 +226,7 @@ static int get_pow_tables(Ctx* c, uint32_t log_n, const zkh::Fe& base, const zkh
   return ZK_OK;
 }
 
-static int get_twiddles(Ctx* c, uint32_t log_n, const zkh::Fe& w, const fe** out, hipStream_t st) {
+static int get_twiddles(Ctx* c, uint32_t log_n, const zkh::Fe& w, const uint32_t** out, hipStream_t st) {
   std::array<uint64_t, 5> key = {log_n, w.v[0], w.v[1], w.v[2], w.v[3]};
   auto it = c->twiddles.find(key);
   if (it == c->twiddles.end()) {
@@ -224,13 +235,13 @@ static int get_twiddles(Ctx* c, uint32_t log_n, const zkh::Fe& w, const fe** out
     int rc = get_pow_tables(c, log_n, w, zkh::one(zkh::FR), &hi, &lo, st);
     if (rc) return rc;
     DevBuf b;
-    b.bytes = count * 32;
+    b.bytes = count * 36;
     ZK_HIP(c, hipMalloc(&b.ptr, b.bytes));
-    pow_expand_kernel<<<grid_for(count, 256), 256, 0, st>>>(hi, lo, (fe*)b.ptr, count);
+    pow_expand_kernel<<<grid_for(count, 256), 256, 0, st>>>(hi, lo, (uint32_t*)b.ptr, count);
     ZK_HIP(c, hipGetLastError());
     it = c->twiddles.emplace(key, b).first;
   }
-  *out = (const fe*)it->second.ptr;
+  *out = (const uint32_t*)it->second.ptr;
   return ZK_OK;
 }
 
@@ -285,7 +296,7 @@ int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint
   }
   if (inverse) w = zkh::inv(zkh::FR, w);
 
-  const fe* tw;
+  const uint32_t* tw;
   int rc = get_twiddles(c, log_n, w, &tw, st);
   if (rc) return rc;
 
@@ -373,7 +384,7 @@ int ntt_prepare(Ctx* c, uint32_t log_n, bool inverse, const zkh::Fe* omega, cons
   const uint64_t n = 1ull << log_n;
   zkh::Fe w = omega ? *omega : standard_root(log_n);
   if (inverse) w = zkh::inv(zkh::FR, w);
-  const fe* tw;
+  const uint32_t* tw;
   int rc = get_twiddles(c, log_n, w, &tw, st);
   if (rc) return rc;
   const fe *h, *l;
