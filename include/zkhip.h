@@ -208,6 +208,20 @@ int zk_fr_gather(zk_ctx* ctx, const void* src, const uint32_t* idx, uint64_t n, 
 int zk_fr_inv(zk_ctx* ctx, const void* a, void* out, uint64_t n, void* stream);
 int zk_fr_count_nonzero(zk_ctx* ctx, const void* a, uint64_t n, uint64_t* result, void* stream);
 
+/* PLONK quotient numerator and Z_H on the extended coset of size m (a
+ * multiple of n by at most 16), fused into one device pass: replaces the
+ * PLONK_T_QUOTIENT_EVALS operator (runtime/kernels/cpu/kernels.py:607-694,
+ * used by protocols/plonk/prove.py:363-424).  ext holds 15 device vectors of
+ * m Fr Montgomery elements in the order a, b, c, z, pi, ql, qr, qm, qo, qc,
+ * s1, s2, s3, z(omega X), L1 (their coset evaluations); all scalars are
+ * Montgomery words.  num_out[i] = gate + alpha perm + alpha^2 (z - 1) L1 at
+ * x_i = shift omega_m^i, zh_out[i] = shift_n (omega_m^n)^i - 1. */
+int zk_plonk_quotient_evals(zk_ctx* ctx, const void* const* ext, uint64_t n, uint64_t m,
+                            const uint64_t omega_m[4], const uint64_t shift[4], const uint64_t shift_n[4],
+                            const uint64_t alpha[4], const uint64_t beta[4], const uint64_t gamma[4],
+                            const uint64_t k1[4], const uint64_t k2[4], void* num_out, void* zh_out,
+                            void* stream);
+
 /* G2 on the host (CPU threads).  Points are affine over Fq2 = Fq[u]/(u^2+1)
  * on the twist y^2 = x^3 + 3/(9+u), CANONICAL little-endian words
  * (x0, x1, y0, y1; 16 x u64, all zero = infinity); scalars canonical.

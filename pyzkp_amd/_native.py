@@ -63,6 +63,8 @@ SIGNATURES = {
     "zk_fr_gather": [_vp, _vp, _vp, ct.c_uint64, _vp, _vp],
     "zk_fr_inv": [_vp, _vp, _vp, ct.c_uint64, _vp],
     "zk_fr_count_nonzero": [_vp, _vp, ct.c_uint64, ct.POINTER(ct.c_uint64), _vp],
+    "zk_plonk_quotient_evals": [_vp, ct.POINTER(ct.c_void_p), ct.c_uint64, ct.c_uint64] + [_vp] * 8
+                               + [_vp, _vp, _vp],
     "zk_g2_msm_host": [_vp, _vp, ct.c_uint64, ct.c_int, ct.c_int, _vp],
     "zk_g2_mul_fixed_host": [_vp, _vp, ct.c_uint64, ct.c_int, _vp],
     "zk_g2_add_host": [_vp, _vp, _vp],
@@ -310,6 +312,20 @@ class Context:
 
     def fr_inv(self, a: int, out: int, n: int, stream=None) -> None:
         self._check(self.lib.zk_fr_inv(self.handle, a, out, int(n), stream))
+
+    def plonk_quotient_evals(self, ext, n: int, m: int, omega_m_mont: int, shift_mont: int, shift_n_mont: int,
+                             alpha_mont: int, beta_mont: int, gamma_mont: int, k1_mont: int, k2_mont: int,
+                             num_out: int, zh_out: int, stream=None) -> None:
+        """Fused PLONK quotient numerator / Z_H over the size-m coset; ext = 15
+        device pointers (a, b, c, z, pi, ql, qr, qm, qo, qc, s1, s2, s3,
+        z(omega X), L1), scalars as Montgomery ints."""
+        if len(ext) != 15:
+            raise ValueError("plonk quotient needs 15 evaluation vectors")
+        arr = (ct.c_void_p * 15)(*[int(p) for p in ext])
+        w = [_words(v) for v in (omega_m_mont, shift_mont, shift_n_mont, alpha_mont, beta_mont, gamma_mont,
+                                 k1_mont, k2_mont)]
+        self._check(self.lib.zk_plonk_quotient_evals(self.handle, arr, int(n), int(m), *w, num_out, zh_out,
+                                                     stream))
 
     def fr_count_nonzero(self, a: int, n: int, stream=None) -> int:
         r = ct.c_uint64(0)

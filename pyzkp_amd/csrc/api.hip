@@ -419,6 +419,21 @@ int zk_fr_inv(zk_ctx* ctx, const void* a, void* out, uint64_t n, void* stream) {
   return fr_inv(&ctx->c, (const fe*)a, n, (fe*)out, pick(ctx, stream));
 }
 
+int zk_plonk_quotient_evals(zk_ctx* ctx, const void* const* ext, uint64_t n, uint64_t m,
+                            const uint64_t omega_m[4], const uint64_t shift[4], const uint64_t shift_n[4],
+                            const uint64_t alpha[4], const uint64_t beta[4], const uint64_t gamma[4],
+                            const uint64_t k1[4], const uint64_t k2[4], void* num_out, void* zh_out,
+                            void* stream) {
+  CTX_CHECK(ctx);
+  if (!ext || !num_out || !zh_out || !omega_m || !shift || !shift_n || !alpha || !beta || !gamma || !k1 || !k2)
+    return set_err(&ctx->c, ZK_ERR_INVALID, "null argument");
+  const fe* e[15];
+  for (int k = 0; k < 15; k++) e[k] = (const fe*)ext[k];
+  return plonk_quotient_evals(&ctx->c, e, n, m, words_to_fe(omega_m), words_to_fe(shift), words_to_fe(shift_n),
+                              words_to_fe(alpha), words_to_fe(beta), words_to_fe(gamma), words_to_fe(k1),
+                              words_to_fe(k2), (fe*)num_out, (fe*)zh_out, pick(ctx, stream));
+}
+
 int zk_fr_count_nonzero(zk_ctx* ctx, const void* a, uint64_t n, uint64_t* result, void* stream) {
   CTX_CHECK(ctx);
   if (!result || (n && !a)) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");

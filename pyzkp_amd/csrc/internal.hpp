@@ -42,7 +42,7 @@ struct Ctx {
   // keyed by (log_n, base words, scale words)
   std::map<std::array<uint64_t, 9>, std::pair<DevBuf, DevBuf>> pow_tables;
   // growable scratch arenas
-  DevBuf scratch[6];
+  DevBuf scratch[8];  // 0-1 NTT, 2-3 MSM, 4 counters, 5 powers, 6 inversion, 7 PLONK quotient
   // tuning knobs (set through zk_set_tuning)
   int ntt_max_pass_bits = 0;      // 0: planner default
   uint32_t msm_segment = 64;      // sorted (bucket, point) pairs per accumulation thread
@@ -72,6 +72,16 @@ void phase_end(Ctx* c, const char* name, hipStream_t st, hipEvent_t ev);
 int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint32_t batch,
             bool inverse, const zkh::Fe* omega, const zkh::Fe* shift, hipStream_t st,
             int max_pass_bits, fe* tmp_buf = nullptr);
+// two-level power tables hi[h] = scale base^(2^POW_TABLE_LO_BITS h), lo[i] = base^i
+// (R = 2^261 packed), cached per context
+static constexpr uint32_t POW_TABLE_LO_BITS = 10;
+int ntt_pow_tables(Ctx* c, uint32_t log_n, const zkh::Fe& base, const zkh::Fe& scale, const fe** hi,
+                   const fe** lo, hipStream_t st);
+// PLONK quotient numerator and Z_H on the extended coset
+int plonk_quotient_evals(Ctx* c, const fe* const ext[15], uint64_t n, uint64_t m, const zkh::Fe& omega_m,
+                         const zkh::Fe& shift, const zkh::Fe& shift_n, const zkh::Fe& alpha,
+                         const zkh::Fe& beta, const zkh::Fe& gamma, const zkh::Fe& k1, const zkh::Fe& k2,
+                         fe* num, fe* zh, hipStream_t st);
 int ntt_prepare(Ctx* c, uint32_t log_n, bool inverse, const zkh::Fe* omega, const zkh::Fe* shift,
                 hipStream_t st);
 

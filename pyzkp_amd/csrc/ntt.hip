@@ -38,7 +38,7 @@ static constexpr int NTT_TILE = 1024;   // elements per LDS tile (36 KiB as limb
 static constexpr int NTT_TILE_BITS = 10;
 static constexpr int NTT_THREADS = 256;
 static constexpr int NTT_BFLY_PER_THREAD = NTT_TILE / 2 / NTT_THREADS;
-static constexpr int POW_LO_BITS = 10;  // two-level power tables: lo has 2^10 entries
+static constexpr int POW_LO_BITS = POW_TABLE_LO_BITS;  // two-level power tables: lo has 2^10 entries
 
 // tables: R = 2^261 values packed in 32 bytes
 __device__ __forceinline__ f9 tab9(const fe* __restrict__ t, uint64_t i) { return unpack9(load_fe(t + i)); }
@@ -224,6 +224,11 @@ static int get_pow_tables(Ctx* c, uint32_t log_n, const zkh::Fe& base, const zkh
   *hi = (const fe*)it->second.first.ptr;
   *lo = (const fe*)it->second.second.ptr;
   return ZK_OK;
+}
+
+int ntt_pow_tables(Ctx* c, uint32_t log_n, const zkh::Fe& base, const zkh::Fe& scale, const fe** hi,
+                   const fe** lo, hipStream_t st) {
+  return get_pow_tables(c, log_n, base, scale, hi, lo, st);
 }
 
 static int get_twiddles(Ctx* c, uint32_t log_n, const zkh::Fe& w, const uint32_t** out, hipStream_t st) {

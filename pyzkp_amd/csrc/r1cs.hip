@@ -31,12 +31,14 @@ __global__ __launch_bounds__(256) void gather_kernel(const fe* __restrict__ src,
 }
 
 // Montgomery inverse: a R -> a^-1 R, by a^(r-2) (0 -> 0)
-__global__ __launch_bounds__(256) void fr_inv_kernel(const fe* __restrict__ a, uint64_t n, fe* __restrict__ out) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// Element-wise inverse with 0 -> 0 (crypto/field/batch.py:8-38), by
+// simultaneous inversion: thread t takes the elements t, t + T, t + 2T, ...
+// (coalesced across threads), keeps their running product in out[] on the
+// way up, inverts the product once (a^(r-2)) and walks back down:
+// about 3 products per element plus one exponentiation per thread.
+__device__ fe fr_pow_r_minus_2(const fe& x) {
   const uint32_t e[8] = {0xefffffffu, 0x43e1f593u, 0x79b97091u, 0x2833e848u,
                          0x8181585du, 0xb85045b6u, 0xe131a029u, 0x30644e72u};  // r - 2
-  const fe x = load_fe(a + i);
   fe r = Fr::one();
   for (int wi = 7; wi >= 0; wi--) {
     const uint32_t word = e[wi];
@@ -45,7 +47,32 @@ __global__ __launch_bounds__(256) void fr_inv_kernel(const fe* __restrict__ a, u
       if ((word >> b) & 1u) r = Fr::mul(r, x);
     }
   }
-  store_fe(out + i, r);
+  return r;
+}
+
+// pre holds the running products (it is `out` itself unless a and out alias)
+__global__ __launch_bounds__(256) void fr_inv_kernel(const fe* a, uint64_t n, uint64_t T, fe* pre, fe* out) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T || t >= n) return;
+  fe acc = Fr::one();
+  for (uint64_t i = t; i < n; i += T) {
+    const fe x = load_fe(a + i);
+    store_fe(pre + i, acc);  // product of the earlier non-zero elements
+    if (!Fr::is_zero(x)) acc = Fr::mul(acc, x);
+  }
+  fe inv = fr_pow_r_minus_2(acc);
+  const uint64_t last = t + ((n - 1 - t) / T) * T;
+  for (uint64_t i = last + T; i >= t + T; i -= T) {
+    const uint64_t k = i - T;
+    const fe x = load_fe(a + k);
+    const fe p = load_fe(pre + k);
+    if (Fr::is_zero(x)) {
+      store_fe(out + k, Fr::zero());
+    } else {
+      store_fe(out + k, Fr::mul(p, inv));
+      inv = Fr::mul(inv, x);
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void count_nonzero_kernel(const fe* __restrict__ a, uint64_t n,
@@ -71,7 +98,17 @@ int fr_gather(Ctx* c, const fe* src, const uint32_t* idx, uint64_t n, fe* out, h
 
 int fr_inv(Ctx* c, const fe* a, uint64_t n, fe* out, hipStream_t st) {
   if (n == 0) return ZK_OK;
-  fr_inv_kernel<<<grid_for(n, 256), 256, 0, st>>>(a, n, out);
+  // about 64 elements per thread once there are enough of them for the device
+  uint64_t T = n / 64;
+  if (T < 65536) T = n < 65536 ? n : 65536;
+  fe* pre = out;
+  if ((const void*)a == (const void*)out) {
+    void* p;
+    int rc = ensure_scratch(c, 6, (size_t)n * 32, &p);
+    if (rc) return rc;
+    pre = (fe*)p;
+  }
+  fr_inv_kernel<<<grid_for(T, 256), 256, 0, st>>>(a, n, T, pre, out);
   ZK_HIP(c, hipGetLastError());
   return ZK_OK;
 }

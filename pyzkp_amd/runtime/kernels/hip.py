@@ -13,6 +13,8 @@ Registered under (op, Device.HIP, Backend.HIP):
   POINTWISE_MUL, POLY_SUB                     metal kernels.py:122-222
   MSM_G1                                      metal kernels.py:84-120
   KZG_BATCH_COMMIT, KZG_COMMIT, MSM_G1_BATCH  metal kernels.py:22-82, cpu kernels.py:338-496
+  BATCH_INV                                   cpu kernels.py:253-269
+  PLONK_T_QUOTIENT_EVALS                      cpu kernels.py:607-694
   FROM_DEVICE (device side)                   cpu kernels.py:75-77 registers it on METAL too
 and, under (op, Device.CPU, Backend.HIP), the transfer ops TO_DEVICE /
 FROM_DEVICE (cpu kernels.py:98-172), which pack on the host only what has
@@ -76,6 +78,8 @@ def register_hip_kernels(registry: KernelRegistry) -> None:
     registry.register(OpType.MSM_G1_BATCH, H, _msm_g1_batch, backend=B)
     registry.register(OpType.KZG_COMMIT, H, _kzg_commit, backend=B)
     registry.register(OpType.KZG_BATCH_COMMIT, H, _kzg_batch_commit, backend=B)
+    registry.register(OpType.BATCH_INV, H, _batch_inv, backend=B)
+    registry.register(OpType.PLONK_T_QUOTIENT_EVALS, H, _plonk_t_quotient_evals, backend=B)
 
 
 # ------------------------------------------------------------------ helpers
@@ -266,6 +270,43 @@ def _pointwise_mul(ctx):
 
 def _poly_sub(ctx):
     return _binary(ctx, "poly_sub", "fr_sub")
+
+
+def _batch_inv(ctx):
+    """BATCH_INV (cpu kernels.py:253-269, crypto/field/batch.py:8-38): element-wise
+    inverse, 0 -> 0, by simultaneous inversion on the device."""
+    node = ctx["node"]
+    a = _fr_buffer(ctx, ctx["inputs"][0], "batch_inv")
+    out = _new_fr(ctx, a.n)
+    if a.n:
+        _hip(ctx).fr_inv(a.ptr, out.ptr, a.n)
+    oid = node.outputs[0]
+    return {"outputs": {oid: Buffer(id=oid, device=Device.HIP, dtype=DType.FR, data=out, meta={"n": a.n})}}
+
+
+_PLONK_Q_INPUTS = ("a", "b", "c", "z", "pi", "ql", "qr", "qm", "qo", "qc", "s1", "s2", "s3", "zshift", "l1")
+
+
+def _plonk_t_quotient_evals(ctx):
+    """PLONK_T_QUOTIENT_EVALS (cpu kernels.py:607-694): num and Z_H over the
+    size-m coset in one fused device pass.  attrs: n, m, omega_m, shift,
+    shift_n, alpha, beta, gamma, k1, k2 (canonical ints)."""
+    node, attrs = ctx["node"], ctx["attrs"]
+    n, m = int(attrs["n"]), int(attrs["m"])
+    ins = [_fr_buffer(ctx, b, "plonk_t_quotient_evals") for b in ctx["inputs"][:15]]
+    if len(ins) != 15:
+        raise ValueError("plonk_t_quotient_evals needs 15 inputs")
+    if not all(len(v) == m for v in ins):
+        raise ValueError("eval length mismatch")
+    mont = [fr_to_mont(int(attrs[k]) % FR_MODULUS)
+            for k in ("omega_m", "shift", "shift_n", "alpha", "beta", "gamma", "k1", "k2")]
+    num, zh = _new_fr(ctx, m), _new_fr(ctx, m)
+    _hip(ctx).plonk_quotient_evals([b.ptr for b in ins], n, m, *mont, num.ptr, zh.ptr)
+    o_num, o_zh = node.outputs[0], node.outputs[1]
+    return {"outputs": {
+        o_num: Buffer(id=o_num, device=Device.HIP, dtype=DType.FR, data=num, meta={"m": m}),
+        o_zh: Buffer(id=o_zh, device=Device.HIP, dtype=DType.FR, data=zh, meta={"m": m}),
+    }}
 
 
 # ---------------------------------------------------------------------- MSM
