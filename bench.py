@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["msm", "ntt", "batch", "serve", "copy"], default="msm")
+    ap.add_argument("--workload", choices=["msm", "ntt", "batch", "serve", "copy", "plonkq"], default="msm")
     ap.add_argument("--requests", type=int, default=48, help="serve: number of proof requests")
     ap.add_argument("--serve-log", type=str, default="10,20", help="serve: constraint range 2^lo..2^hi")
     ap.add_argument("--cpu-threads", type=int, default=16, help="serve: host threads for G2 MSMs")
@@ -213,6 +213,64 @@ def run_copy(args):
                       "ms_per_step": round(dt * 1e3, 4), "steps": args.steps, "warmup": args.warmup}))
 
 
+def run_plonkq(args, ctx):
+    """PLONK quotient (protocols/plonk/prove.py:300-424) at n = 2^log_n
+    (default 2^20, extended coset m = 4n): one step = the 15 coset NTTs of
+    the zero-padded coefficient vectors, the fused numerator / Z_H kernel,
+    the batch inversion, the point-wise product and the coset iNTT, with
+    the inputs resident in HBM (random coefficients, seed 0x6001).  The
+    degree-check download is not part of the step."""
+    import numpy as np
+    from pyzkp_amd import workload
+    from pyzkp_amd.field import FR_GENERATOR, FR_MODULUS, fr_to_mont, omega_for_size
+
+    log_n = args.log_n or 20
+    n, m = 1 << log_n, 1 << (log_n + 2)
+    omega, omega_m = omega_for_size(n), omega_for_size(m)
+    shift = FR_GENERATOR
+    u = pow(omega_m, n, FR_MODULUS)
+    assert pow(shift, n, FR_MODULUS) not in {1, u, u * u % FR_MODULUS, u * u * u % FR_MODULUS}
+    log(f"plonk quotient n=2^{log_n}, m=2^{log_n + 2}")
+    src = workload.device_fr_mont(ctx, workload.fr_uniform(0x6001, 14 * n))   # 13 inputs + L1
+    ext = [ctx.alloc(32 * m) for _ in range(15)]
+    num, zh, t = ctx.alloc(32 * m), ctx.alloc(32 * m), ctx.alloc(32 * m)
+    sh_m, shw_m = fr_to_mont(shift), fr_to_mont(shift * omega % FR_MODULUS)
+    consts = [fr_to_mont(v) for v in (omega_m, shift, pow(shift, n, FR_MODULUS), 3, 5, 7, 11, 13)]
+    order = list(range(13)) + [3, 13]          # z(omega X) from z; L1 last
+
+    def step():
+        for k, j in enumerate(order):
+            ctx.ntt(src.ptr + 32 * n * j, n, ext[k].ptr, log_n + 2, False,
+                    shift_mont=shw_m if k == 13 else sh_m)
+        ctx.plonk_quotient_evals([b.ptr for b in ext], n, m, *consts, num.ptr, zh.ptr)
+        ctx.fr_inv(zh.ptr, zh.ptr, m)
+        ctx.fr_mul(num.ptr, zh.ptr, t.ptr, m)
+        ctx.ntt(t.ptr, m, t.ptr, log_n + 2, True, shift_mont=sh_m)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    # bytes: 16 NTTs of m (64 m each) + fused pass (17 vectors) + inversion (3 reads / 2 writes) + product (3)
+    algo_bytes = 16 * 64 * m + 17 * 32 * m + 5 * 32 * m + 3 * 32 * m
+    line = {
+        "metric": "PLONK quotient (15 coset NTTs at 4n + fused numerator + batch inverse + coset iNTT) per second",
+        "value": round(1.0 / dt, 3), "unit": "quotients/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32 limbs: Fr Montgomery (254-bit integer)", "data": "synthetic",
+        "config": {"workload": f"PLONK quotient n=2^{log_n} (m=2^{log_n + 2}), random coefficients seed 0x6001, "
+                               "shift = 5", "algorithmic_GBps": round(algo_bytes / dt / 1e9, 1)},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+    for b in ext + [num, zh, t, src]:
+        b.free()
+
+
 def run_serve(args, ctx):
     """Served Groth16: Poisson arrivals (seed 0x5001) of requests whose circuit
     has floor(2^U) constraints, U ~ U[lo, hi] (seed 0x5002), padded to 2^k;
@@ -341,6 +399,10 @@ def main():
 
     if args.workload == "batch":
         run_batch(args, ctx, peak_mads)
+        return
+
+    if args.workload == "plonkq":
+        run_plonkq(args, ctx)
         return
 
     if args.workload == "serve":
