@@ -110,6 +110,7 @@ def setup(circ: SyntheticCircuit, ctx, seed: int = 0x5EED, window_bits: int = 0)
     ctx.g1_fixed_base(_g1_table(ctx).ptr, scal.ptr, total, pts.ptr)
     ctx.sync()
     fixed = ctx.fixed_bases_create(pts.ptr, total, window_bits)
+    csr_arrays = {name: (m.row_ptr, m.cols, m.coeffs) for name, m in (("A", circ.A), ("B", circ.B), ("C", circ.C))}
 
     # public inputs: ic_i = [K_i / gamma] G1   (setup.py:77-79, 91-93)
     ctx.fr_from_mont(K.ptr, tmp.ptr, nv)
@@ -130,7 +131,12 @@ def setup(circ: SyntheticCircuit, ctx, seed: int = 0x5EED, window_bits: int = 0)
                     trapdoor_gamma=gamma, trapdoor_delta=delta, n=n, n_vars=nv, public_ids=list(circ.public_ids),
                     ctx=ctx, fixed=fixed, query_offsets=(0, nv, 2 * nv, 2 * nv + n_h), query_points=pts,
                     b_query_g2=b_g2, csr=csr, aux_dev=aux_dev, query_total=total)
-    W = (255 + fixed.window_bits - 1) // fixed.window_bits
-    csr_bytes = sum(len(m.row_ptr) * 4 + len(m.cols) * 36 for m in (circ.A, circ.B, circ.C))
-    pk.hbm_bytes = 64 * total * (W + 1) + csr_bytes + 4 * len(aux)
+    pk.csr_host = csr_arrays
+    pk.hbm_bytes = key_hbm_bytes(pk)
     return pk
+
+
+def key_hbm_bytes(pk: ProvingKey) -> int:
+    W = (255 + pk.fixed.window_bits - 1) // pk.fixed.window_bits
+    csr_bytes = sum(len(rp) * 4 + len(cols) * 36 for rp, cols, _ in pk.csr_host.values())
+    return 64 * pk.query_total * (W + 1) + csr_bytes + 4 * len(pk.aux_ids)

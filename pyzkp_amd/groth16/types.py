@@ -91,3 +91,4 @@ class ProvingKey:
         return out
 
     query_total: int = 0
+    csr_host: Any = None                  # dict name -> (row_ptr, cols, coeffs) host arrays

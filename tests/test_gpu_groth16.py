@@ -70,3 +70,22 @@ def test_groth16_batch_equals_alone(zctx):
         wi = words_to_ints(ws[i][[circ.public_ids[1]]])
         assert verify(pk.vk, [1, wi[0]], many[i])
     pk.free()
+
+
+def test_key_cache_roundtrip(zctx, tmp_path):
+    from pyzkp_amd.groth16.keycache import CacheMismatchError, load_key, save_key
+    circ = synthetic_circuit(9, 0x5310)
+    pk = setup(circ, zctx, seed=0x5311)
+    path = str(tmp_path / "key.npz")
+    save_key(path, pk, circ)
+    pk2 = load_key(path, circ, zctx)
+    w = synthetic_witness(circ, 0x5312)
+    a = prove(pk, w, 11, 22)
+    b = prove(pk2, w, 11, 22)
+    assert a.to_bytes() == b.to_bytes()
+    pub = words_to_ints(w[circ.public_ids])
+    assert verify(pk2.vk, pub, b)
+    with pytest.raises(CacheMismatchError):
+        load_key(path, synthetic_circuit(9, 0x5313), zctx)
+    pk.free()
+    pk2.free()
