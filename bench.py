@@ -78,7 +78,7 @@ def ntt_block(ctx, log_n, steps, warmup):
     from pyzkp_amd.field import FR_GENERATOR, fr_to_mont
 
     n = 1 << log_n
-    src = workload.device_fr_mont(ctx, workload.fr_uniform(0x2002, n))
+    src = workload.device_fr_mont(ctx, workload.fr_uniform(0x2001 if log_n == 20 else 0x2002, n))
     dst = ctx.alloc(32 * n)
     shift = fr_to_mont(FR_GENERATOR)
     res = {}
@@ -421,7 +421,8 @@ def main():
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": fwd["ms"],
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "u32x8 Montgomery Fr (254-bit integer)", "data": "synthetic",
-            "config": {"workload": f"Fr NTT forward, n=2^{log_n}, uniform coefficients seed 0x2002",
+            "config": {"workload": f"Fr NTT forward, n=2^{log_n}, uniform coefficients seed "
+                                   f"{'0x2001' if log_n == 20 else '0x2002'}",
                        "variants": r},
             "roofline": {"bound": "valu", "achieved": round(mads / (fwd["device_ms"] / 1e3) / 1e12, 3),
                          "peak": round(peak_mads / 1e12, 3), "unit": "T lane-mads/s",
@@ -437,12 +438,14 @@ def main():
     log_n = args.log_n or 22
     n = 1 << log_n
     log(f"generating MSM inputs n=2^{log_n}{' (skewed)' if args.skewed else ''}")
-    ks = workload.fr_uniform(0x3002, n)
+    # the 2^12 single-MSM case uses its own seeds (0x1001 scalars, 0x1002 bases)
+    seed_s, seed_k = (0x1001, 0x1002) if log_n == 12 else (0x3001, 0x3002)
+    ks = workload.fr_uniform(seed_k, n)
     if args.skewed:
         ss = workload.scalars_skewed(0x3003, n)
         ks = workload.repeat_some(0x3004, ks, 0.05)
     else:
-        ss = workload.fr_uniform(0x3001, n)
+        ss = workload.fr_uniform(seed_s, n)
     bases = workload.device_bases(ctx, ks)
     scal = workload.device_fr_mont(ctx, ss)
 
