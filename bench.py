@@ -472,9 +472,13 @@ def main():
     wall = time.perf_counter() - t0
     ctx.profile_enable(False)
     phases = {}
+    launches = {}
     for ph in ("msm_digits", "msm_sort", "msm_accumulate", "msm_fixup", "msm_reduce"):
         ms, cnt = ctx.profile_get(ph)
-        phases[ph] = round(ms / max(cnt, 1), 4)
+        # device time per MSM, summed over its window groups (groups on two
+        # streams overlap, so the phases can add up to more than the step)
+        phases[ph] = round(ms / args.steps, 4)
+        launches[ph] = cnt // max(args.steps, 1)
     ms_per_step = wall / args.steps * 1e3
     value = n * args.steps / wall
 
@@ -493,6 +497,7 @@ def main():
                 "unit": "T lane-mads/s", "frac": round(achieved / peak_mads, 4),
                 "traffic": measured_traffic(f"msm_2^{log_n}_c{c}", "msm_accumulate_kernel"),
                 "kernel": "msm_accumulate_kernel",
+                "launches_per_step": launches["msm_accumulate"],
                 "work_per_launch": f"n*W*10 Fq-mul*136 = {algo_mads:.3e} v_mad_u64_u32"}
 
     secondary = {}

@@ -25,6 +25,16 @@ int hip_check(Ctx* c, hipError_t e, const char* what) {
   return set_err(c, ZK_ERR_RUNTIME, buf);
 }
 
+int ensure_aux(Ctx* c, size_t count) {
+  if (!c->aux_stream) ZK_HIP(c, hipStreamCreateWithFlags(&c->aux_stream, hipStreamNonBlocking));
+  while (c->events.size() < count) {
+    hipEvent_t e;
+    ZK_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    c->events.push_back(e);
+  }
+  return ZK_OK;
+}
+
 int ensure_scratch(Ctx* c, int slot, size_t bytes, void** out) {
   DevBuf& b = c->scratch[slot];
   if (b.bytes < bytes) {
@@ -160,6 +170,8 @@ int zk_destroy(zk_ctx* ctx) {
   }
   for (auto& b : ctx->c.scratch)
     if (b.ptr) hipFree(b.ptr);
+  for (hipEvent_t e : ctx->c.events) hipEventDestroy(e);
+  if (ctx->c.aux_stream) hipStreamDestroy(ctx->c.aux_stream);
   hipStreamDestroy(ctx->c.stream);
   delete ctx;
   return ZK_OK;
@@ -453,6 +465,9 @@ int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value) {
   } else if (k == "msm_reduce_chunk") {
     if (value < 0 || value > 65536) return set_err(&ctx->c, ZK_ERR_INVALID, "msm_reduce_chunk must be in [0, 65536]");
     ctx->c.msm_reduce_chunk = (uint32_t)value;
+  } else if (k == "msm_groups") {
+    if (value < 0 || value > 16) return set_err(&ctx->c, ZK_ERR_INVALID, "msm_groups must be in [0, 16]");
+    ctx->c.msm_groups = (uint32_t)value;
   } else {
     return set_err(&ctx->c, ZK_ERR_INVALID, "unknown tuning key: " + k);
   }

@@ -47,6 +47,10 @@ struct Ctx {
   int ntt_max_pass_bits = 0;      // 0: planner default
   uint32_t msm_segment = 64;      // sorted (bucket, point) pairs per accumulation thread
   uint32_t msm_reduce_chunk = 0;  // buckets per reduction thread (0: sized to fill the device)
+  uint32_t msm_groups = 0;        // window groups pipelined over two streams (0: chosen per shape)
+  // second stream (default priority) and events for pipelined launches, created on first use
+  hipStream_t aux_stream = nullptr;
+  std::vector<hipEvent_t> events;
   bool profiling = false;
   std::map<std::string, PhaseTimer> phases;
 };
@@ -62,6 +66,8 @@ int hip_check(Ctx* c, hipError_t e, const char* what);
 
 // device memory helpers (context-owned) -------------------------------------
 int ensure_scratch(Ctx* c, int slot, size_t bytes, void** out);
+// the context's second stream and at least `count` timing-free events
+int ensure_aux(Ctx* c, size_t count);
 int upload_table(Ctx* c, const std::vector<zkh::Fe>& host, DevBuf* out, hipStream_t st);
 
 // profiling helpers: bracket a launch with events when profiling is on

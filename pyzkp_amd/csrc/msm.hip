@@ -830,7 +830,9 @@ static int make_shape(Ctx* c, int cb, uint32_t nrows, const std::vector<MsmRow>&
   return ZK_OK;
 }
 
-static size_t layout(const MsmShape& sh, char* base, MsmBuffers* b) {
+// digits_ext: the rows' digits live elsewhere (then the sorted entries reuse
+// that region, which must hold sh.pairs words and belong to these rows only)
+static size_t layout(const MsmShape& sh, char* base, MsmBuffers* b, uint32_t* digits_ext = nullptr) {
   size_t off = 0;
   auto take = [&](size_t bytes) {
     size_t o = off;
@@ -838,7 +840,7 @@ static size_t layout(const MsmShape& sh, char* base, MsmBuffers* b) {
     return base ? (void*)(base + o) : nullptr;
   };
   MsmBuffers t;
-  t.digits = (uint32_t*)take(sh.pairs * 4);
+  t.digits = digits_ext ? digits_ext : (uint32_t*)take(sh.pairs * 4);
   t.sorted = t.digits;  // the digits are dead once the scatter pass has run
   t.hist = (uint32_t*)take((size_t)(sh.hist_len + 1) * 4);
   t.items_v = (uint32_t*)take(sh.pairs * 4);
@@ -875,7 +877,8 @@ static void scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t len, uint
 
 // sort -> accumulate -> fix-up -> per-row reduction -> export (R = 2^256 XYZZ
 // per row in b.rexp).  No allocation, no host synchronisation.
-static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* bases9, hipStream_t st) {
+static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* bases9, hipStream_t st,
+                    hipEvent_t sorted_ev = nullptr) {
   hipEvent_t ev;
   phase_begin(c, "msm_sort", st, &ev);
   const dim3 g(sh.max_tiles, sh.nrows);
@@ -904,6 +907,7 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
   }
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_sort", st, ev);
+  if (sorted_ev) ZK_HIP(c, hipEventRecord(sorted_ev, st));
 
   phase_begin(c, "msm_accumulate", st, &ev);
   msm_accumulate_kernel<<<grid_for(sh.Smax, 256), 256, 0, st>>>(bases9, b.sorted, b.row_ptr, sh.nkeys, sh.L,
@@ -956,6 +960,14 @@ static void export_jacobian(const xyzz& acc, uint64_t out[12]) {
   }
 }
 
+// With the msm_groups tuning knob above 1, the windows are split into G
+// groups, each its own sort -> accumulate -> fix-up -> reduce sequence;
+// groups alternate between the context stream and a second stream, and
+// group g + 1 starts sorting when group g's sort is done, so that the sort
+// and reduction of one group can run beside the accumulation of its
+// neighbour.  On MI355X at 2^22 this measured slower than one group (the
+// accumulation occupies every CU, so little overlaps, and every group pays
+// the reduction's latency), so the default is one group.
 int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
                uint64_t out_jac[12], hipStream_t st) {
   for (int i = 0; i < 12; i++) out_jac[i] = 0;
@@ -964,44 +976,79 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   const int cb = window_bits > 0 ? window_bits : default_window_bits(n);
   if (cb < 2 || cb > 24) return set_err(c, ZK_ERR_INVALID, "window_bits must be in [2, 24]");
   const uint32_t W = (255 + cb - 1) / cb;
+  uint32_t G = c->msm_groups ? c->msm_groups : 1u;
+  if (G > W) G = W;
+  std::vector<uint32_t> wb(G + 1);
+  for (uint32_t g = 0; g <= G; g++) wb[g] = (uint32_t)((uint64_t)g * W / G);
   // rows = windows: row w holds digits[w n .. w n + n), base index = position
-  std::vector<MsmRow> rows(W);
   const uint32_t nt = (uint32_t)((n + SORT_TILE - 1) / SORT_TILE);
-  for (uint32_t w = 0; w < W; w++) {
-    rows[w].off = (uint64_t)w * n;
-    rows[w].len = (uint32_t)n;
-    rows[w].inner = (uint32_t)n;
-    rows[w].stride = 0;
-    rows[w].ntiles = nt;
-    rows[w].tile0 = w * nt;
-    rows[w].boff = 0;
+  std::vector<std::vector<MsmRow>> rows(G);
+  std::vector<MsmShape> sh(G);
+  std::vector<size_t> off(G + 1);
+  const size_t o_bases = align256((size_t)W * n * 4);
+  off[0] = align256(o_bases + (size_t)n * 64);
+  for (uint32_t g = 0; g < G; g++) {
+    for (uint32_t w = wb[g]; w < wb[g + 1]; w++) {
+      MsmRow r;
+      r.off = (uint64_t)(w - wb[g]) * n;
+      r.len = (uint32_t)n;
+      r.inner = (uint32_t)n;
+      r.stride = 0;
+      r.ntiles = nt;
+      r.tile0 = (w - wb[g]) * nt;
+      r.boff = 0;
+      rows[g].push_back(r);
+    }
+    int rc = make_shape(c, cb, wb[g + 1] - wb[g], rows[g], sh[g]);
+    if (rc) return rc;
+    uint32_t dummy = 0;
+    off[g + 1] = align256(off[g] + layout(sh[g], nullptr, nullptr, &dummy));
   }
-  MsmShape sh;
-  int rc = make_shape(c, cb, W, rows, sh);
-  if (rc) return rc;
-  const size_t bytes = layout(sh, nullptr, nullptr);
-  const size_t o_bases = align256(bytes);
   void* base;
-  rc = ensure_scratch(c, 2, o_bases + (size_t)n * 64, &base);
+  int rc = ensure_scratch(c, 2, off[G], &base);
   if (rc) return rc;
-  MsmBuffers b;
-  layout(sh, (char*)base, &b);
+  if (G > 1) {
+    rc = ensure_aux(c, G + 2);
+    if (rc) return rc;
+  }
+  uint32_t* digits = (uint32_t*)base;
   fe* bases9 = (fe*)((char*)base + o_bases);
-  ZK_HIP(c, hipMemcpyAsync(b.rows, rows.data(), rows.size() * sizeof(MsmRow), hipMemcpyHostToDevice, st));
+  std::vector<MsmBuffers> b(G);
+  for (uint32_t g = 0; g < G; g++) {
+    layout(sh[g], (char*)base + off[g], &b[g], digits + (uint64_t)wb[g] * n);
+    ZK_HIP(c, hipMemcpyAsync(b[g].rows, rows[g].data(), rows[g].size() * sizeof(MsmRow), hipMemcpyHostToDevice, st));
+  }
 
   hipEvent_t ev;
   phase_begin(c, "msm_digits", st, &ev);
   msm_bases_convert_kernel<<<grid_for(n, 256), 256, 0, st>>>(bases, n, bases9);
-  msm_digits_kernel<<<grid_for(n, 256), 256, 0, st>>>(scalars, n, (uint32_t)cb, W, b.digits);
+  msm_digits_kernel<<<grid_for(n, 256), 256, 0, st>>>(scalars, n, (uint32_t)cb, W, digits);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_digits", st, ev);
 
-  rc = run_rows(c, sh, b, bases9, st);
-  if (rc) return rc;
+  if (G == 1) {
+    rc = run_rows(c, sh[0], b[0], bases9, st);
+    if (rc) return rc;
+  } else {
+    hipStream_t S[2] = {st, c->aux_stream};
+    hipEvent_t* E = c->events.data();  // E[g]: group g sorted; E[G]: digits ready; E[G + 1]: aux done
+    ZK_HIP(c, hipEventRecord(E[G], st));
+    ZK_HIP(c, hipStreamWaitEvent(S[1], E[G], 0));
+    for (uint32_t g = 0; g < G; g++) {
+      hipStream_t s = S[g & 1];
+      if (g > 0) ZK_HIP(c, hipStreamWaitEvent(s, E[g - 1], 0));
+      rc = run_rows(c, sh[g], b[g], bases9, s, E[g]);
+      if (rc) return rc;
+    }
+    ZK_HIP(c, hipEventRecord(E[G + 1], S[1]));
+    ZK_HIP(c, hipStreamWaitEvent(st, E[G + 1], 0));
+  }
 
   // window combination on the host: acc = Sum_w 2^(c w) S_w (Horner)
   std::vector<xyzz> hs(W);
-  ZK_HIP(c, hipMemcpyAsync(hs.data(), b.rexp, (size_t)W * sizeof(xyzz), hipMemcpyDeviceToHost, st));
+  for (uint32_t g = 0; g < G; g++)
+    ZK_HIP(c, hipMemcpyAsync(hs.data() + wb[g], b[g].rexp, (size_t)(wb[g + 1] - wb[g]) * sizeof(xyzz),
+                             hipMemcpyDeviceToHost, st));
   ZK_HIP(c, hipStreamSynchronize(st));
   xyzz acc = hs[W - 1];
   for (int w = (int)W - 2; w >= 0; w--) {

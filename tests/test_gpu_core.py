@@ -137,6 +137,25 @@ def test_msm_same_scalar_everywhere(zctx):
     zctx.set_tuning("msm_segment", 64)
 
 
+def test_msm_window_groups(zctx):
+    # windows split into groups pipelined over two streams (tuning knob)
+    from pyzkp_amd import workload
+    from pyzkp_amd.field import words_to_ints
+    n = 1 << 16
+    kw = workload.fr_uniform(0x61, n)
+    sw = workload.fr_uniform(0x62, n)
+    bases = workload.device_bases(zctx, kw)
+    sc = workload.device_fr_mont(zctx, sw)
+    expect = M.msm_closed_form(words_to_ints(kw), words_to_ints(sw))
+    try:
+        for g in (2, 3, 5):
+            zctx.set_tuning("msm_groups", g)
+            for c in (8, 13):
+                assert msm_affine(zctx, bases, sc, n, c) == expect, (g, c)
+    finally:
+        zctx.set_tuning("msm_groups", 0)
+
+
 def test_msm_wide_windows(zctx):
     # c >= 20: up to 2^13 fine keys per coarse key in the bucket sort
     n = 100
