@@ -58,6 +58,23 @@ def parse():
     return ap.parse_args()
 
 
+def cpu_msm_baseline(ctx, bases_ptr: int, scalar_words, m: int):
+    """The C restatement of the reference Pippenger (crypto/msm/pippenger.py),
+    one thread, on m canonical points / scalars; returns (pts/s, seconds, result)."""
+    import numpy as np
+    from oracle import cref
+    from oracle.msm import default_window_bits
+    bc = ctx.alloc(64 * m)
+    ctx.fq_from_mont(bases_ptr, bc.ptr, 2 * m)
+    braw = np.frombuffer(ctx.download(bc.ptr, 64 * m), dtype="<u8")
+    bc.free()
+    sraw = np.ascontiguousarray(scalar_words[:m]).reshape(-1)
+    t0 = time.perf_counter()
+    res = cref.msm_pippenger_words(braw, sraw, m, default_window_bits(m), threads=1)
+    dt = time.perf_counter() - t0
+    return m / dt, dt, res
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` for this workload, read from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a
@@ -186,6 +203,13 @@ def run_batch(args, ctx, peak_mads):
                    "speedup_vs_one_at_a_time": round(t_plain / step, 3), "verified_closed_form": bool(ok)},
         "roofline": None, "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        m = nb
+        rate, dt, _ = cpu_msm_baseline(ctx, bases.ptr, ss, m)
+        line["cpu_baseline"] = {"value": round(rate, 1), "unit": "pts/s", "cores": 1, "kind": "port",
+                                "sample": f"C restatement of crypto/msm/pippenger.py, one MSM over the whole "
+                                          f"2^18-point SRS with the batch's first 2^18 scalars, 1 thread, "
+                                          f"{dt:.2f} s"}
     print(json.dumps(line))
     fb.free()
     for b in keep + [bases, scal]:
@@ -266,6 +290,18 @@ def run_plonkq(args, ctx):
                                "shift = 5", "algorithmic_GBps": round(algo_bytes / dt / 1e9, 1)},
         "roofline": None, "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        from oracle import plonk as OP
+        from pyzkp_amd.field import words_to_ints
+        nc = 1 << 11
+        vals = words_to_ints(workload.fr_uniform(0x6001, 13 * nc))
+        co = {k: vals[i * nc:(i + 1) * nc] for i, k in enumerate(OP.Q_INPUTS[:13])}
+        t0 = time.perf_counter()
+        OP.quotient_full(co, n=nc, omega=omega_for_size(nc), k1=7, k2=13, alpha=3, beta=5, gamma=7, shift=shift)
+        cdt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(1.0 / cdt, 3), "unit": "quotients/s", "cores": 1, "kind": "port",
+                                "sample": f"the oracle's restatement of protocols/plonk/prove.py:300-424 (Python), "
+                                          f"n=2^11, 1 thread, {cdt:.2f} s: a smaller instance than the GPU's"}
     print(json.dumps(line))
     for b in ext + [num, zh, t, src]:
         b.free()
@@ -368,6 +404,22 @@ def run_serve(args, ctx):
                    "hbm_budget_gb": args.hbm_budget_gb},
         "roofline": None, "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        from oracle import groth16 as OG
+        k0 = ks[0]
+        circ = circuits[k0]
+        w = words_to_ints(synthetic_witness(circ, 0x5100))
+        A, B, C = circ.A.row_dicts(), circ.B.row_dicts(), circ.C.row_dicts()
+        pk0 = keys[k0]
+        td = (pk0.trapdoor_tau, pk0.trapdoor_alpha, pk0.trapdoor_beta, pk0.trapdoor_gamma, pk0.trapdoor_delta)
+        opk = OG.setup(A, B, C, circ.n, circ.n_vars, circ.public_ids, list(circ.aux_ids), td)
+        t0 = time.perf_counter()
+        OG.prove(opk, A, B, C, w, 3, 4)
+        cdt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(1.0 / cdt, 4), "unit": "proofs/s", "cores": 1, "kind": "port",
+                                "sample": f"the oracle's Groth16 prover (Python restatement of "
+                                          f"protocols/groth16/prove.py), one proof of the smallest class 2^{k0}, "
+                                          f"1 thread, {cdt:.2f} s"}
     print(json.dumps(line))
     for pk in keys.values():
         release_work(pk)
@@ -432,6 +484,18 @@ def main():
                          "hbm_GBps_algorithmic": fwd["GBps"], "hbm_frac": round(fwd["GBps"] / 8000.0, 4)},
             "cpu_baseline": None,
         }
+        if not args.no_cpu_baseline:
+            from oracle import cref
+            from pyzkp_amd.field import omega_for_size
+            nc = 1 << min(22, log_n)
+            wds = workload.fr_uniform(0x2001, nc)
+            t0 = time.perf_counter()
+            cref.ntt_words(wds, nc, nc.bit_length() - 1, omega_for_size(nc), False, None, 1)
+            cdt = time.perf_counter() - t0
+            line["cpu_baseline"] = {"value": round(64 * nc / cdt / 1e9, 3), "unit": "GB/s", "cores": 1,
+                                    "kind": "port",
+                                    "sample": f"C restatement of crypto/poly/ntt.py (bit-reverse radix-2 DIT), "
+                                              f"forward n=2^{nc.bit_length() - 1}, 1 thread, {cdt:.2f} s"}
         print(json.dumps(line))
         return
 
