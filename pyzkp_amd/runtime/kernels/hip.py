@@ -369,13 +369,53 @@ def _msm_g1(ctx):
     return {"outputs": {oid: Buffer(id=oid, device=Device.CPU, dtype=DType.G1, data=acc)}}
 
 
+# Fixed-base sets (W tables 2^(c w) B_i of an SRS prefix) cached like the
+# device bases, for nodes whose attrs ask for fixed-base MSMs (the
+# reference's fixed_base / warmup_fixed_base attrs, runtime/warmup.py:84-150,
+# crypto/msm/pippenger.py:217-342).  With them a batch of commitments over
+# one SRS is one device launch.
+_FIXED_CACHE: Dict[tuple, tuple] = {}
+
+
+def fixed_set(ctx, points, n: int, window_bits: int = 0):
+    hip = _hip(ctx)
+    key = (id(points), int(n), int(window_bits), id(hip))
+    ent = _FIXED_CACHE.get(key)
+    if ent is None:
+        bases = device_bases(ctx, points, n)
+        ent = (points, hip.fixed_bases_create(bases.dbuf.ptr, n, window_bits))
+        _FIXED_CACHE[key] = ent
+    return ent[1]
+
+
+def _commit_fixed(ctx, points, scalar_lists, max_len: int) -> list:
+    attrs = ctx.get("attrs") or {}
+    fb = fixed_set(ctx, points, max_len, int(attrs.get("fixed_base_window_bits", 0) or 0))
+    sizes = [len(sc) for sc in scalar_lists]
+    offs, acc = [], 0
+    for m in sizes:
+        offs.append(acc)
+        acc += m
+    hb = upload_fr(ctx, [v for sc in scalar_lists for v in sc])
+    res = _hip(ctx).msm_g1_fixed_batch(fb, hb.ptr, offs, sizes)
+    pool = ctx.get("pool")
+    if pool is not None and hasattr(pool, "release_device"):
+        pool.release_device(hb.dbuf)
+    else:
+        hb.dbuf.free()
+    return [from_jacobian_mont(*r) if m else G1_ZERO for r, m in zip(res, sizes)]
+
+
 def _commit_many(ctx, points, scalar_lists) -> list:
-    """One MSM per scalar list over the prefix of `points` (host ints)."""
+    """One MSM per scalar list over the prefix of `points` (host ints); with
+    attrs["fixed_base"] all of them in one fixed-base batch launch."""
     max_len = max((len(s) for s in scalar_lists), default=0)
     if max_len > len(points):
         raise ValueError("SRS too small for polynomial degree")
     if max_len == 0:
         return [G1_ZERO] * len(scalar_lists)
+    if (ctx.get("attrs") or {}).get("fixed_base"):
+        return _commit_fixed(ctx, points, scalar_lists, max_len)
     bases = device_bases(ctx, points, max_len)
     outs = []
     for sc in scalar_lists:

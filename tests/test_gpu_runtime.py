@@ -118,6 +118,18 @@ def test_kzg_batch_commit_golden():
     for k, cm in zip(names, got):
         assert normalize(cm) == tuple(pk["commits"][k]), k
     assert normalize(got[-1]) is None
+    # the same commitments as one fixed-base batch launch (attrs fixed_base)
+    for wbits in (0, 5):
+        g3 = Graph()
+        g3.add_buffer(id="srs", device=Device.CPU, dtype=DType.OBJ, data=SRS.g1_powers)
+        g3.add_buffer(id="polys", device=Device.CPU, dtype=DType.OBJ, data=[pk["coeffs"][k] for k in names] + [[]])
+        g3.add_node(op=OpType.KZG_BATCH_COMMIT, inputs=["srs", "polys"], outputs=["cms"],
+                    attrs={"fixed_base": True, "fixed_base_window_bits": wbits})
+        _exe().run(g3, runtime_config=RuntimeConfig(backend=Backend.HIP))
+        got3 = g3.buffers["cms"].data
+        for k, cm in zip(names, got3):
+            assert normalize(cm) == tuple(pk["commits"][k]), (k, wbits)
+        assert normalize(got3[-1]) is None
     # polynomial longer than the SRS -> ValueError
     g2 = Graph()
     g2.add_buffer(id="srs", device=Device.CPU, dtype=DType.OBJ, data=SRS())
