@@ -127,6 +127,19 @@ def test_msm_closed_form_large(zctx, log_n, skewed):
     scal.free()
 
 
+def test_msm_2_24_closed_form(zctx):
+    # the largest shape here: three-level scans in the bucket sort
+    n = 1 << 24
+    ks = workload.fr_uniform(0x3102, n)
+    ss = workload.fr_uniform(0x3101, n)
+    bases = workload.device_bases(zctx, ks)
+    scal = workload.device_fr_mont(zctx, ss)
+    expect = _closed(ks, ss)
+    assert _msm(zctx, bases, scal, n, 17) == expect
+    bases.free()
+    scal.free()
+
+
 @pytest.mark.parametrize("c", [0, 8, 13])
 def test_fixed_base_batch(zctx, c):
     n = 1 << 12
