@@ -58,6 +58,24 @@ def parse():
     return ap.parse_args()
 
 
+def host_cpu_info() -> dict:
+    """CPU model and the number of cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"model": model, "usable_cores": usable, "os_cpu_count": os.cpu_count()}
+
+
 def cpu_msm_baseline(ctx, bases_ptr: int, scalar_words, m: int):
     """The C restatement of the reference Pippenger (crypto/msm/pippenger.py),
     one thread, on m canonical points / scalars; returns (pts/s, seconds, result)."""
@@ -437,7 +455,7 @@ def main():
 
     import numpy as np
     from pyzkp_amd import get_context, workload
-    from pyzkp_amd.curve import G1_GENERATOR, from_jacobian_mont, g1_eq, g1_mul
+    from pyzkp_amd.curve import G1_GENERATOR, from_jacobian_mont, g1_eq, g1_mul, normalize
     from pyzkp_amd.field import FR_MODULUS, words_to_ints
 
     if args.workload == "copy":
@@ -593,6 +611,22 @@ def main():
                          f"first 2^{m.bit_length() - 1} points/scalars of the same inputs, 1 thread, "
                          f"{cpu_s:.2f} s",
                "gpu_match": gaff == cres}
+        # the same restatement on the host's cores (OpenMP over windows), the full size
+        threads = max(1, min(args.cpu_threads, host_cpu_info()["usable_cores"] or 1))
+        log(f"CPU baseline: C Pippenger, n=2^{log_n}, {threads} threads")
+        bc = ctx.alloc(64 * n)
+        ctx.fq_from_mont(bases.ptr, bc.ptr, 2 * n)
+        bfull = np.frombuffer(ctx.download(bc.ptr, 64 * n), dtype="<u8")
+        bc.free()
+        t0 = time.perf_counter()
+        cfull = cref.msm_pippenger_words(bfull, np.ascontiguousarray(ss).reshape(-1), n, default_window_bits(n),
+                                         threads=threads)
+        mt_s = time.perf_counter() - t0
+        full_aff = normalize(from_jacobian_mont(*res))
+        cpu["all_cores"] = {"value": round(n / mt_s, 1), "unit": "pts/s", "cores": threads,
+                            "sample": f"the same restatement, full 2^{log_n} inputs, {threads} threads, "
+                                      f"{mt_s:.2f} s", "gpu_match": cfull == full_aff}
+        cpu["host"] = host_cpu_info()
 
     line = {
         "metric": "G1 MSM pts/s @2^22; NTT GB/s @2^24; served proofs/s + p99 vs plain loop",
