@@ -535,12 +535,16 @@ def main():
     cands = [args.window_bits] if args.window_bits else list(range(13, 19))
     tune = {}
     for c in cands:
-        ctx.msm_g1(bases.ptr, scal.ptr, n, c)
-        t0 = time.perf_counter()
-        ctx.msm_g1(bases.ptr, scal.ptr, n, c)
-        tune[c] = time.perf_counter() - t0
-        log(f"  c={c}: {tune[c] * 1e3:.2f} ms")
-    c = min(tune, key=tune.get)
+        for k_red in (0, 8, 16):   # buckets per reduction thread (0: the library's rule)
+            ctx.set_tuning("msm_reduce_chunk", k_red)
+            ctx.msm_g1(bases.ptr, scal.ptr, n, c)
+            t0 = time.perf_counter()
+            ctx.msm_g1(bases.ptr, scal.ptr, n, c)
+            ctx.msm_g1(bases.ptr, scal.ptr, n, c)
+            tune[(c, k_red)] = (time.perf_counter() - t0) / 2
+            log(f"  c={c} reduce chunk {k_red}: {tune[(c, k_red)] * 1e3:.2f} ms")
+    c, k_red = min(tune, key=tune.get)
+    ctx.set_tuning("msm_reduce_chunk", k_red)
 
     for _ in range(args.warmup):
         res = ctx.msm_g1(bases.ptr, scal.ptr, n, c)
@@ -635,7 +639,8 @@ def main():
         "vs_baseline": None, "dtype": "u32x8 Montgomery Fq/Fr (254-bit integer)", "data": "synthetic",
         "config": {"workload": f"BN254 G1 MSM n=2^{log_n} {'skewed' if args.skewed else 'uniform'} scalars, "
                                "bases [k_i]G1 affine (device-resident)",
-                   "n": n, "window_bits": c, "autotune_ms": {k: round(v * 1e3, 3) for k, v in tune.items()},
+                   "n": n, "window_bits": c, "reduce_chunk": k_red,
+                   "autotune_ms": {f"c{k[0]}_k{k[1]}": round(v * 1e3, 3) for k, v in tune.items()},
                    "phases_ms": phases, "verified_closed_form": bool(verified)},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -816,8 +816,9 @@ static int make_shape(Ctx* c, int cb, uint32_t nrows, const std::vector<MsmRow>&
   if (sh.Smax == 0) sh.Smax = 1;
   uint32_t K = c->msm_reduce_chunk;
   if (K == 0) {
+    // ~64 K reduction threads, at most 8 buckets each (measured best at 2^22, c = 17)
     const uint64_t per = ((uint64_t)nrows * sh.nbins) / 65536;
-    K = per < 2 ? 2 : (uint32_t)per;
+    K = per < 2 ? 2 : (per > 8 ? 8 : (uint32_t)per);
   }
   if (K > sh.nbins) K = sh.nbins;
   sh.K = K;
