@@ -79,31 +79,45 @@ ZK_HD xyzz9 xyzz9_dbl(const xyzz9& p) {
   return r;
 }
 
-// madd-2008-s: p += a (a affine, not the identity)
+// Additions with lazy reductions.  Bounds, in units of p, for inputs below
+// 2p (p / 2^261 < 0.0060, and a product is below p + a*b / 2^261):
+//   U, S products < 1.03          P = U2 - U1 + 2p, R = S2 - S1 + 2p in (0, 3.03)
+//   PP < 1.06, PPP and Q < 1.02   (P == 0 mod p iff P is p, 2p or 3p)
+//   X3 = R^2 + (4p - PPP - 2Q) < 5.06, then below 2p by two conditional
+//        subtractions (4p, 2p)
+//   Y3 = R (Q - X3 + 2p) + S1 (2p - PPP), one reduction for both products,
+//        < 1.08
+//   ZZ3, ZZZ3 < 1.02
+// so every output coordinate is again below 2p.
+ZK_HD void xyzz9_finish(const f9& R, const f9& ppp, const f9& q, const f9 s1, f9& x3, f9& y3) {
+  x3 = Fq9::sqr_plus(R, Fq9::sub_4p_a_2b(ppp, q));
+  x3 = Fq9::reduce2p(Fq9::sub_if_ge(x3, Fq29Params::P4));
+  y3 = Fq9::mul2(R, Fq9::sub_nr(q, x3), s1, Fq9::neg_nr(ppp));
+}
+
+// madd-2008-s: p += a (a affine, not the identity; a.y may be 2p - y for a
+// negated base)
 ZK_HD void xyzz9_madd(xyzz9& p, const aff9& a) {
   if (xyzz9_is_inf(p)) {
     p = xyzz9_from_aff(a);
     return;
   }
-  f9 u2 = Fq9::mul(a.x, p.zz);
-  f9 s2 = Fq9::mul(a.y, p.zzz);
-  f9 P = Fq9::sub(u2, p.x);
-  f9 R = Fq9::sub(s2, p.y);
-  if (Fq9::is_zero(P)) {
-    if (Fq9::is_zero(R)) {
+  const f9 u2 = Fq9::mul(a.x, p.zz);
+  const f9 s2 = Fq9::mul(a.y, p.zzz);
+  const f9 P = Fq9::sub_nr(u2, p.x);
+  const f9 R = Fq9::sub_nr(s2, p.y);
+  if (Fq9::is_zero_lt4p(P)) {
+    if (Fq9::is_zero_lt4p(R)) {
       p = xyzz9_dbl_aff(a);
     } else {
       p = xyzz9_inf();
     }
     return;
   }
-  f9 pp = Fq9::sqr(P);
-  f9 ppp = Fq9::mul(P, pp);
-  f9 q = Fq9::mul(p.x, pp);
-  f9 x3 = Fq9::sub(Fq9::sub(Fq9::sqr(R), ppp), Fq9::dbl(q));
-  f9 y3 = Fq9::sub(Fq9::mul(R, Fq9::sub(q, x3)), Fq9::mul(p.y, ppp));
-  p.x = x3;
-  p.y = y3;
+  const f9 pp = Fq9::sqr(P);
+  const f9 ppp = Fq9::mul(P, pp);
+  const f9 q = Fq9::mul(p.x, pp);
+  xyzz9_finish(R, ppp, q, p.y, p.x, p.y);
   p.zz = Fq9::mul(p.zz, pp);
   p.zzz = Fq9::mul(p.zzz, ppp);
 }
@@ -112,22 +126,21 @@ ZK_HD void xyzz9_madd(xyzz9& p, const aff9& a) {
 ZK_HD xyzz9 xyzz9_add(const xyzz9& p, const xyzz9& q) {
   if (xyzz9_is_inf(p)) return q;
   if (xyzz9_is_inf(q)) return p;
-  f9 u1 = Fq9::mul(p.x, q.zz);
-  f9 u2 = Fq9::mul(q.x, p.zz);
-  f9 s1 = Fq9::mul(p.y, q.zzz);
-  f9 s2 = Fq9::mul(q.y, p.zzz);
-  f9 P = Fq9::sub(u2, u1);
-  f9 R = Fq9::sub(s2, s1);
-  if (Fq9::is_zero(P)) {
-    if (Fq9::is_zero(R)) return xyzz9_dbl(p);
+  const f9 u1 = Fq9::mul(p.x, q.zz);
+  const f9 u2 = Fq9::mul(q.x, p.zz);
+  const f9 s1 = Fq9::mul(p.y, q.zzz);
+  const f9 s2 = Fq9::mul(q.y, p.zzz);
+  const f9 P = Fq9::sub_nr(u2, u1);
+  const f9 R = Fq9::sub_nr(s2, s1);
+  if (Fq9::is_zero_lt4p(P)) {
+    if (Fq9::is_zero_lt4p(R)) return xyzz9_dbl(p);
     return xyzz9_inf();
   }
-  f9 pp = Fq9::sqr(P);
-  f9 ppp = Fq9::mul(P, pp);
-  f9 qq = Fq9::mul(u1, pp);
+  const f9 pp = Fq9::sqr(P);
+  const f9 ppp = Fq9::mul(P, pp);
+  const f9 qq = Fq9::mul(u1, pp);
   xyzz9 r;
-  r.x = Fq9::sub(Fq9::sub(Fq9::sqr(R), ppp), Fq9::dbl(qq));
-  r.y = Fq9::sub(Fq9::mul(R, Fq9::sub(qq, r.x)), Fq9::mul(s1, ppp));
+  xyzz9_finish(R, ppp, qq, s1, r.x, r.y);
   r.zz = Fq9::mul(Fq9::mul(p.zz, q.zz), pp);
   r.zzz = Fq9::mul(Fq9::mul(p.zzz, q.zzz), ppp);
   return r;

@@ -34,6 +34,8 @@ static constexpr u32 M29 = (1u << 29) - 1;
 struct Fq29Params {
   static constexpr u32 P[9] = {0x187cfd47u, 0x010460b6u, 0x1c72a34fu, 0x02d522d0u, 0x1585d978u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
   static constexpr u32 P2[9] = {0x10f9fa8eu, 0x0208c16du, 0x18e5469eu, 0x05aa45a1u, 0x0b0bb2f0u, 0x05b68181u, 0x014dc282u, 0x1cb84c68u, 0x0060c89cu};
+  static constexpr u32 P3[9] = {0x0976f7d5u, 0x030d2224u, 0x1557e9edu, 0x087f6872u, 0x00918c68u, 0x0891c242u, 0x01f4a3c3u, 0x0b14729cu, 0x00912cebu};
+  static constexpr u32 P4[9] = {0x01f3f51cu, 0x041182dbu, 0x11ca8d3cu, 0x0b548b43u, 0x161765e0u, 0x0b6d0302u, 0x029b8504u, 0x197098d0u, 0x00c19139u};
   static constexpr u32 PINV = 0x04866389u;  // -p^-1 mod 2^29
   static constexpr u32 ONE[9] = {0x157ccc21u, 0x141c2758u, 0x185230d3u, 0x014c0419u, 0x0aa36fb9u, 0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};                    // 2^261 mod p
   static constexpr u32 TO261[9] = {0x13349ca1u, 0x1a5d84a8u, 0x0a3e5cacu, 0x100249e0u, 0x12b951e8u, 0x0e92d304u, 0x14cb95b3u, 0x041b9d3du, 0x00058003u};         // 2^266 mod p: x 2^256 -> x 2^261
@@ -42,11 +44,29 @@ struct Fq29Params {
 struct Fr29Params {
   static constexpr u32 P[9] = {0x10000001u, 0x1f0fac9fu, 0x0e5c2450u, 0x07d090f3u, 0x1585d283u, 0x02db40c0u, 0x00a6e141u, 0x0e5c2634u, 0x0030644eu};
   static constexpr u32 P2[9] = {0x00000002u, 0x1e1f593fu, 0x1cb848a1u, 0x0fa121e6u, 0x0b0ba506u, 0x05b68181u, 0x014dc282u, 0x1cb84c68u, 0x0060c89cu};
+  static constexpr u32 P3[9] = {0x10000003u, 0x1d2f05deu, 0x0b146cf2u, 0x1771b2dau, 0x00917789u, 0x0891c242u, 0x01f4a3c3u, 0x0b14729cu, 0x00912cebu};
+  static constexpr u32 P4[9] = {0x00000004u, 0x1c3eb27eu, 0x19709143u, 0x1f4243cdu, 0x16174a0cu, 0x0b6d0302u, 0x029b8504u, 0x197098d0u, 0x00c19139u};
   static constexpr u32 PINV = 0x0fffffffu;  // -p^-1 mod 2^29
   static constexpr u32 ONE[9] = {0x0fffff57u, 0x1ea70ab4u, 0x052c068bu, 0x17504f49u, 0x0aa8075bu, 0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};                    // 2^261 mod p
   static constexpr u32 TO261[9] = {0x0fffead7u, 0x1d5444f4u, 0x04438aa5u, 0x03b4d096u, 0x134c84dau, 0x0e92d304u, 0x14cb95b3u, 0x041b9d3du, 0x00058003u};         // 2^266 mod p: x 2^256 -> x 2^261
   static constexpr u32 TO256[9] = {0x0ffffffbu, 0x04b1a0e2u, 0x18334a6bu, 0x18ed2b3eu, 0x1462e36fu, 0x11b7bc3cu, 0x1cbd99bau, 0x183340fbu, 0x000e0a77u};         // 2^256 mod p: x 2^261 -> x 2^256
 };
+
+// Each multiply-add of a product column is chained through the accumulator:
+// the empty asm makes the running sum opaque, so the compiler keeps one
+// v_mad_u64_u32 per partial product instead of re-associating a column into
+// separate sums that then need 64-bit additions (about 14 fewer
+// instructions per product; the waves of a SIMD hide the chain's latency).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define ZK_OPAQUE(x) asm("" : "+v"(x))
+#else
+#define ZK_OPAQUE(x) (void)0
+#endif
+#define ZK_MAD(acc, x, y)        \
+  do {                           \
+    (acc) += (u64)(x) * (y);     \
+    ZK_OPAQUE(acc);              \
+  } while (0)
 
 template <class F>
 struct Mont29 {
@@ -69,7 +89,27 @@ struct Mont29 {
     return r;
   }
 
-  // FIPS Montgomery product: columns k = 0..16 of a*b + m*p in one u64
+  // Reduction part of FIPS column k: add the m*p terms, then either pick
+  // m_k (k < 9) so the column's low 29 bits vanish, or emit limb k - 9.
+  static ZK_HD void redc_column(u64& acc, u32 (&m)[9], f9& r, int k) {
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const int j = k - i;
+      if (i < k && j >= 0 && j < 9) ZK_MAD(acc, m[i], F::P[j]);
+    }
+    if (k < 9) {
+      m[k] = ((u32)acc * F::PINV) & M29;
+      ZK_MAD(acc, m[k], F::P[0]);
+      acc >>= 29;
+    } else {
+      r.l[k - 9] = (u32)acc & M29;
+      acc >>= 29;
+    }
+  }
+
+  // FIPS Montgomery product: columns k = 0..16 of a*b + m*p in one u64.
+  // Output < p + a*b/R: below 1.03p for inputs below 2p, below 2p for
+  // inputs whose product is below R*p (e.g. 13p x 13p).
   static ZK_HD f9 mul(const f9& a, const f9& b) {
     u32 m[9];
     f9 r;
@@ -79,28 +119,40 @@ struct Mont29 {
 #pragma unroll
       for (int i = 0; i < 9; i++) {
         const int j = k - i;
-        if (j >= 0 && j < 9) acc += (u64)a.l[i] * b.l[j];
+        if (j >= 0 && j < 9) ZK_MAD(acc, a.l[i], b.l[j]);
       }
-#pragma unroll
-      for (int i = 0; i < 9; i++) {
-        const int j = k - i;
-        if (i < k && j >= 0 && j < 9) acc += (u64)m[i] * F::P[j];
-      }
-      if (k < 9) {
-        m[k] = ((u32)acc * F::PINV) & M29;
-        acc += (u64)m[k] * F::P[0];
-        acc >>= 29;
-      } else {
-        r.l[k - 9] = (u32)acc & M29;
-        acc >>= 29;
-      }
+      redc_column(acc, m, r, k);
     }
     r.l[8] = (u32)acc;
     return r;
   }
 
-  // squaring: cross products once with a doubled operand
-  static ZK_HD f9 sqr(const f9& a) {
+  // (a*b + c*d) / R with one reduction: a column holds at most 27 products
+  // of < 2^58 plus a < 2^35 carry, still below 2^64.
+  static ZK_HD f9 mul2(const f9& a, const f9& b, const f9& c, const f9& d) {
+    u32 m[9];
+    f9 r;
+    u64 acc = 0;
+#pragma unroll
+    for (int k = 0; k < 17; k++) {
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        const int j = k - i;
+        if (j >= 0 && j < 9) {
+          ZK_MAD(acc, a.l[i], b.l[j]);
+          ZK_MAD(acc, c.l[i], d.l[j]);
+        }
+      }
+      redc_column(acc, m, r, k);
+    }
+    r.l[8] = (u32)acc;
+    return r;
+  }
+
+  // squaring: cross products once with a doubled operand; v (normalised
+  // limbs) is added to the result through the high columns, so the output
+  // is a^2 / R + v with no separate addition pass
+  static ZK_HD f9 sqr_plus(const f9& a, const f9& v) {
     u32 a2[9];
 #pragma unroll
     for (int i = 0; i < 9; i++) a2[i] = a.l[i] << 1;
@@ -112,34 +164,24 @@ struct Mont29 {
 #pragma unroll
       for (int i = 0; i < 9; i++) {
         const int j = k - i;
-        if (j > i && j < 9) acc += (u64)a2[i] * a.l[j];
+        if (j > i && j < 9) ZK_MAD(acc, a2[i], a.l[j]);
       }
-      if ((k & 1) == 0 && k / 2 < 9) acc += (u64)a.l[k / 2] * a.l[k / 2];
-#pragma unroll
-      for (int i = 0; i < 9; i++) {
-        const int j = k - i;
-        if (i < k && j >= 0 && j < 9) acc += (u64)m[i] * F::P[j];
-      }
-      if (k < 9) {
-        m[k] = ((u32)acc * F::PINV) & M29;
-        acc += (u64)m[k] * F::P[0];
-        acc >>= 29;
-      } else {
-        r.l[k - 9] = (u32)acc & M29;
-        acc >>= 29;
-      }
+      if ((k & 1) == 0 && k / 2 < 9) ZK_MAD(acc, a.l[k / 2], a.l[k / 2]);
+      if (k >= 9) acc += v.l[k - 9];
+      redc_column(acc, m, r, k);
     }
-    r.l[8] = (u32)acc;
+    r.l[8] = (u32)acc + v.l[8];
     return r;
   }
+  static ZK_HD f9 sqr(const f9& a) { return sqr_plus(a, zero()); }
 
-  // s -> s - 2p if s >= 2p  (s < 4p, limbs normalised)
-  static ZK_HD f9 reduce2p(const f9& s) {
+  // s - K if s >= K else s (K normalised limbs; s - K must fit below 2^261)
+  static ZK_HD f9 sub_if_ge(const f9& s, const u32 (&K)[9]) {
     f9 d;
     int32_t br = 0;
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-      const int32_t t = (int32_t)s.l[i] - (int32_t)F::P2[i] + br;
+      const int32_t t = (int32_t)s.l[i] - (int32_t)K[i] + br;
       d.l[i] = (u32)t & M29;
       br = t >> 29;
     }
@@ -148,6 +190,48 @@ struct Mont29 {
     for (int i = 0; i < 9; i++) r.l[i] = br ? s.l[i] : d.l[i];
     return r;
   }
+
+  // 2p - a for a <= 2p, no reduction: result in [0, 2p]
+  static ZK_HD f9 neg_nr(const f9& a) {
+    f9 s;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const int32_t t = (int32_t)F::P2[i] - (int32_t)a.l[i] + c;
+      s.l[i] = (u32)t & M29;
+      c = t >> 29;
+    }
+    return s;
+  }
+
+  // 4p - a - 2b, no reduction; needs a + 2b <= 4p
+  static ZK_HD f9 sub_4p_a_2b(const f9& a, const f9& b) {
+    f9 s;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const int32_t t = (int32_t)F::P4[i] - (int32_t)a.l[i] - (int32_t)(b.l[i] << 1) + c;
+      s.l[i] = (u32)t & M29;
+      c = t >> 29;
+    }
+    return s;
+  }
+
+  // a in [0, 4p): a == 0 mod p  <=>  a is 0, p, 2p or 3p
+  static ZK_HD bool is_zero_lt4p(const f9& a) {
+    u32 z = 0, e1 = 0, e2 = 0, e3 = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      z |= a.l[i];
+      e1 |= a.l[i] ^ F::P[i];
+      e2 |= a.l[i] ^ F::P2[i];
+      e3 |= a.l[i] ^ F::P3[i];
+    }
+    return z == 0 || e1 == 0 || e2 == 0 || e3 == 0;
+  }
+
+  // s -> s - 2p if s >= 2p  (s < 4p, limbs normalised)
+  static ZK_HD f9 reduce2p(const f9& s) { return sub_if_ge(s, F::P2); }
 
   static ZK_HD f9 add(const f9& a, const f9& b) {
     f9 s;

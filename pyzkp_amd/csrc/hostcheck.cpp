@@ -156,6 +156,66 @@ int zkhost_xyzz29(int op, const uint64_t* p, const uint64_t* q, uint32_t k, uint
   return 0;
 }
 
+// Lazy-reduction primitives (4 inputs of 4 words each; unused ones may be null):
+// op 0 mul2(a, b, c, d)  1 sqr_plus(a, b)  2 sub_nr(a, b)  3 neg_nr(a)
+//    4 sub_4p_a_2b(a, b)  5 sub_if_ge(a, 4p)  6 is_zero_lt4p(a)
+// Outputs are 9 x u32 limbs per value (results may exceed 2^256).
+int zkhost_field29_lazy(int field, int op, const uint64_t* a, const uint64_t* b, const uint64_t* c,
+                        const uint64_t* d, uint32_t* out, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++) {
+    const f9 x = unpack9(rd(a + 4 * i));
+    const f9 y = b ? unpack9(rd(b + 4 * i)) : Fr9::zero();
+    const f9 z = c ? unpack9(rd(c + 4 * i)) : Fr9::zero();
+    const f9 w = d ? unpack9(rd(d + 4 * i)) : Fr9::zero();
+    f9 r;
+    if (field == 0) {
+      switch (op) {
+        case 0: r = Fr9::mul2(x, y, z, w); break;
+        case 1: r = Fr9::sqr_plus(x, y); break;
+        case 2: r = Fr9::sub_nr(x, y); break;
+        case 3: r = Fr9::neg_nr(x); break;
+        case 4: r = Fr9::sub_4p_a_2b(x, y); break;
+        case 5: r = Fr9::sub_if_ge(x, Fr29Params::P4); break;
+        case 6: r = Fr9::zero(); r.l[0] = Fr9::is_zero_lt4p(x); break;
+        default: return 1;
+      }
+    } else {
+      switch (op) {
+        case 0: r = Fq9::mul2(x, y, z, w); break;
+        case 1: r = Fq9::sqr_plus(x, y); break;
+        case 2: r = Fq9::sub_nr(x, y); break;
+        case 3: r = Fq9::neg_nr(x); break;
+        case 4: r = Fq9::sub_4p_a_2b(x, y); break;
+        case 5: r = Fq9::sub_if_ge(x, Fq29Params::P4); break;
+        case 6: r = Fq9::zero(); r.l[0] = Fq9::is_zero_lt4p(x); break;
+        default: return 1;
+      }
+    }
+    for (int j = 0; j < 9; j++) out[9 * i + j] = r.l[j];
+  }
+  return 0;
+}
+
+// The bucket-accumulation step of the MSM: p += each affine base in turn
+// (R = 2^261 values, 8 words per base), the base negated as 2p - y where
+// neg[i] is set; 16 words of XYZZ out after every step.
+int zkhost_madd_chain(const uint64_t* p16, const uint64_t* affs, const uint8_t* neg, uint32_t k, uint64_t* out) {
+  xyzz9 P;
+  P.x = unpack9(rd(p16)); P.y = unpack9(rd(p16 + 4)); P.zz = unpack9(rd(p16 + 8)); P.zzz = unpack9(rd(p16 + 12));
+  for (uint32_t i = 0; i < k; i++) {
+    aff9 a;
+    a.x = unpack9(rd(affs + 8 * i));
+    a.y = unpack9(rd(affs + 8 * i + 4));
+    if (!aff9_is_inf(a)) {
+      if (neg[i]) a.y = Fq9::neg_nr(a.y);
+      xyzz9_madd(P, a);
+    }
+    uint64_t* o = out + 16 * i;
+    wr(o, pack9(P.x)); wr(o + 4, pack9(P.y)); wr(o + 8, pack9(P.zz)); wr(o + 12, pack9(P.zzz));
+  }
+  return 0;
+}
+
 // wire affine (R = 2^256) -> limb form -> XYZZ (R = 2^256, canonical) round trip
 int zkhost_wire_roundtrip(const uint64_t* xy, uint64_t* out16) {
   aff9 a = aff9_from_wire(rd(xy), rd(xy + 4));

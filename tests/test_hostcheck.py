@@ -121,6 +121,9 @@ def host29(host):
     host.zkhost_field29.argtypes = [ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint64]
     host.zkhost_xyzz29.argtypes = [ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_uint32, ct.c_void_p]
     host.zkhost_wire_roundtrip.argtypes = [ct.c_void_p, ct.c_void_p]
+    host.zkhost_field29_lazy.argtypes = [ct.c_int, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                         ct.c_void_p, ct.c_uint64]
+    host.zkhost_madd_chain.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_uint32, ct.c_void_p]
     return host
 
 
@@ -221,3 +224,106 @@ def test_wire_roundtrip(host29):
     out = (ct.c_uint64 * 16)()
     host29.zkhost_wire_roundtrip(_arr([0, 0]), out)
     assert _vals(out, 4)[2] == 0
+
+
+# ------------------------------------------------- lazy reductions (29-bit limbs)
+def _lazy(host, field, op, n, *ins):
+    args = [_arr(v) if v is not None else None for v in ins] + [None] * (4 - len(ins))
+    out = (ct.c_uint32 * (9 * n))()
+    assert host.zkhost_field29_lazy(field, op, *args, out, n) == 0
+    vals = []
+    for i in range(n):
+        limbs = out[9 * i:9 * i + 9]
+        assert all(l < (1 << 29) for l in limbs[:8])
+        vals.append(sum(l << (29 * j) for j, l in enumerate(limbs)))
+    return vals
+
+
+@pytest.mark.parametrize("field,p", [(0, R_MOD), (1, Q_MOD)])
+def test_field29_lazy_primitives(host29, field, p):
+    """Exact value ranges of the lazily reduced forms the point formulas
+    rely on, at and near the ends of each stated input range."""
+    rng = random.Random(300 + field)
+    n = 400
+    top = (1 << 256) - 1
+
+    def pick(hi, extra):
+        return [rng.randrange(hi) for _ in range(n - len(extra))] + extra
+
+    rinv = pow(R261, -1, p)
+    # mul2: any inputs below 2^256 (column sums stay below 2^64), output
+    # below p + (ab + cd) / 2^261
+    a = pick(1 << 256, [top, top, 0, 3 * p])
+    b = pick(1 << 256, [top, 1, 0, 3 * p])
+    c = pick(1 << 256, [top, top, top, 2 * p])
+    d = pick(1 << 256, [top, 0, top, 2 * p])
+    for v, x, y, z, w in zip(_lazy(host29, field, 0, n, a, b, c, d), a, b, c, d):
+        assert v % p == (x * y + z * w) * rinv % p
+        assert v * R261 < p * R261 + x * y + z * w
+    # sqr_plus: a^2 / 2^261 + v
+    a = pick(4 * p, [4 * p - 1, 0, 3 * p + 5])
+    v_ = pick(4 * p, [4 * p - 1, 4 * p - 1, 0])
+    for r, x, y in zip(_lazy(host29, field, 1, n, a, v_), a, v_):
+        assert r % p == (x * x * rinv + y) % p
+        assert (r - y) * R261 < p * R261 + x * x
+    # sub_nr: a - b + 2p for b <= 2p
+    a = pick(4 * p, [0, 0, 4 * p - 1])
+    b = pick(2 * p + 1, [2 * p, 0, 2 * p])
+    assert _lazy(host29, field, 2, n, a, b) == [x - y + 2 * p for x, y in zip(a, b)]
+    # neg_nr: 2p - a for a <= 2p
+    a = pick(2 * p + 1, [0, 2 * p, p])
+    assert _lazy(host29, field, 3, n, a) == [2 * p - x for x in a]
+    # 4p - a - 2b for a + 2b <= 4p
+    a = [rng.randrange(2 * p) for _ in range(n)]
+    b = [rng.randrange((4 * p - x) // 2 + 1) for x in a]
+    a[:3], b[:3] = [0, 2 * p - 2, 4 * p], [2 * p, 1, 0]
+    assert _lazy(host29, field, 4, n, a, b) == [4 * p - x - 2 * y for x, y in zip(a, b)]
+    # sub_if_ge(a, 4p)
+    a = pick(1 << 256, [4 * p, 4 * p - 1, 0, top])
+    assert _lazy(host29, field, 5, n, a) == [x - 4 * p if x >= 4 * p else x for x in a]
+    # is_zero_lt4p
+    a = pick(4 * p, [0, p, 2 * p, 3 * p, p - 1, p + 1, 3 * p + 1, 4 * p - 1])
+    assert _lazy(host29, field, 6, n, a) == [1 if x % p == 0 else 0 for x in a]
+
+
+def _madd_chain(host, start, bases, negs):
+    k = len(bases)
+    flat = []
+    for b in bases:
+        flat += b
+    out = (ct.c_uint64 * (16 * k))()
+    neg = (ct.c_uint8 * k)(*[1 if x else 0 for x in negs])
+    assert host.zkhost_madd_chain(_arr(start), _arr(flat), neg, k, out) == 0
+    return [_vals(out, 4 * k)[4 * i:4 * i + 4] for i in range(k)]
+
+
+def test_madd_chain_lazy(host29):
+    """Long chains of the bucket-accumulation step: distinct bases, repeats
+    (the doubling case), negations (the identity case), restarts from the
+    identity; every intermediate point equals the oracle's and every
+    coordinate stays below 2p, so the lazy bounds hold step after step."""
+    rng = random.Random(909)
+    G = C.G1_GENERATOR
+    pts = [C.g1_mul_affine(G, rng.randrange(1, R_MOD)) for _ in range(12)]
+    seq, negs = [], []
+    for i in range(160):
+        r = rng.random()
+        if r < 0.15 and seq:
+            seq.append(seq[-1])           # same base again
+            negs.append(negs[-1])
+        elif r < 0.25 and seq:
+            seq.append(seq[-1])           # its negation
+            negs.append(not negs[-1])
+        else:
+            seq.append(rng.randrange(len(pts)))
+            negs.append(rng.random() < 0.5)
+    bases = []
+    for j in seq:
+        x, y = pts[j]
+        bases.append([_lazy261(x, rng), _lazy261(y, rng)])
+    acc = None
+    outs = _madd_chain(host29, _pt29(None, 1, rng), bases, negs)
+    for j, ng, got in zip(seq, negs, outs):
+        pt = C.aff_neg(pts[j]) if ng else pts[j]
+        acc = C.aff_add(acc, pt)
+        assert _aff29(got) == acc
