@@ -217,6 +217,21 @@ struct Mont29 {
     return s;
   }
 
+  // x - q p with q = floor(x_8 / (p_8 + 1)), for any x below 2^261 with
+  // normalised limbs: q p <= x, and the result is below p + p / 2^20
+  static ZK_HD f9 reduce_est(const f9& x) {
+    const u32 q = x.l[8] / (F::P[8] + 1);
+    f9 r;
+    int64_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const int64_t t = (int64_t)x.l[i] - (int64_t)((u64)q * F::P[i]) + c;
+      r.l[i] = (u32)t & M29;
+      c = t >> 29;
+    }
+    return r;
+  }
+
   // a in [0, 4p): a == 0 mod p  <=>  a is 0, p, 2p or 3p
   static ZK_HD bool is_zero_lt4p(const f9& a) {
     u32 z = 0, e1 = 0, e2 = 0, e3 = 0;

@@ -158,12 +158,18 @@ int zkhost_xyzz29(int op, const uint64_t* p, const uint64_t* q, uint32_t k, uint
 
 // Lazy-reduction primitives (4 inputs of 4 words each; unused ones may be null):
 // op 0 mul2(a, b, c, d)  1 sqr_plus(a, b)  2 sub_nr(a, b)  3 neg_nr(a)
-//    4 sub_4p_a_2b(a, b)  5 sub_if_ge(a, 4p)  6 is_zero_lt4p(a)
-// Outputs are 9 x u32 limbs per value (results may exceed 2^256).
+//    4 sub_4p_a_2b(a, b)  5 sub_if_ge(a, 4p)  6 is_zero_lt4p(a)  7 reduce_est(a)
+// Inputs of op 7 are 9 x u32 limbs (a below 2^261).  Outputs are 9 x u32 limbs per value (results may exceed 2^256).
 int zkhost_field29_lazy(int field, int op, const uint64_t* a, const uint64_t* b, const uint64_t* c,
                         const uint64_t* d, uint32_t* out, uint64_t n) {
   for (uint64_t i = 0; i < n; i++) {
-    const f9 x = unpack9(rd(a + 4 * i));
+    f9 x;
+    if (op == 7) {
+      const uint32_t* l = reinterpret_cast<const uint32_t*>(a) + 9 * i;
+      for (int j = 0; j < 9; j++) x.l[j] = l[j];
+    } else {
+      x = unpack9(rd(a + 4 * i));
+    }
     const f9 y = b ? unpack9(rd(b + 4 * i)) : Fr9::zero();
     const f9 z = c ? unpack9(rd(c + 4 * i)) : Fr9::zero();
     const f9 w = d ? unpack9(rd(d + 4 * i)) : Fr9::zero();
@@ -177,6 +183,7 @@ int zkhost_field29_lazy(int field, int op, const uint64_t* a, const uint64_t* b,
         case 4: r = Fr9::sub_4p_a_2b(x, y); break;
         case 5: r = Fr9::sub_if_ge(x, Fr29Params::P4); break;
         case 6: r = Fr9::zero(); r.l[0] = Fr9::is_zero_lt4p(x); break;
+        case 7: r = Fr9::reduce_est(x); break;
         default: return 1;
       }
     } else {
@@ -188,6 +195,7 @@ int zkhost_field29_lazy(int field, int op, const uint64_t* a, const uint64_t* b,
         case 4: r = Fq9::sub_4p_a_2b(x, y); break;
         case 5: r = Fq9::sub_if_ge(x, Fq29Params::P4); break;
         case 6: r = Fq9::zero(); r.l[0] = Fq9::is_zero_lt4p(x); break;
+        case 7: r = Fq9::reduce_est(x); break;
         default: return 1;
       }
     }

@@ -25,10 +25,11 @@
 // buffers hold R = 2^256 Montgomery values; since the transform is linear,
 // reading them as R = 2^261 values scales every element by the same 2^-5,
 // which the output carries back out.  Twiddles and scaling tables are kept
-// in R = 2^261 form (packed 32 bytes per entry), intermediate passes store
-// lazily reduced values (< 4p: a butterfly reduces only its first input,
-// then forms u + w v and u - w v + 2p without a final subtraction), and the
-// last pass stores canonical values.
+// in R = 2^261 form (packed 32 bytes per entry).  Inside a pass values are
+// not reduced at all (a stage adds at most 2p to their bound, and 9 x 29
+// bits leave room for 24p); a pass ends with one quotient-estimate
+// reduction per element (below 1.0001p), and the last pass stores canonical
+// values.
 #include "internal.hpp"
 #include "field29.hpp"
 
@@ -91,11 +92,32 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
   }
   __syncthreads();
 
-  // ---- k radix-2 Stockham stages in LDS
+  // ---- k radix-2 Stockham stages in LDS.  Values grow lazily (bounds in
+  //      units of p): a stage forms u + w v and u - w v + 2p with w v < 1.2p,
+  //      so it adds at most 2 to the bound, from < 4 after the load to < 24
+  //      after 10 stages, inside the 2^261 the limbs hold; the product takes
+  //      such inputs (its output is below p + v w / 2^261).  Global stage 0
+  //      has twiddle 1 and reduces v to [0, 2p) instead.
   const uint32_t halfRT = RT >> 1;
+  // twiddle of butterfly b at pass stage t; loaded one stage ahead so the
+  // gather overlaps the current stage's products
+  auto tw_at = [&](uint32_t b, uint32_t t) -> f9 {
+    const uint32_t c = b & (T - 1), q = b >> log_t;
+    const uint32_t d = q & ((1u << t) - 1);
+    const uint64_t low = (j0 + c) & (Ns - 1);
+    return tw9(tw, (low + (uint64_t)d * Ns) << (log_n - 1 - (s0 + t)));
+  };
+  f9 twc[NTT_BFLY_PER_THREAD];
+  if (s0 != 0) {
+#pragma unroll
+    for (int i = 0; i < NTT_BFLY_PER_THREAD; i++) {
+      const uint32_t b = tid + i * NTT_THREADS;
+      if (b < halfRT) twc[i] = tw_at(b, 0);
+    }
+  }
   for (uint32_t t = 0; t < k; t++) {
     const uint32_t s = s0 + t;
-    f9 u[NTT_BFLY_PER_THREAD], v[NTT_BFLY_PER_THREAD];
+    f9 u[NTT_BFLY_PER_THREAD], v[NTT_BFLY_PER_THREAD], twn[NTT_BFLY_PER_THREAD];
 #pragma unroll
     for (int i = 0; i < NTT_BFLY_PER_THREAD; i++) {
       const uint32_t b = tid + i * NTT_THREADS;
@@ -103,6 +125,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         const uint32_t c = b & (T - 1), q = b >> log_t;
         u[i] = lds[(q << log_t) + c];
         v[i] = lds[((q + (R >> 1)) << log_t) + c];
+        if (t + 1 < k) twn[i] = tw_at(b, t + 1);
       }
     }
     __syncthreads();
@@ -112,24 +135,20 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
       if (b < halfRT) {
         const uint32_t c = b & (T - 1), q = b >> log_t;
         const uint32_t d = q & ((1u << t) - 1);
-        f9 vv;
-        if (s != 0) {
-          const uint64_t low = (j0 + c) & (Ns - 1);
-          const uint64_t e = (low + (uint64_t)d * Ns) << (log_n - 1 - s);
-          vv = Fr9::mul(v[i], tw9(tw, e));  // v < 4p is a valid input: the product is < 2p
-        } else {
-          vv = Fr9::reduce2p(v[i]);
-        }
-        const f9 uu = Fr9::reduce2p(u[i]);
+        const f9 vv = (s != 0) ? Fr9::mul(v[i], twc[i]) : Fr9::reduce2p(v[i]);
         const uint32_t o = ((q - d) << 1) + d;
-        lds[(o << log_t) + c] = Fr9::add_nr(uu, vv);
-        lds[((o + (1u << t)) << log_t) + c] = Fr9::sub_nr(uu, vv);
+        lds[(o << log_t) + c] = Fr9::add_nr(u[i], vv);
+        lds[((o + (1u << t)) << log_t) + c] = Fr9::sub_nr(u[i], vv);
       }
+      twc[i] = twn[i];
     }
     __syncthreads();
   }
 
-  // ---- store: element (sp, c) -> out[(j / Ns) * Ns * R + (j % Ns) + sp * Ns], j = j0 + c
+  // ---- store: element (sp, c) -> out[(j / Ns) * Ns * R + (j % Ns) + sp * Ns], j = j0 + c.
+  //      Values leave below 2p: through the scaling product (< 1.3p) or the
+  //      quotient-estimate reduction (< 1.0001p); the last pass makes them
+  //      canonical.
   const f9 post_c = (post_mode == SCALE_CONST) ? tab9(post_hi, 0) : Fr9::zero();
   for (uint32_t e = tid; e < RT; e += NTT_THREADS) {
     uint32_t sp, c;
@@ -147,8 +166,10 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
       x = Fr9::mul(x, post_c);
     } else if (post_mode == SCALE_TABLE) {
       x = Fr9::mul(x, pow_lookup(post_hi, post_lo, pos));
+    } else {
+      x = Fr9::reduce_est(x);
     }
-    if (last) x = Fr9::canonical(Fr9::reduce2p(x));
+    if (last) x = Fr9::canonical(x);
     store_fe(out + pos, pack9(x));
   }
 }

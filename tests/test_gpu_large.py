@@ -43,6 +43,41 @@ def test_ntt_2_20_bit_exact(zctx, variant):
     dst.free()
 
 
+@pytest.mark.parametrize("pass_bits", [0, 10])
+def test_ntt_2_20_extreme_inputs(zctx, pass_bits):
+    """Buffers may hold any 256-bit Montgomery word, canonical or not.  The
+    largest ones (2^256 - 1, r - 1, v + r) with passes of ten stages give the
+    largest intermediate values the lazily reduced butterflies see."""
+    log_n = 20
+    n = 1 << log_n
+    rng = np.random.default_rng(0x2003)
+    table = [(1 << 256) - 1, R_MOD - 1]
+    for j in range(4094):
+        v = int.from_bytes(rng.bytes(32), "little") % R_MOD
+        table.append(v + R_MOD if j % 2 == 0 and v + R_MOD < (1 << 256) else v)
+    rinv = pow(1 << 256, -1, R_MOD)
+    mask = (1 << 64) - 1
+    raw_t = np.array([[v >> (64 * j) & mask for j in range(4)] for v in table], dtype=np.uint64)
+    plain_t = np.array([[(v * rinv % R_MOD) >> (64 * j) & mask for j in range(4)] for v in table], dtype=np.uint64)
+    idx = rng.integers(0, len(table), n)
+    idx[0::4], idx[1::4] = 0, 1      # a quarter each of the two largest words
+    raw = raw_t[idx].tobytes()
+    plain = plain_t[idx].reshape(-1)
+    src = zctx.upload(raw)
+    dst = zctx.alloc(32 * n)
+    zctx.set_tuning("ntt_max_pass_bits", pass_bits)
+    try:
+        for inv, shift in ((False, None), (True, FR_GENERATOR)):
+            zctx.ntt(src.ptr, n, dst.ptr, log_n, inv, shift_mont=fr_to_mont(shift) if shift else None)
+            got = _dev_words(zctx, dst.ptr, n).reshape(-1)
+            expect = cref.ntt_words(plain, n, log_n, N.omega_for_size(n), inv, shift, threads=0)
+            assert np.array_equal(got, expect), (inv, pass_bits)
+    finally:
+        zctx.set_tuning("ntt_max_pass_bits", 0)
+    src.free()
+    dst.free()
+
+
 def test_ntt_2_24_roundtrip_and_rows(zctx):
     log_n = 24
     n = 1 << log_n

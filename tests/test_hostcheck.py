@@ -284,6 +284,17 @@ def test_field29_lazy_primitives(host29, field, p):
     # is_zero_lt4p
     a = pick(4 * p, [0, p, 2 * p, 3 * p, p - 1, p + 1, 3 * p + 1, 4 * p - 1])
     assert _lazy(host29, field, 6, n, a) == [1 if x % p == 0 else 0 for x in a]
+    # reduce_est: any value below 2^261 (as 9 limbs) to below p + p / 2^20
+    a = pick(1 << 261, [(1 << 261) - 1, 0, p - 1, p, 24 * p - 1, 24 * p, 2 * p - 1, (1 << 256) - 1])
+    limbs = (ct.c_uint32 * (9 * n))()
+    for i, x in enumerate(a):
+        for j in range(9):
+            limbs[9 * i + j] = (x >> (29 * j)) & ((1 << 29) - 1)
+    out = (ct.c_uint32 * (9 * n))()
+    assert host29.zkhost_field29_lazy(field, 7, limbs, None, None, None, out, n) == 0
+    for i, x in enumerate(a):
+        v = sum(out[9 * i + j] << (29 * j) for j in range(9))
+        assert v % p == x % p and v < p + (p >> 20)
 
 
 def _madd_chain(host, start, bases, negs):
