@@ -95,11 +95,13 @@ ZK_HD void xyzz9_finish(const f9& R, const f9& ppp, const f9& q, const f9 s1, f9
   y3 = Fq9::mul2(R, Fq9::sub_nr(q, x3), s1, Fq9::neg_nr(ppp));
 }
 
-// madd-2008-s: p += a (a affine, not the identity; a.y may be 2p - y for a
-// negated base)
-ZK_HD void xyzz9_madd(xyzz9& p, const aff9& a) {
-  if (xyzz9_is_inf(p)) {
+// madd-2008-s: p += a (a.y may be 2p - y for a negated base), with the
+// caller keeping p_inf == (p is the identity) up to date, so the bucket
+// accumulation does not test ZZ at every step
+ZK_HD void xyzz9_madd_t(xyzz9& p, bool& p_inf, const aff9& a) {
+  if (p_inf) {
     p = xyzz9_from_aff(a);
+    p_inf = aff9_is_inf(a);
     return;
   }
   const f9 u2 = Fq9::mul(a.x, p.zz);
@@ -111,6 +113,7 @@ ZK_HD void xyzz9_madd(xyzz9& p, const aff9& a) {
       p = xyzz9_dbl_aff(a);
     } else {
       p = xyzz9_inf();
+      p_inf = true;
     }
     return;
   }
@@ -120,6 +123,12 @@ ZK_HD void xyzz9_madd(xyzz9& p, const aff9& a) {
   xyzz9_finish(R, ppp, q, p.y, p.x, p.y);
   p.zz = Fq9::mul(p.zz, pp);
   p.zzz = Fq9::mul(p.zzz, ppp);
+}
+
+// madd-2008-s: p += a (a affine, not the identity)
+ZK_HD void xyzz9_madd(xyzz9& p, const aff9& a) {
+  bool inf = xyzz9_is_inf(p);
+  xyzz9_madd_t(p, inf, a);
 }
 
 // add-2008-s

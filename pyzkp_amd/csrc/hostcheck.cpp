@@ -210,14 +210,16 @@ int zkhost_field29_lazy(int field, int op, const uint64_t* a, const uint64_t* b,
 int zkhost_madd_chain(const uint64_t* p16, const uint64_t* affs, const uint8_t* neg, uint32_t k, uint64_t* out) {
   xyzz9 P;
   P.x = unpack9(rd(p16)); P.y = unpack9(rd(p16 + 4)); P.zz = unpack9(rd(p16 + 8)); P.zzz = unpack9(rd(p16 + 12));
+  bool inf = xyzz9_is_inf(P);  // tracked as the accumulation kernel tracks it
   for (uint32_t i = 0; i < k; i++) {
     aff9 a;
     a.x = unpack9(rd(affs + 8 * i));
     a.y = unpack9(rd(affs + 8 * i + 4));
     if (!aff9_is_inf(a)) {
       if (neg[i]) a.y = Fq9::neg_nr(a.y);
-      xyzz9_madd(P, a);
+      xyzz9_madd_t(P, inf, a);
     }
+    if (inf != xyzz9_is_inf(P)) return 2;
     uint64_t* o = out + 16 * i;
     wr(o, pack9(P.x)); wr(o + 4, pack9(P.y)); wr(o + 8, pack9(P.zz)); wr(o + 12, pack9(P.zzz));
   }

@@ -51,22 +51,45 @@ __device__ __forceinline__ aff9 load_aff9(const fe* __restrict__ bases, uint32_t
   return a;
 }
 
-__device__ __forceinline__ xyzz9 load_xyzz9(const xyzz* p) {
-  const fe* f = reinterpret_cast<const fe*>(p);
+// Buckets, head partials and reduction partials stay in limb form: 4 x 9
+// limbs (144 bytes, nine 16-byte words), so the hot loop's bucket stores
+// and the fix-up / reduction loads need no packing.
+struct __align__(16) pt9 {
+  uint4 q[9];
+};
+
+__device__ __forceinline__ xyzz9 load_xyzz9(const pt9* p) {
+  uint32_t w[36];
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const uint4 v = p->q[i];
+    w[4 * i] = v.x;
+    w[4 * i + 1] = v.y;
+    w[4 * i + 2] = v.z;
+    w[4 * i + 3] = v.w;
+  }
   xyzz9 r;
-  r.x = unpack9(load_fe(f));
-  r.y = unpack9(load_fe(f + 1));
-  r.zz = unpack9(load_fe(f + 2));
-  r.zzz = unpack9(load_fe(f + 3));
+#pragma unroll
+  for (int j = 0; j < 9; j++) {
+    r.x.l[j] = w[j];
+    r.y.l[j] = w[9 + j];
+    r.zz.l[j] = w[18 + j];
+    r.zzz.l[j] = w[27 + j];
+  }
   return r;
 }
 
-__device__ __forceinline__ void store_xyzz9(xyzz* p, const xyzz9& v) {
-  fe* f = reinterpret_cast<fe*>(p);
-  store_fe(f, pack9(v.x));
-  store_fe(f + 1, pack9(v.y));
-  store_fe(f + 2, pack9(v.zz));
-  store_fe(f + 3, pack9(v.zzz));
+__device__ __forceinline__ void store_xyzz9(pt9* p, const xyzz9& v) {
+  uint32_t w[36];
+#pragma unroll
+  for (int j = 0; j < 9; j++) {
+    w[j] = v.x.l[j];
+    w[9 + j] = v.y.l[j];
+    w[18 + j] = v.zz.l[j];
+    w[27 + j] = v.zzz.l[j];
+  }
+#pragma unroll
+  for (int i = 0; i < 9; i++) p->q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
 }
 
 // wire bases (affine, R = 2^256, x = y = 0 for infinity) -> R = 2^261 form
@@ -80,7 +103,7 @@ __global__ __launch_bounds__(256) void msm_bases_convert_kernel(const fe* __rest
 }
 
 // window sums: limb form -> R = 2^256 canonical XYZZ for the host combination
-__global__ void msm_wsum_export_kernel(const xyzz* __restrict__ in, uint32_t W, xyzz* __restrict__ out) {
+__global__ void msm_wsum_export_kernel(const pt9* __restrict__ in, uint32_t W, xyzz* __restrict__ out) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= W) return;
   const xyzz9 p = load_xyzz9(in + w);
@@ -503,8 +526,8 @@ __global__ __launch_bounds__(FINE_STAGE_THREADS) void msm_fine_scatter_staged_ke
 //    in the middle of is its "head" partial, added in by the fix-up pass.
 __global__ __launch_bounds__(256) void msm_accumulate_kernel(
     const fe* __restrict__ bases, const uint32_t* __restrict__ sorted,
-    const uint32_t* __restrict__ row_ptr, uint32_t nkeys, uint32_t L, xyzz* __restrict__ buckets,
-    uint32_t* __restrict__ head_key, xyzz* __restrict__ head) {
+    const uint32_t* __restrict__ row_ptr, uint32_t nkeys, uint32_t L, pt9* __restrict__ buckets,
+    uint32_t* __restrict__ head_key, pt9* __restrict__ head) {
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t M = row_ptr[nkeys];
   const uint64_t start = s * L;
@@ -520,6 +543,7 @@ __global__ __launch_bounds__(256) void msm_accumulate_kernel(
   uint32_t rbeg = row_ptr[k], rend = row_ptr[k + 1];
   uint32_t hk = NO_KEY;
   xyzz9 acc = xyzz9_inf();
+  bool acc_inf = true;
   for (uint64_t p = start; p < end; p++) {
     while (p >= rend) {
       if (rend > rbeg) {
@@ -531,6 +555,7 @@ __global__ __launch_bounds__(256) void msm_accumulate_kernel(
         }
       }
       acc = xyzz9_inf();
+      acc_inf = true;
       k++;
       rbeg = rend;
       rend = row_ptr[k + 1];
@@ -539,7 +564,7 @@ __global__ __launch_bounds__(256) void msm_accumulate_kernel(
     aff9 a = load_aff9(bases, v & ~SIGN);
     if (!aff9_is_inf(a)) {
       if (v & SIGN) a.y = Fq9::neg_nr(a.y);  // 2p - y, at most 2p: a valid input to the addition
-      xyzz9_madd(acc, a);
+      xyzz9_madd_t(acc, acc_inf, a);
     }
   }
   if (rbeg < start) {
@@ -571,7 +596,7 @@ __global__ __launch_bounds__(256) void msm_fixup_round_kernel(const uint32_t* __
                                                               uint32_t nkeys, uint32_t L,
                                                               uint64_t stride,
                                                               const uint32_t* __restrict__ head_key,
-                                                              xyzz* __restrict__ head) {
+                                                              pt9* __restrict__ head) {
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t M = row_ptr[nkeys];
   const uint64_t S = (M + L - 1) / L;
@@ -593,9 +618,9 @@ __global__ __launch_bounds__(256) void msm_fixup_round_kernel(const uint32_t* __
 
 __global__ __launch_bounds__(256) void msm_fixup_final_kernel(const uint32_t* __restrict__ row_ptr,
                                                               uint32_t nkeys, uint32_t L,
-                                                              xyzz* __restrict__ buckets,
+                                                              pt9* __restrict__ buckets,
                                                               const uint32_t* __restrict__ head_key,
-                                                              const xyzz* __restrict__ head) {
+                                                              const pt9* __restrict__ head) {
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t M = row_ptr[nkeys];
   const uint64_t S = (M + L - 1) / L;
@@ -611,11 +636,11 @@ __global__ __launch_bounds__(256) void msm_fixup_final_kernel(const uint32_t* __
 //    [lo, lo + K) with lo = 1 + t K; its weighted sum
 //      Sum_b b B_b = Sum_b (b - lo + 1) B_b + (lo - 1) Sum_b B_b
 //    is a running sum from the top plus one small scalar multiple.
-__global__ __launch_bounds__(128) void msm_reduce_chunks_kernel(const xyzz* __restrict__ buckets,
+__global__ __launch_bounds__(128) void msm_reduce_chunks_kernel(const pt9* __restrict__ buckets,
                                                                 const uint32_t* __restrict__ row_ptr,
                                                                 uint32_t W, uint32_t B, uint32_t K,
                                                                 uint32_t cpw,
-                                                                xyzz* __restrict__ partial) {
+                                                                pt9* __restrict__ partial) {
   const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= W * cpw) return;
   const uint32_t w = gid / cpw, t = gid - w * cpw;
@@ -637,8 +662,8 @@ __global__ __launch_bounds__(128) void msm_reduce_chunks_kernel(const xyzz* __re
 //    the host repeats until one value per window is left
 static constexpr uint32_t RED_GROUP = 64;
 
-__global__ __launch_bounds__(RED_GROUP) void msm_reduce_groups_kernel(const xyzz* __restrict__ in, uint32_t m,
-                                                                     uint32_t G, xyzz* __restrict__ out) {
+__global__ __launch_bounds__(RED_GROUP) void msm_reduce_groups_kernel(const pt9* __restrict__ in, uint32_t m,
+                                                                     uint32_t G, pt9* __restrict__ out) {
   __shared__ xyzz9 sh[RED_GROUP];
   const uint32_t w = blockIdx.y, g = blockIdx.x, tid = threadIdx.x;
   const uint32_t t = g * RED_GROUP + tid;
@@ -773,7 +798,8 @@ struct MsmShape {
 struct MsmBuffers {
   uint32_t *digits, *hist, *items_v, *row_ptr, *sums, *sorted, *hkey, *fc0, *fhist;
   uint16_t* items_f;
-  xyzz *buckets, *head, *partial, *partial2, *rsum, *rexp;
+  pt9 *buckets, *head, *partial, *partial2, *rsum;
+  xyzz* rexp;
   MsmRow* rows;
 };
 
@@ -850,12 +876,12 @@ static size_t layout(const MsmShape& sh, char* base, MsmBuffers* b, uint32_t* di
   t.sums = (uint32_t*)take((size_t)sh.nchunks * 4);
   t.fc0 = (uint32_t*)take(((size_t)sh.nrows * sh.NC + 1) * 4);
   t.fhist = (uint32_t*)take(((size_t)sh.fchunks << sh.S) * 4 + 4);
-  t.buckets = (xyzz*)take((size_t)sh.nkeys * sizeof(xyzz));
+  t.buckets = (pt9*)take((size_t)sh.nkeys * sizeof(pt9));
   t.hkey = (uint32_t*)take(sh.Smax * 4);
-  t.head = (xyzz*)take(sh.Smax * sizeof(xyzz));
-  t.partial = (xyzz*)take((size_t)sh.nrows * sh.cpw * sizeof(xyzz));
-  t.partial2 = (xyzz*)take((size_t)sh.nrows * ((sh.cpw + RED_GROUP - 1) / RED_GROUP) * sizeof(xyzz));
-  t.rsum = (xyzz*)take((size_t)sh.nrows * sizeof(xyzz));
+  t.head = (pt9*)take(sh.Smax * sizeof(pt9));
+  t.partial = (pt9*)take((size_t)sh.nrows * sh.cpw * sizeof(pt9));
+  t.partial2 = (pt9*)take((size_t)sh.nrows * ((sh.cpw + RED_GROUP - 1) / RED_GROUP) * sizeof(pt9));
+  t.rsum = (pt9*)take((size_t)sh.nrows * sizeof(pt9));
   t.rexp = (xyzz*)take((size_t)sh.nrows * sizeof(xyzz));
   t.rows = (MsmRow*)take((size_t)sh.nrows * sizeof(MsmRow));
   if (b) *b = t;
@@ -930,12 +956,12 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
       b.buckets, b.row_ptr, sh.nrows, sh.B, sh.K, sh.cpw, b.partial);
   {
     // chunk partials -> one sum per window, 64 to 1 per level
-    xyzz* src = b.partial;
-    xyzz* spare[2] = {b.partial2, b.partial};
+    pt9* src = b.partial;
+    pt9* spare[2] = {b.partial2, b.partial};
     uint32_t m = sh.cpw;
     for (int lvl = 0;; lvl++) {
       const uint32_t G = (m + RED_GROUP - 1) / RED_GROUP;
-      xyzz* dst = (G == 1) ? b.rsum : spare[lvl & 1];
+      pt9* dst = (G == 1) ? b.rsum : spare[lvl & 1];
       msm_reduce_groups_kernel<<<dim3(G, sh.nrows), RED_GROUP, 0, st>>>(src, m, G, dst);
       if (G == 1) break;
       src = dst;
