@@ -42,8 +42,9 @@ __device__ __forceinline__ void store_xyzz(xyzz* p, const xyzz& v) {
   store_fe(f + 3, v.zzz);
 }
 
-// Hot-loop storage: coordinates are R = 2^261 limb values (field29.hpp)
-// packed back into 32 bytes each, so a base is 64 bytes and a bucket 128.
+// Bases: coordinates are R = 2^261 limb values (field29.hpp) packed back
+// into 32 bytes each, so a base is 64 bytes (half a 128-byte line per
+// gather).
 __device__ __forceinline__ aff9 load_aff9(const fe* __restrict__ bases, uint32_t idx) {
   aff9 a;
   a.x = unpack9(load_fe(bases + 2 * (uint64_t)idx));
