@@ -209,7 +209,7 @@ def run_batch(args, ctx, peak_mads):
     ntt_elems = sum(cnt << lg for lg, cnt in groups.items())
 
     tuner = Tuner(ctx, use_cache=False)
-    c, fb = tuner.fixed_window(bases.ptr, nb, scal.ptr, offs, sizes, cands=range(9, 15))
+    c, fb = tuner.fixed_window(bases.ptr, nb, scal.ptr, offs, sizes, cands=range(11, 18))
     log(f"  fixed-base window c={c}: {tuner.log}")
     plans = {}
     for mode in ("graph", "eager"):

@@ -84,7 +84,7 @@ class Tuner:
 
     # ---- fixed-base batch: table window c (builds one table per candidate)
     def fixed_window(self, bases: int, nb: int, scalars: int, offsets, sizes,
-                     cands: Iterable[int] = range(8, 15)) -> Tuple[int, object]:
+                     cands: Iterable[int] = range(10, 18)) -> Tuple[int, object]:
         ctx = self.ctx
         key = f"fixed_c/nb={nb}/count={len(sizes)}/total={sum(sizes)}"
 
