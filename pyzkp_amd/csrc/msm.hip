@@ -280,7 +280,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void scan_add_kernel(uint32_t* out, c
   }
 }
 
-// c. scatter of one tile into coarse-key order
+// c. scatter of one tile into coarse-key order.  Packed form (PK): the
+//    fine key rides in bits [ps, 31) of the entry (ps = 31 - S, used when
+//    every base index of the launch is below 2^ps), so no separate u16
+//    array is written, staged or read by the fine level.
+template <bool PK>
 __global__ __launch_bounds__(SORT_THREADS) void msm_sort_scatter_kernel(
     const uint32_t* __restrict__ digits, const MsmRow* __restrict__ rows, uint32_t S, uint32_t NC,
     const uint32_t* __restrict__ hoff, uint32_t* __restrict__ items_v, uint16_t* __restrict__ items_f) {
@@ -288,7 +292,7 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_sort_scatter_kernel(
   __shared__ uint32_t cstart[SORT_MAX_COARSE];
   __shared__ uint32_t gbase[SORT_MAX_COARSE];
   __shared__ uint32_t stage_v[SORT_TILE];
-  __shared__ uint16_t stage_f[SORT_TILE];
+  __shared__ uint16_t stage_f[PK ? 1 : SORT_TILE];
   __shared__ uint16_t stage_k[SORT_TILE];
   __shared__ uint32_t scan_sh[SORT_THREADS];
   const MsmRow row = rows[blockIdx.y];
@@ -299,6 +303,7 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_sort_scatter_kernel(
   __syncthreads();
   const uint32_t p0 = tile * SORT_TILE;
   const uint32_t fmask = (1u << S) - 1;
+  const uint32_t ps = 31 - S;
   uint32_t d[SORT_ITEMS], rank[SORT_ITEMS];
 #pragma unroll
   for (uint32_t i = 0; i < SORT_ITEMS; i++) {
@@ -331,15 +336,20 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_sort_scatter_kernel(
     const uint32_t k = (mag - 1) >> S;
     const uint32_t slot = cstart[k] + rank[i];
     const uint32_t wi = p / row.inner;
-    stage_v[slot] = (wi * row.stride + row.boff + (p - wi * row.inner)) | (d[i] & SIGN);
-    stage_f[slot] = (uint16_t)((mag - 1) & fmask);
+    const uint32_t idx = (wi * row.stride + row.boff + (p - wi * row.inner)) | (d[i] & SIGN);
+    if (PK) {
+      stage_v[slot] = idx | (((mag - 1) & fmask) << ps);
+    } else {
+      stage_v[slot] = idx;
+      stage_f[slot] = (uint16_t)((mag - 1) & fmask);
+    }
     stage_k[slot] = (uint16_t)k;
   }
   __syncthreads();
   for (uint32_t slot = tid; slot < total; slot += SORT_THREADS) {
     const uint32_t g = gbase[stage_k[slot]] + slot;
     items_v[g] = stage_v[slot];
-    items_f[g] = stage_f[slot];
+    if (!PK) items_f[g] = stage_f[slot];
   }
 }
 
@@ -391,8 +401,23 @@ __device__ __forceinline__ bool fine_chunk(const uint32_t* __restrict__ fc0, uin
   return true;
 }
 
+// fine key of coarse-ordered entry p (packed form: bits [31 - S, 31) of the entry)
+template <bool PK>
+__device__ __forceinline__ uint32_t fine_key(const uint32_t* __restrict__ items_v,
+                                             const uint16_t* __restrict__ items_f, uint32_t p, uint32_t S) {
+  if (PK) return (items_v[p] >> (31 - S)) & ((1u << S) - 1);
+  return items_f[p];
+}
+// the entry as the accumulation reads it (base index | sign)
+template <bool PK>
+__device__ __forceinline__ uint32_t entry_value(uint32_t v, uint32_t S) {
+  return PK ? (v & ~(((1u << S) - 1) << (31 - S))) : v;
+}
+
+template <bool PK>
 __global__ __launch_bounds__(FINE_THREADS) void msm_fine_count_kernel(
-    const uint16_t* __restrict__ items_f, const uint32_t* __restrict__ hoff, const MsmRow* __restrict__ rows,
+    const uint32_t* __restrict__ items_v, const uint16_t* __restrict__ items_f,
+    const uint32_t* __restrict__ hoff, const MsmRow* __restrict__ rows,
     uint32_t S, uint32_t NC, uint32_t nseg, uint32_t fch, const uint32_t* __restrict__ fc0,
     uint32_t* __restrict__ fhist) {
   extern __shared__ uint32_t fcnt[];
@@ -405,7 +430,7 @@ __global__ __launch_bounds__(FINE_THREADS) void msm_fine_count_kernel(
   __syncthreads();
   const uint32_t a = s0 + j * fch;
   const uint32_t e = (s1 - a > fch) ? a + fch : s1;
-  for (uint32_t p = a + threadIdx.x; p < e; p += FINE_THREADS) atomicAdd(&fcnt[items_f[p]], 1u);
+  for (uint32_t p = a + threadIdx.x; p < e; p += FINE_THREADS) atomicAdd(&fcnt[fine_key<PK>(items_v, items_f, p, S)], 1u);
   __syncthreads();
   uint32_t* out = fhist + (uint64_t)fc0[g] * NF + j;
   for (uint32_t f = threadIdx.x; f < NF; f += FINE_THREADS) out[(uint64_t)f * nc] = fcnt[f];
@@ -414,6 +439,7 @@ __global__ __launch_bounds__(FINE_THREADS) void msm_fine_count_kernel(
 // places the chunk's entries at their bucket positions; chunk 0 of a
 // segment writes the CSR row pointers of its buckets (bucket |d| = bin + 1,
 // key = row B + |d|; bucket 0 of each row is empty)
+template <bool PK>
 __global__ __launch_bounds__(FINE_THREADS) void msm_fine_scatter_kernel(
     const uint32_t* __restrict__ items_v, const uint16_t* __restrict__ items_f,
     const uint32_t* __restrict__ hoff, const MsmRow* __restrict__ rows, uint32_t S, uint32_t NC,
@@ -439,8 +465,8 @@ __global__ __launch_bounds__(FINE_THREADS) void msm_fine_scatter_kernel(
   const uint32_t a = s0 + j * fch;
   const uint32_t e = (s1 - a > fch) ? a + fch : s1;
   for (uint32_t p = a + threadIdx.x; p < e; p += FINE_THREADS) {
-    const uint32_t pos = atomicAdd(&fpos[items_f[p]], 1u);
-    sorted[pos] = items_v[p];
+    const uint32_t pos = atomicAdd(&fpos[fine_key<PK>(items_v, items_f, p, S)], 1u);
+    sorted[pos] = entry_value<PK>(items_v[p], S);
   }
 }
 
@@ -454,6 +480,7 @@ static constexpr uint32_t FINE_STAGE_THREADS = 512;
 static constexpr uint32_t FINE_STAGE_ITEMS = FINE_STAGE / FINE_STAGE_THREADS;
 static constexpr uint32_t FINE_STAGE_KEYS = 2 * FINE_STAGE_THREADS;
 
+template <bool PK>
 __global__ __launch_bounds__(FINE_STAGE_THREADS) void msm_fine_scatter_staged_kernel(
     const uint32_t* __restrict__ items_v, const uint16_t* __restrict__ items_f,
     const uint32_t* __restrict__ hoff, const MsmRow* __restrict__ rows, uint32_t S, uint32_t NC,
@@ -462,7 +489,7 @@ __global__ __launch_bounds__(FINE_STAGE_THREADS) void msm_fine_scatter_staged_ke
   __shared__ uint32_t gpos[FINE_STAGE_KEYS];
   __shared__ uint32_t lcnt[FINE_STAGE_KEYS];
   __shared__ uint32_t stage_v[FINE_STAGE];
-  __shared__ uint16_t stage_f[FINE_STAGE];
+  __shared__ uint16_t stage_f[PK ? 1 : FINE_STAGE];
   __shared__ uint32_t scan_sh[FINE_STAGE_THREADS];
   uint32_t g, j, nc;
   if (!fine_chunk(fc0, nseg, blockIdx.x, g, j, nc)) return;
@@ -491,7 +518,7 @@ __global__ __launch_bounds__(FINE_STAGE_THREADS) void msm_fine_scatter_staged_ke
     const uint32_t q = i * FINE_STAGE_THREADS + tid;
     if (q < cnt) {
       v[i] = items_v[a + q];
-      f[i] = items_f[a + q];
+      f[i] = PK ? (v[i] >> (31 - S)) & (NF - 1) : items_f[a + q];
       rank[i] = atomicAdd(&lcnt[f[i]], 1u);
     }
   }
@@ -511,13 +538,14 @@ __global__ __launch_bounds__(FINE_STAGE_THREADS) void msm_fine_scatter_staged_ke
     if (q < cnt) {
       const uint32_t slot = lcnt[f[i]] + rank[i];
       stage_v[slot] = v[i];
-      stage_f[slot] = (uint16_t)f[i];
+      if (!PK) stage_f[slot] = (uint16_t)f[i];
     }
   }
   __syncthreads();
   for (uint32_t slot = tid; slot < cnt; slot += FINE_STAGE_THREADS) {
-    const uint32_t ff = stage_f[slot];
-    sorted[gpos[ff] + (slot - lcnt[ff])] = stage_v[slot];
+    const uint32_t sv = stage_v[slot];
+    const uint32_t ff = PK ? (sv >> (31 - S)) & (NF - 1) : stage_f[slot];
+    sorted[gpos[ff] + (slot - lcnt[ff])] = entry_value<PK>(sv, S);
   }
 }
 
@@ -777,6 +805,7 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // Shape of one pipeline launch sequence over `nrows` rows of digits.
 struct MsmShape {
+  bool packed = false;  // fine key carried in the entry's bits [31 - S, 31)
   int c = 0;
   uint32_t W = 0;       // windows per scalar
   uint32_t B = 0;       // buckets per row (incl. the empty bucket 0)
@@ -829,6 +858,17 @@ static int make_shape(Ctx* c, int cb, uint32_t nrows, const std::vector<MsmRow>&
     sh.pairs += r.len;
   }
   if (sh.pairs >= (1ull << 32)) return set_err(c, ZK_ERR_INVALID, "total digit count must be below 2^32");
+  {
+    // packed sort entries need every base index below 2^(31 - S)
+    uint64_t bound = 0;
+    for (const MsmRow& r : rows) {
+      if (r.len == 0) continue;
+      const uint64_t nw = (r.len + r.inner - 1) / r.inner;
+      const uint64_t e = (nw - 1) * r.stride + r.boff + (r.inner < r.len ? r.inner : r.len);
+      bound = e > bound ? e : bound;
+    }
+    sh.packed = bound <= (1ull << (31 - sh.S));
+  }
   if (nrows > 65535) return set_err(c, ZK_ERR_INVALID, "too many MSM rows in one launch");
   sh.hist_len = (uint64_t)sh.tile_slots * sh.NC;
   if ((1u << sh.S) <= FINE_STAGE_KEYS) sh.fch = FINE_STAGE;  // staged placement
@@ -905,6 +945,31 @@ static void scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t len, uint
 
 // sort -> accumulate -> fix-up -> per-row reduction -> export (R = 2^256 XYZZ
 // per row in b.rexp).  No allocation, no host synchronisation.
+// coarse scatter and fine placement (after the coarse histogram scan)
+template <bool PK>
+static void sort_placement(const MsmShape& sh, const MsmBuffers& b, hipStream_t st) {
+  const dim3 g(sh.max_tiles, sh.nrows);
+  if (sh.max_tiles > 0)
+    msm_sort_scatter_kernel<PK><<<g, SORT_THREADS, 0, st>>>(b.digits, b.rows, sh.S, sh.NC, b.hist, b.items_v,
+                                                             b.items_f);
+  const uint32_t nseg = sh.nrows * sh.NC, NF = 1u << sh.S;
+  msm_fine_chunks_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(b.hist, b.rows, sh.NC, nseg, sh.fch, b.fc0);
+  scan_exclusive(b.fc0, b.fc0, nseg, b.sums, st);  // fc0[nseg] = chunks in use (<= sh.fchunks)
+  msm_fine_count_kernel<PK><<<sh.fchunks, FINE_THREADS, NF * 4, st>>>(b.items_v, b.items_f, b.hist, b.rows, sh.S,
+                                                                      sh.NC, nseg, sh.fch, b.fc0, b.fhist);
+  // entries past the chunks in use are never read; scanning them is harmless
+  scan_exclusive(b.fhist, b.fhist, (uint64_t)sh.fchunks * NF, b.sums, st);
+  if (NF <= FINE_STAGE_KEYS)
+    msm_fine_scatter_staged_kernel<PK><<<sh.fchunks, FINE_STAGE_THREADS, 0, st>>>(
+        b.items_v, b.items_f, b.hist, b.rows, sh.S, sh.NC, nseg, sh.fch, b.fc0, b.fhist, sh.nbins, sh.B,
+        b.row_ptr, b.sorted);
+  else
+    msm_fine_scatter_kernel<PK><<<sh.fchunks, FINE_THREADS, NF * 4, st>>>(b.items_v, b.items_f, b.hist, b.rows,
+                                                                          sh.S, sh.NC, nseg, sh.fch, b.fc0,
+                                                                          b.fhist, sh.nbins, sh.B, b.row_ptr,
+                                                                          b.sorted);
+}
+
 static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* bases9, hipStream_t st,
                     hipEvent_t sorted_ev = nullptr) {
   hipEvent_t ev;
@@ -913,26 +978,10 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
   if (sh.max_tiles > 0)
     msm_sort_count_kernel<<<g, SORT_THREADS, 0, st>>>(b.digits, b.rows, sh.S, sh.NC, b.hist);
   scan_exclusive(b.hist, b.hist, sh.hist_len, b.sums, st);  // in place; hist[hist_len] = total
-  if (sh.max_tiles > 0)
-    msm_sort_scatter_kernel<<<g, SORT_THREADS, 0, st>>>(b.digits, b.rows, sh.S, sh.NC, b.hist, b.items_v,
-                                                         b.items_f);
-  {
-    const uint32_t nseg = sh.nrows * sh.NC, NF = 1u << sh.S;
-    msm_fine_chunks_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(b.hist, b.rows, sh.NC, nseg, sh.fch, b.fc0);
-    scan_exclusive(b.fc0, b.fc0, nseg, b.sums, st);  // fc0[nseg] = chunks in use (<= sh.fchunks)
-    msm_fine_count_kernel<<<sh.fchunks, FINE_THREADS, NF * 4, st>>>(b.items_f, b.hist, b.rows, sh.S, sh.NC, nseg,
-                                                                    sh.fch, b.fc0, b.fhist);
-    // entries past the chunks in use are never read; scanning them is harmless
-    scan_exclusive(b.fhist, b.fhist, (uint64_t)sh.fchunks * NF, b.sums, st);
-    if (NF <= FINE_STAGE_KEYS)
-      msm_fine_scatter_staged_kernel<<<sh.fchunks, FINE_STAGE_THREADS, 0, st>>>(
-          b.items_v, b.items_f, b.hist, b.rows, sh.S, sh.NC, nseg, sh.fch, b.fc0, b.fhist, sh.nbins, sh.B,
-          b.row_ptr, b.sorted);
-    else
-      msm_fine_scatter_kernel<<<sh.fchunks, FINE_THREADS, NF * 4, st>>>(b.items_v, b.items_f, b.hist, b.rows, sh.S,
-                                                                        sh.NC, nseg, sh.fch, b.fc0, b.fhist,
-                                                                        sh.nbins, sh.B, b.row_ptr, b.sorted);
-  }
+  if (sh.packed)
+    sort_placement<true>(sh, b, st);
+  else
+    sort_placement<false>(sh, b, st);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_sort", st, ev);
   if (sorted_ev) ZK_HIP(c, hipEventRecord(sorted_ev, st));
