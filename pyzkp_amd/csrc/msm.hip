@@ -123,12 +123,8 @@ __global__ void msm_wsum_export_kernel(const pt9* __restrict__ in, uint32_t W, x
 
 // 1. scalar (Montgomery) -> canonical -> W signed c-bit digits
 //    digits[w * n + i] = |d| | (d < 0 ? SIGN : 0)
-__global__ __launch_bounds__(256) void msm_digits_kernel(const fe* __restrict__ scalars, uint64_t n,
-                                                         uint32_t c, uint32_t W,
-                                                         uint32_t* __restrict__ digits) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const fe s = Fr::from_mont(load_fe(scalars + i));
+template <class Sink>
+__device__ __forceinline__ void signed_digits(const fe& s, uint32_t c, uint32_t W, Sink&& sink) {
   uint64_t buf = (uint64_t)s.w[0] | ((uint64_t)s.w[1] << 32);
   int have = 64;
   uint32_t nx[6];
@@ -157,8 +153,17 @@ __global__ __launch_bounds__(256) void msm_digits_kernel(const fe* __restrict__ 
       carry = 0;
       out = val;
     }
-    digits[(uint64_t)w * n + i] = out;
+    sink(w, out);
   }
+}
+
+__global__ __launch_bounds__(256) void msm_digits_kernel(const fe* __restrict__ scalars, uint64_t n,
+                                                         uint32_t c, uint32_t W,
+                                                         uint32_t* __restrict__ digits) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  signed_digits(Fr::from_mont(load_fe(scalars + i)), c, W,
+                [&](uint32_t w, uint32_t out) { digits[(uint64_t)w * n + i] = out; });
 }
 
 // 2. bucket sort of the (row, |d|) pairs in two levels.  A row is one window
@@ -221,6 +226,35 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_sort_count_kernel(const uint
   __syncthreads();
   for (uint32_t k = threadIdx.x; k < NC; k += SORT_THREADS)
     hist[(uint64_t)row.tile0 * NC + (uint64_t)k * row.ntiles + tile] = cnt[k];
+}
+
+// the same digits, one workgroup per sort tile (SORT_TILE scalars), which
+// also counts the tile's coarse keys of every window in LDS and writes the
+// coarse histogram the sort scans (window w is row w, tile0 = w * nt), so
+// the sort does not read the digits once more just to count them
+static constexpr uint32_t DIGITS_COUNT_THREADS = 512;
+__global__ __launch_bounds__(DIGITS_COUNT_THREADS) void msm_digits_count_kernel(
+    const fe* __restrict__ scalars, uint64_t n, uint32_t c, uint32_t W, uint32_t S, uint32_t NC, uint32_t nt,
+    uint32_t* __restrict__ digits, uint32_t* __restrict__ hist) {
+  extern __shared__ uint32_t wcnt[];  // W x NC
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t k = tid; k < W * NC; k += DIGITS_COUNT_THREADS) wcnt[k] = 0;
+  __syncthreads();
+  const uint64_t p0 = (uint64_t)blockIdx.x * SORT_TILE;
+  for (uint32_t j = tid; j < SORT_TILE; j += DIGITS_COUNT_THREADS) {
+    const uint64_t i = p0 + j;
+    if (i >= n) break;
+    signed_digits(Fr::from_mont(load_fe(scalars + i)), c, W, [&](uint32_t w, uint32_t out) {
+      digits[(uint64_t)w * n + i] = out;
+      const uint32_t mag = out & ~SIGN;
+      if (mag) atomicAdd(&wcnt[w * NC + ((mag - 1) >> S)], 1u);
+    });
+  }
+  __syncthreads();
+  for (uint32_t idx = tid; idx < W * NC; idx += DIGITS_COUNT_THREADS) {
+    const uint32_t w = idx / NC, k = idx - w * NC;
+    hist[((uint64_t)w * nt) * NC + (uint64_t)k * nt + blockIdx.x] = wcnt[idx];
+  }
 }
 
 // exclusive scan building blocks (a hierarchical scan over any length)
@@ -970,12 +1004,13 @@ static void sort_placement(const MsmShape& sh, const MsmBuffers& b, hipStream_t 
                                                                           b.sorted);
 }
 
+// counted: the coarse histogram is already written (msm_digits_count_kernel)
 static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* bases9, hipStream_t st,
-                    hipEvent_t sorted_ev = nullptr) {
+                    hipEvent_t sorted_ev = nullptr, bool counted = false) {
   hipEvent_t ev;
   phase_begin(c, "msm_sort", st, &ev);
   const dim3 g(sh.max_tiles, sh.nrows);
-  if (sh.max_tiles > 0)
+  if (sh.max_tiles > 0 && !counted)
     msm_sort_count_kernel<<<g, SORT_THREADS, 0, st>>>(b.digits, b.rows, sh.S, sh.NC, b.hist);
   scan_exclusive(b.hist, b.hist, sh.hist_len, b.sums, st);  // in place; hist[hist_len] = total
   if (sh.packed)
@@ -1099,12 +1134,16 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   hipEvent_t ev;
   phase_begin(c, "msm_digits", st, &ev);
   msm_bases_convert_kernel<<<grid_for(n, 256), 256, 0, st>>>(bases, n, bases9);
-  msm_digits_kernel<<<grid_for(n, 256), 256, 0, st>>>(scalars, n, (uint32_t)cb, W, digits);
+  if (G == 1)
+    msm_digits_count_kernel<<<nt, DIGITS_COUNT_THREADS, (size_t)W * sh[0].NC * 4, st>>>(
+        scalars, n, (uint32_t)cb, W, sh[0].S, sh[0].NC, nt, digits, b[0].hist);
+  else
+    msm_digits_kernel<<<grid_for(n, 256), 256, 0, st>>>(scalars, n, (uint32_t)cb, W, digits);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_digits", st, ev);
 
   if (G == 1) {
-    rc = run_rows(c, sh[0], b[0], bases9, st);
+    rc = run_rows(c, sh[0], b[0], bases9, st, nullptr, true);
     if (rc) return rc;
   } else {
     hipStream_t S[2] = {st, c->aux_stream};
