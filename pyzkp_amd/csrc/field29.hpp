@@ -37,6 +37,9 @@ struct Fq29Params {
   static constexpr u32 P3[9] = {0x0976f7d5u, 0x030d2224u, 0x1557e9edu, 0x087f6872u, 0x00918c68u, 0x0891c242u, 0x01f4a3c3u, 0x0b14729cu, 0x00912cebu};
   static constexpr u32 P4[9] = {0x01f3f51cu, 0x041182dbu, 0x11ca8d3cu, 0x0b548b43u, 0x161765e0u, 0x0b6d0302u, 0x029b8504u, 0x197098d0u, 0x00c19139u};
   static constexpr u32 PINV = 0x04866389u;  // -p^-1 mod 2^29
+  // 2p with limbs 0..7 raised by 2^29 (borrowed from the next limb): every
+  // limb of a normalised value below 1.2p is at most the matching limb here
+  static constexpr u32 K2P[9] = {0x30f9fa8eu, 0x2208c16cu, 0x38e5469du, 0x25aa45a0u, 0x2b0bb2efu, 0x25b68180u, 0x214dc281u, 0x3cb84c67u, 0x0060c89bu};
   static constexpr u32 ONE[9] = {0x157ccc21u, 0x141c2758u, 0x185230d3u, 0x014c0419u, 0x0aa36fb9u, 0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};                    // 2^261 mod p
   static constexpr u32 TO261[9] = {0x13349ca1u, 0x1a5d84a8u, 0x0a3e5cacu, 0x100249e0u, 0x12b951e8u, 0x0e92d304u, 0x14cb95b3u, 0x041b9d3du, 0x00058003u};         // 2^266 mod p: x 2^256 -> x 2^261
   static constexpr u32 TO256[9] = {0x058f0d9du, 0x1aea1c6eu, 0x11c2cf74u, 0x11d651ebu, 0x1462c0a7u, 0x11b7bc3cu, 0x1cbd99bau, 0x183340fbu, 0x000e0a77u};         // 2^256 mod p: x 2^261 -> x 2^256
@@ -47,6 +50,7 @@ struct Fr29Params {
   static constexpr u32 P3[9] = {0x10000003u, 0x1d2f05deu, 0x0b146cf2u, 0x1771b2dau, 0x00917789u, 0x0891c242u, 0x01f4a3c3u, 0x0b14729cu, 0x00912cebu};
   static constexpr u32 P4[9] = {0x00000004u, 0x1c3eb27eu, 0x19709143u, 0x1f4243cdu, 0x16174a0cu, 0x0b6d0302u, 0x029b8504u, 0x197098d0u, 0x00c19139u};
   static constexpr u32 PINV = 0x0fffffffu;  // -p^-1 mod 2^29
+  static constexpr u32 K2P[9] = {0x20000002u, 0x3e1f593eu, 0x3cb848a0u, 0x2fa121e5u, 0x2b0ba505u, 0x25b68180u, 0x214dc281u, 0x3cb84c67u, 0x0060c89bu};
   static constexpr u32 ONE[9] = {0x0fffff57u, 0x1ea70ab4u, 0x052c068bu, 0x17504f49u, 0x0aa8075bu, 0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};                    // 2^261 mod p
   static constexpr u32 TO261[9] = {0x0fffead7u, 0x1d5444f4u, 0x04438aa5u, 0x03b4d096u, 0x134c84dau, 0x0e92d304u, 0x14cb95b3u, 0x041b9d3du, 0x00058003u};         // 2^266 mod p: x 2^256 -> x 2^261
   static constexpr u32 TO256[9] = {0x0ffffffbu, 0x04b1a0e2u, 0x18334a6bu, 0x18ed2b3eu, 0x1462e36fu, 0x11b7bc3cu, 0x1cbd99bau, 0x183340fbu, 0x000e0a77u};         // 2^256 mod p: x 2^261 -> x 2^256
@@ -293,6 +297,37 @@ struct Mont29 {
     for (int i = 0; i < 9; i++) {
       const int32_t t = (int32_t)a.l[i] - (int32_t)b.l[i] + (int32_t)F::P2[i] + c;
       s.l[i] = (u32)t & M29;
+      c = t >> 29;
+    }
+    return s;
+  }
+
+  // Carry-free forms (the NTT butterflies).  Limbs may exceed 29 bits; the
+  // value sum_i l_i 2^(29 i) is exact as long as no limb leaves [0, 2^32).
+  // a + b, limb by limb
+  static ZK_HD f9 add_cf(const f9& a, const f9& b) {
+    f9 s;
+#pragma unroll
+    for (int i = 0; i < 9; i++) s.l[i] = a.l[i] + b.l[i];
+    return s;
+  }
+  // a - b + 2p, limb by limb, for b normalised with value below 1.2p (then
+  // b's limbs are at most K2P's, so no limb goes negative); limbs grow by
+  // less than 2^30
+  static ZK_HD f9 sub_cf(const f9& a, const f9& b) {
+    f9 s;
+#pragma unroll
+    for (int i = 0; i < 9; i++) s.l[i] = a.l[i] + (F::K2P[i] - b.l[i]);
+    return s;
+  }
+  // carry propagation back to 29-bit limbs (same value; it must be below 2^261)
+  static ZK_HD f9 normalize(const f9& a) {
+    f9 s;
+    u32 c = 0;
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      const u32 t = a.l[i] + c;
+      s.l[i] = (i < 8) ? (t & M29) : t;
       c = t >> 29;
     }
     return s;

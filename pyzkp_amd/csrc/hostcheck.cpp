@@ -226,6 +226,49 @@ int zkhost_madd_chain(const uint64_t* p16, const uint64_t* affs, const uint8_t* 
   return 0;
 }
 
+// The arithmetic of one NTT pass holding the whole transform (n = 2^log_n
+// <= 1024 elements, global stage 0 first), in the pass kernel's order:
+// caller words reduced below 4p on load, stage 0 with the carrying forms,
+// later stages with the carry-free forms and u normalised at every even
+// stage, then normalise, quotient-estimate reduction and canonical form.
+// tw: omega^e (e < n/2), 4 words each, R = 2^261 canonical.  max_limb gets
+// the largest limb any stage wrote.
+int zkhost_ntt_tile(const uint64_t* in, const uint64_t* tw, uint32_t log_n, uint64_t* out, uint32_t* max_limb) {
+  if (log_n == 0 || log_n > 10) return 1;
+  const uint32_t n = 1u << log_n;
+  f9 a[1024], b[1024];
+  for (uint32_t j = 0; j < n; j++) a[j] = Fr9::reduce2p(unpack9(rd(in + 4 * j)));
+  uint32_t mx = 0;
+  for (uint32_t t = 0; t < log_n; t++) {
+    for (uint32_t q = 0; q < n / 2; q++) {
+      const uint32_t d = q & ((1u << t) - 1);
+      const f9 u = a[q], v = a[q + n / 2];
+      const uint32_t o = ((q - d) << 1) + d;
+      f9 lo, hi;
+      if (t != 0) {
+        const f9 vv = Fr9::mul(v, unpack9(rd(tw + 4 * ((uint64_t)d << (log_n - 1 - t)))));
+        const f9 uu = ((t & 1) == 0) ? Fr9::normalize(u) : u;
+        lo = Fr9::add_cf(uu, vv);
+        hi = Fr9::sub_cf(uu, vv);
+      } else {
+        const f9 vv = Fr9::reduce2p(v);
+        lo = Fr9::add_nr(u, vv);
+        hi = Fr9::sub_nr(u, vv);
+      }
+      for (int i = 0; i < 9; i++) {
+        mx = lo.l[i] > mx ? lo.l[i] : mx;
+        mx = hi.l[i] > mx ? hi.l[i] : mx;
+      }
+      b[o] = lo;
+      b[o + (1u << t)] = hi;
+    }
+    for (uint32_t j = 0; j < n; j++) a[j] = b[j];
+  }
+  for (uint32_t j = 0; j < n; j++) wr(out + 4 * j, pack9(Fr9::canonical(Fr9::reduce_est(Fr9::normalize(a[j])))));
+  *max_limb = mx;
+  return 0;
+}
+
 // wire affine (R = 2^256) -> limb form -> XYZZ (R = 2^256, canonical) round trip
 int zkhost_wire_roundtrip(const uint64_t* xy, uint64_t* out16) {
   aff9 a = aff9_from_wire(rd(xy), rd(xy + 4));

@@ -98,6 +98,14 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
   //      after 10 stages, inside the 2^261 the limbs hold; the product takes
   //      such inputs (its output is below p + v w / 2^261).  Global stage 0
   //      has twiddle 1 and reduces v to [0, 2p) instead.
+  //      The additions carry nothing between limbs: a limb of u + w v or
+  //      u - w v + 2p (the latter through 2p with limbs near 2^29, K2P) is
+  //      below u's limb + 2^30.  From 29-bit limbs one stage gives limbs
+  //      below 1.5 * 2^30 and a second below 2.5 * 2^30, so u is normalised
+  //      at every even pass stage after the first and no limb exceeds
+  //      2.5 * 2^30.  v enters the product with such limbs: a column then
+  //      sums at most 9 products below 2.5 * 2^59 (v by the twiddle), 9
+  //      below 2^58 (m by p) and a carry below 2^35, under 27 * 2^59 < 2^64.
   const uint32_t halfRT = RT >> 1;
   // twiddle of butterfly b at pass stage t; loaded one stage ahead so the
   // gather overlaps the current stage's products
@@ -135,10 +143,21 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
       if (b < halfRT) {
         const uint32_t c = b & (T - 1), q = b >> log_t;
         const uint32_t d = q & ((1u << t) - 1);
-        const f9 vv = (s != 0) ? Fr9::mul(v[i], twc[i]) : Fr9::reduce2p(v[i]);
         const uint32_t o = ((q - d) << 1) + d;
-        lds[(o << log_t) + c] = Fr9::add_nr(u[i], vv);
-        lds[((o + (1u << t)) << log_t) + c] = Fr9::sub_nr(u[i], vv);
+        f9 lo, hi;
+        if (s != 0) {
+          const f9 vv = Fr9::mul(v[i], twc[i]);
+          const f9 uu = ((t & 1) == 0 && t != 0) ? Fr9::normalize(u[i]) : u[i];
+          lo = Fr9::add_cf(uu, vv);
+          hi = Fr9::sub_cf(uu, vv);
+        } else {
+          // v below 2p here, not 1.2p: the carrying forms
+          const f9 vv = Fr9::reduce2p(v[i]);
+          lo = Fr9::add_nr(u[i], vv);
+          hi = Fr9::sub_nr(u[i], vv);
+        }
+        lds[(o << log_t) + c] = lo;
+        lds[((o + (1u << t)) << log_t) + c] = hi;
       }
       twc[i] = twn[i];
     }
@@ -167,7 +186,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     } else if (post_mode == SCALE_TABLE) {
       x = Fr9::mul(x, pow_lookup(post_hi, post_lo, pos));
     } else {
-      x = Fr9::reduce_est(x);
+      x = Fr9::reduce_est(Fr9::normalize(x));
     }
     if (last) x = Fr9::canonical(x);
     store_fe(out + pos, pack9(x));

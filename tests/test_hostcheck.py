@@ -338,3 +338,33 @@ def test_madd_chain_lazy(host29):
         pt = C.aff_neg(pts[j]) if ng else pts[j]
         acc = C.aff_add(acc, pt)
         assert _aff29(got) == acc
+
+
+@pytest.mark.parametrize("log_n", [1, 2, 3, 5, 8, 9, 10])
+def test_ntt_tile_carry_free(host29, log_n):
+    """The NTT pass arithmetic (carry-free butterflies after the first stage,
+    u normalised at every even stage) against the DFT of the input words,
+    on random words and on the largest words a caller can hand in; no limb
+    written by any stage reaches 2.5 * 2^30, the bound the product's column
+    sums rely on."""
+    from oracle.ntt import ntt_inplace, omega_for_size
+    host29.zkhost_ntt_tile.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_uint32, ct.c_void_p, ct.c_void_p]
+    n = 1 << log_n
+    w = omega_for_size(n)
+    tw = _arr([pow(w, e, R_MOD) * R261 % R_MOD for e in range(max(n // 2, 1))])
+    rng = random.Random(4100 + log_n)
+    top = (1 << 256) - 1
+    cases = [
+        [rng.randrange(1 << 256) for _ in range(n)],
+        [top] * n,
+        [R_MOD - 1] * n,
+        [rng.choice([top, R_MOD - 1, R_MOD, 2 * R_MOD - 1, 0]) for _ in range(n)],
+    ]
+    for words in cases:
+        out = (ct.c_uint64 * (4 * n))()
+        mx = ct.c_uint32(0)
+        assert host29.zkhost_ntt_tile(_arr(words), tw, log_n, out, ct.byref(mx)) == 0
+        want = [x % R_MOD for x in words]
+        ntt_inplace(want, w)
+        assert _vals(out, n) == want
+        assert mx.value < 5 << 29
