@@ -95,22 +95,29 @@ ZK_HD void xyzz9_finish(const f9& R, const f9& ppp, const f9& q, const f9 s1, f9
   y3 = Fq9::mul2(R, Fq9::sub_nr(q, x3), s1, Fq9::neg_nr(ppp));
 }
 
-// madd-2008-s: p += a (a.y may be 2p - y for a negated base), with the
-// caller keeping p_inf == (p is the identity) up to date, so the bucket
-// accumulation does not test ZZ at every step
-ZK_HD void xyzz9_madd_t(xyzz9& p, bool& p_inf, const aff9& a) {
+// madd-2008-s: p += a, or p -= a when neg is set, with the caller keeping
+// p_inf == (p is the identity) up to date, so the bucket accumulation does
+// not test ZZ at every step (a's coordinates below 2p).  For the negated
+// base, S2 = (4p - y) ZZZ1 takes 4p - y in the carry-free form (limbs below
+// 2^30, a valid product input; S2 stays below 1.03p); the identity and
+// doubling cases, which use the coordinate further, take the normalised
+// 2p - y.
+ZK_HD void xyzz9_madd_t(xyzz9& p, bool& p_inf, const aff9& a, bool neg) {
   if (p_inf) {
-    p = xyzz9_from_aff(a);
     p_inf = aff9_is_inf(a);
+    p = xyzz9_from_aff(a);
+    if (neg && !p_inf) p.y = Fq9::neg_nr(a.y);
     return;
   }
   const f9 u2 = Fq9::mul(a.x, p.zz);
-  const f9 s2 = Fq9::mul(a.y, p.zzz);
+  const f9 s2 = Fq9::mul(neg ? Fq9::neg_cf(a.y) : a.y, p.zzz);
   const f9 P = Fq9::sub_nr(u2, p.x);
   const f9 R = Fq9::sub_nr(s2, p.y);
   if (Fq9::is_zero_lt4p(P)) {
     if (Fq9::is_zero_lt4p(R)) {
-      p = xyzz9_dbl_aff(a);
+      aff9 b = a;
+      if (neg) b.y = Fq9::neg_nr(a.y);
+      p = xyzz9_dbl_aff(b);
     } else {
       p = xyzz9_inf();
       p_inf = true;
@@ -128,7 +135,7 @@ ZK_HD void xyzz9_madd_t(xyzz9& p, bool& p_inf, const aff9& a) {
 // madd-2008-s: p += a (a affine, not the identity)
 ZK_HD void xyzz9_madd(xyzz9& p, const aff9& a) {
   bool inf = xyzz9_is_inf(p);
-  xyzz9_madd_t(p, inf, a);
+  xyzz9_madd_t(p, inf, a, false);
 }
 
 // add-2008-s

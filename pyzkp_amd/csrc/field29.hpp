@@ -40,6 +40,8 @@ struct Fq29Params {
   // 2p with limbs 0..7 raised by 2^29 (borrowed from the next limb): every
   // limb of a normalised value below 1.2p is at most the matching limb here
   static constexpr u32 K2P[9] = {0x30f9fa8eu, 0x2208c16cu, 0x38e5469du, 0x25aa45a0u, 0x2b0bb2efu, 0x25b68180u, 0x214dc281u, 0x3cb84c67u, 0x0060c89bu};
+  // 4p in the same raised form
+  static constexpr u32 K4P[9] = {0x21f3f51cu, 0x241182dau, 0x31ca8d3bu, 0x2b548b42u, 0x361765dfu, 0x2b6d0301u, 0x229b8503u, 0x397098cfu, 0x00c19138u};
   static constexpr u32 ONE[9] = {0x157ccc21u, 0x141c2758u, 0x185230d3u, 0x014c0419u, 0x0aa36fb9u, 0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};                    // 2^261 mod p
   static constexpr u32 TO261[9] = {0x13349ca1u, 0x1a5d84a8u, 0x0a3e5cacu, 0x100249e0u, 0x12b951e8u, 0x0e92d304u, 0x14cb95b3u, 0x041b9d3du, 0x00058003u};         // 2^266 mod p: x 2^256 -> x 2^261
   static constexpr u32 TO256[9] = {0x058f0d9du, 0x1aea1c6eu, 0x11c2cf74u, 0x11d651ebu, 0x1462c0a7u, 0x11b7bc3cu, 0x1cbd99bau, 0x183340fbu, 0x000e0a77u};         // 2^256 mod p: x 2^261 -> x 2^256
@@ -51,6 +53,7 @@ struct Fr29Params {
   static constexpr u32 P4[9] = {0x00000004u, 0x1c3eb27eu, 0x19709143u, 0x1f4243cdu, 0x16174a0cu, 0x0b6d0302u, 0x029b8504u, 0x197098d0u, 0x00c19139u};
   static constexpr u32 PINV = 0x0fffffffu;  // -p^-1 mod 2^29
   static constexpr u32 K2P[9] = {0x20000002u, 0x3e1f593eu, 0x3cb848a0u, 0x2fa121e5u, 0x2b0ba505u, 0x25b68180u, 0x214dc281u, 0x3cb84c67u, 0x0060c89bu};
+  static constexpr u32 K4P[9] = {0x20000004u, 0x3c3eb27du, 0x39709142u, 0x3f4243ccu, 0x36174a0bu, 0x2b6d0301u, 0x229b8503u, 0x397098cfu, 0x00c19138u};
   static constexpr u32 ONE[9] = {0x0fffff57u, 0x1ea70ab4u, 0x052c068bu, 0x17504f49u, 0x0aa8075bu, 0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};                    // 2^261 mod p
   static constexpr u32 TO261[9] = {0x0fffead7u, 0x1d5444f4u, 0x04438aa5u, 0x03b4d096u, 0x134c84dau, 0x0e92d304u, 0x14cb95b3u, 0x041b9d3du, 0x00058003u};         // 2^266 mod p: x 2^256 -> x 2^261
   static constexpr u32 TO256[9] = {0x0ffffffbu, 0x04b1a0e2u, 0x18334a6bu, 0x18ed2b3eu, 0x1462e36fu, 0x11b7bc3cu, 0x1cbd99bau, 0x183340fbu, 0x000e0a77u};         // 2^256 mod p: x 2^261 -> x 2^256
@@ -318,6 +321,14 @@ struct Mont29 {
     f9 s;
 #pragma unroll
     for (int i = 0; i < 9; i++) s.l[i] = a.l[i] + (F::K2P[i] - b.l[i]);
+    return s;
+  }
+  // 4p - b, limb by limb, for b normalised and below 2p (its top limb is
+  // then below K4P's); limbs below 2^30
+  static ZK_HD f9 neg_cf(const f9& b) {
+    f9 s;
+#pragma unroll
+    for (int i = 0; i < 9; i++) s.l[i] = F::K4P[i] - b.l[i];
     return s;
   }
   // carry propagation back to 29-bit limbs (same value; it must be below 2^261)

@@ -215,10 +215,7 @@ int zkhost_madd_chain(const uint64_t* p16, const uint64_t* affs, const uint8_t* 
     aff9 a;
     a.x = unpack9(rd(affs + 8 * i));
     a.y = unpack9(rd(affs + 8 * i + 4));
-    if (!aff9_is_inf(a)) {
-      if (neg[i]) a.y = Fq9::neg_nr(a.y);
-      xyzz9_madd_t(P, inf, a);
-    }
+    if (!aff9_is_inf(a)) xyzz9_madd_t(P, inf, a, neg[i] != 0);
     if (inf != xyzz9_is_inf(P)) return 2;
     uint64_t* o = out + 16 * i;
     wr(o, pack9(P.x)); wr(o + 4, pack9(P.y)); wr(o + 8, pack9(P.zz)); wr(o + 12, pack9(P.zzz));

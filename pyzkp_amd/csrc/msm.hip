@@ -625,10 +625,7 @@ __global__ __launch_bounds__(256) void msm_accumulate_kernel(
     }
     const uint32_t v = sorted[p];
     aff9 a = load_aff9(bases, v & ~SIGN);
-    if (!aff9_is_inf(a)) {
-      if (v & SIGN) a.y = Fq9::neg_nr(a.y);  // 2p - y, at most 2p: a valid input to the addition
-      xyzz9_madd_t(acc, acc_inf, a);
-    }
+    if (!aff9_is_inf(a)) xyzz9_madd_t(acc, acc_inf, a, (v & SIGN) != 0);
   }
   if (rbeg < start) {
     hk = k;
