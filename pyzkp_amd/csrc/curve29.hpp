@@ -83,16 +83,18 @@ ZK_HD xyzz9 xyzz9_dbl(const xyzz9& p) {
 // 2p (p / 2^261 < 0.0060, and a product is below p + a*b / 2^261):
 //   U, S products < 1.03          P = U2 - U1 + 2p, R = S2 - S1 + 2p in (0, 3.03)
 //   PP < 1.06, PPP and Q < 1.02   (P == 0 mod p iff P is p, 2p or 3p)
-//   X3 = R^2 + (4p - PPP - 2Q) < 5.06, then below 2p by two conditional
-//        subtractions (4p, 2p)
+//   X3 = R^2 + (4p - PPP - 2Q) < 5.06, the bracket in carry-free limbs
+//        (below 2^32, added into the squaring's 64-bit high columns), then
+//        below 1.0001p by one quotient-estimate reduction
 //   Y3 = R (Q - X3 + 2p) + S1 (2p - PPP), one reduction for both products,
-//        < 1.08
+//        < 1.08; both brackets in carry-free limbs below 1.5 * 2^30 and
+//        2^30, so a column sums at most 9 products below 1.5 * 2^59, 9
+//        below 2^59, 9 below 2^58 (m by p) and a carry: under 27 * 2^59
 //   ZZ3, ZZZ3 < 1.02
-// so every output coordinate is again below 2p.
+// so every output coordinate is again below 2p, with normalised limbs.
 ZK_HD void xyzz9_finish(const f9& R, const f9& ppp, const f9& q, const f9 s1, f9& x3, f9& y3) {
-  x3 = Fq9::sqr_plus(R, Fq9::sub_4p_a_2b(ppp, q));
-  x3 = Fq9::reduce2p(Fq9::sub_if_ge(x3, Fq29Params::P4));
-  y3 = Fq9::mul2(R, Fq9::sub_nr(q, x3), s1, Fq9::neg_nr(ppp));
+  x3 = Fq9::reduce_est(Fq9::sqr_plus(R, Fq9::sub_4p_a_2b_cf(ppp, q)));
+  y3 = Fq9::mul2(R, Fq9::sub_cf(q, x3), s1, Fq9::neg2p_cf(ppp));
 }
 
 // madd-2008-s: p += a, or p -= a when neg is set, with the caller keeping

@@ -42,6 +42,8 @@ struct Fq29Params {
   static constexpr u32 K2P[9] = {0x30f9fa8eu, 0x2208c16cu, 0x38e5469du, 0x25aa45a0u, 0x2b0bb2efu, 0x25b68180u, 0x214dc281u, 0x3cb84c67u, 0x0060c89bu};
   // 4p in the same raised form
   static constexpr u32 K4P[9] = {0x21f3f51cu, 0x241182dau, 0x31ca8d3bu, 0x2b548b42u, 0x361765dfu, 0x2b6d0301u, 0x229b8503u, 0x397098cfu, 0x00c19138u};
+  // 4p with limbs 0..7 raised by 2^31 (at least 3 * 2^29 each)
+  static constexpr u32 K4P31[9] = {0x81f3f51cu, 0x841182d7u, 0x91ca8d38u, 0x8b548b3fu, 0x961765dcu, 0x8b6d02feu, 0x829b8500u, 0x997098ccu, 0x00c19135u};
   static constexpr u32 ONE[9] = {0x157ccc21u, 0x141c2758u, 0x185230d3u, 0x014c0419u, 0x0aa36fb9u, 0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};                    // 2^261 mod p
   static constexpr u32 TO261[9] = {0x13349ca1u, 0x1a5d84a8u, 0x0a3e5cacu, 0x100249e0u, 0x12b951e8u, 0x0e92d304u, 0x14cb95b3u, 0x041b9d3du, 0x00058003u};         // 2^266 mod p: x 2^256 -> x 2^261
   static constexpr u32 TO256[9] = {0x058f0d9du, 0x1aea1c6eu, 0x11c2cf74u, 0x11d651ebu, 0x1462c0a7u, 0x11b7bc3cu, 0x1cbd99bau, 0x183340fbu, 0x000e0a77u};         // 2^256 mod p: x 2^261 -> x 2^256
@@ -54,6 +56,7 @@ struct Fr29Params {
   static constexpr u32 PINV = 0x0fffffffu;  // -p^-1 mod 2^29
   static constexpr u32 K2P[9] = {0x20000002u, 0x3e1f593eu, 0x3cb848a0u, 0x2fa121e5u, 0x2b0ba505u, 0x25b68180u, 0x214dc281u, 0x3cb84c67u, 0x0060c89bu};
   static constexpr u32 K4P[9] = {0x20000004u, 0x3c3eb27du, 0x39709142u, 0x3f4243ccu, 0x36174a0bu, 0x2b6d0301u, 0x229b8503u, 0x397098cfu, 0x00c19138u};
+  static constexpr u32 K4P31[9] = {0x80000004u, 0x9c3eb27au, 0x9970913fu, 0x9f4243c9u, 0x96174a08u, 0x8b6d02feu, 0x829b8500u, 0x997098ccu, 0x00c19135u};
   static constexpr u32 ONE[9] = {0x0fffff57u, 0x1ea70ab4u, 0x052c068bu, 0x17504f49u, 0x0aa8075bu, 0x1d4240ceu, 0x11d54c07u, 0x052ac7a8u, 0x000dc836u};                    // 2^261 mod p
   static constexpr u32 TO261[9] = {0x0fffead7u, 0x1d5444f4u, 0x04438aa5u, 0x03b4d096u, 0x134c84dau, 0x0e92d304u, 0x14cb95b3u, 0x041b9d3du, 0x00058003u};         // 2^266 mod p: x 2^256 -> x 2^261
   static constexpr u32 TO256[9] = {0x0ffffffbu, 0x04b1a0e2u, 0x18334a6bu, 0x18ed2b3eu, 0x1462e36fu, 0x11b7bc3cu, 0x1cbd99bau, 0x183340fbu, 0x000e0a77u};         // 2^256 mod p: x 2^261 -> x 2^256
@@ -156,9 +159,10 @@ struct Mont29 {
     return r;
   }
 
-  // squaring: cross products once with a doubled operand; v (normalised
-  // limbs) is added to the result through the high columns, so the output
-  // is a^2 / R + v with no separate addition pass
+  // squaring: cross products once with a doubled operand; v (any 32-bit
+  // limbs, the sum below 2^261) is added to the result through the high
+  // columns, so the output is a^2 / R + v, normalised, with no separate
+  // addition pass
   static ZK_HD f9 sqr_plus(const f9& a, const f9& v) {
     u32 a2[9];
 #pragma unroll
@@ -321,6 +325,21 @@ struct Mont29 {
     f9 s;
 #pragma unroll
     for (int i = 0; i < 9; i++) s.l[i] = a.l[i] + (F::K2P[i] - b.l[i]);
+    return s;
+  }
+  // 2p - b, limb by limb, for b normalised and below 1.2p; limbs below 2^30
+  static ZK_HD f9 neg2p_cf(const f9& b) {
+    f9 s;
+#pragma unroll
+    for (int i = 0; i < 9; i++) s.l[i] = F::K2P[i] - b.l[i];
+    return s;
+  }
+  // 4p - a - 2b, limb by limb, for a and b normalised and below 1.02p (the
+  // top limbs then sum below K4P31's); limbs below 2^32
+  static ZK_HD f9 sub_4p_a_2b_cf(const f9& a, const f9& b) {
+    f9 s;
+#pragma unroll
+    for (int i = 0; i < 9; i++) s.l[i] = F::K4P31[i] - (a.l[i] + (b.l[i] << 1));
     return s;
   }
   // 4p - b, limb by limb, for b normalised and below 2p (its top limb is

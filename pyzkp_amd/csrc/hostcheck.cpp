@@ -159,12 +159,13 @@ int zkhost_xyzz29(int op, const uint64_t* p, const uint64_t* q, uint32_t k, uint
 // Lazy-reduction primitives (4 inputs of 4 words each; unused ones may be null):
 // op 0 mul2(a, b, c, d)  1 sqr_plus(a, b)  2 sub_nr(a, b)  3 neg_nr(a)
 //    4 sub_4p_a_2b(a, b)  5 sub_if_ge(a, 4p)  6 is_zero_lt4p(a)  7 reduce_est(a)
-// Inputs of op 7 are 9 x u32 limbs (a below 2^261).  Outputs are 9 x u32 limbs per value (results may exceed 2^256).
+//    8 sub_4p_a_2b_cf(a, b)  9 sub_cf(a, b)  10 neg2p_cf(a)  11 neg_cf(a)  12 normalize(a)
+// Inputs of ops 7 and 12 are 9 x u32 limbs (a below 2^261).  Outputs are 9 x u32 limbs per value (results may exceed 2^256).
 int zkhost_field29_lazy(int field, int op, const uint64_t* a, const uint64_t* b, const uint64_t* c,
                         const uint64_t* d, uint32_t* out, uint64_t n) {
   for (uint64_t i = 0; i < n; i++) {
     f9 x;
-    if (op == 7) {
+    if (op == 7 || op == 12) {
       const uint32_t* l = reinterpret_cast<const uint32_t*>(a) + 9 * i;
       for (int j = 0; j < 9; j++) x.l[j] = l[j];
     } else {
@@ -184,6 +185,11 @@ int zkhost_field29_lazy(int field, int op, const uint64_t* a, const uint64_t* b,
         case 5: r = Fr9::sub_if_ge(x, Fr29Params::P4); break;
         case 6: r = Fr9::zero(); r.l[0] = Fr9::is_zero_lt4p(x); break;
         case 7: r = Fr9::reduce_est(x); break;
+        case 8: r = Fr9::sub_4p_a_2b_cf(x, y); break;
+        case 9: r = Fr9::sub_cf(x, y); break;
+        case 10: r = Fr9::neg2p_cf(x); break;
+        case 11: r = Fr9::neg_cf(x); break;
+        case 12: r = Fr9::normalize(x); break;
         default: return 1;
       }
     } else {
@@ -196,10 +202,31 @@ int zkhost_field29_lazy(int field, int op, const uint64_t* a, const uint64_t* b,
         case 5: r = Fq9::sub_if_ge(x, Fq29Params::P4); break;
         case 6: r = Fq9::zero(); r.l[0] = Fq9::is_zero_lt4p(x); break;
         case 7: r = Fq9::reduce_est(x); break;
+        case 8: r = Fq9::sub_4p_a_2b_cf(x, y); break;
+        case 9: r = Fq9::sub_cf(x, y); break;
+        case 10: r = Fq9::neg2p_cf(x); break;
+        case 11: r = Fq9::neg_cf(x); break;
+        case 12: r = Fq9::normalize(x); break;
         default: return 1;
       }
     }
     for (int j = 0; j < 9; j++) out[9 * i + j] = r.l[j];
+  }
+  return 0;
+}
+
+// The X3 / Y3 step shared by the mixed and full additions: (R, PPP, Q, S1)
+// as R = 2^261 words -> X3, Y3 as 9 x u32 limbs each (18 per value).
+int zkhost_xyzz9_finish(const uint64_t* R, const uint64_t* ppp, const uint64_t* q, const uint64_t* s1,
+                        uint32_t* out, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++) {
+    f9 x3, y3;
+    xyzz9_finish(unpack9(rd(R + 4 * i)), unpack9(rd(ppp + 4 * i)), unpack9(rd(q + 4 * i)),
+                 unpack9(rd(s1 + 4 * i)), x3, y3);
+    for (int j = 0; j < 9; j++) {
+      out[18 * i + j] = x3.l[j];
+      out[18 * i + 9 + j] = y3.l[j];
+    }
   }
   return 0;
 }

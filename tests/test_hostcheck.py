@@ -368,3 +368,84 @@ def test_ntt_tile_carry_free(host29, log_n):
         ntt_inplace(want, w)
         assert _vals(out, n) == want
         assert mx.value < 5 << 29
+
+
+def _raw(host, field, op, n, ins, limb_input=False):
+    """Runs a lazy op; returns (value, limbs) per element without assuming
+    normalised output limbs."""
+    if limb_input:
+        a = (ct.c_uint32 * (9 * n))()
+        for i, limbs in enumerate(ins[0]):
+            for j, l in enumerate(limbs):
+                a[9 * i + j] = l
+        args = [a, None, None, None]
+    else:
+        args = [_arr(v) for v in ins] + [None] * (4 - len(ins))
+    out = (ct.c_uint32 * (9 * n))()
+    assert host.zkhost_field29_lazy(field, op, *args, out, n) == 0
+    res = []
+    for i in range(n):
+        limbs = list(out[9 * i:9 * i + 9])
+        res.append((sum(l << (29 * j) for j, l in enumerate(limbs)), limbs))
+    return res
+
+
+def _max_limbs(bound):
+    """The largest value below `bound` whose limbs 0..7 are all 2^29 - 1."""
+    return (((bound - (1 << 232)) >> 232) << 232) + (1 << 232) - 1
+
+
+@pytest.mark.parametrize("field,p", [(0, R_MOD), (1, Q_MOD)])
+def test_field29_carry_free(host29, field, p):
+    """The carry-free forms (limbs past 29 bits) at the ends of their stated
+    input ranges: exact values, the stated limb bounds, and the finishing
+    X3 / Y3 step built on them against exact integer arithmetic."""
+    rng = random.Random(700 + field)
+    n = 300
+
+    def pick(hi, extra):
+        return [rng.randrange(hi) for _ in range(n - len(extra))] + extra
+
+    p102, p12 = p * 102 // 100, p * 12 // 10
+    # 4p - a - 2b for a, b below 1.02p: limbs below 2^32 (u32), exact value
+    a = pick(p102, [_max_limbs(p102), 0, _max_limbs(p102), p102 - 1])
+    b = pick(p102, [_max_limbs(p102), _max_limbs(p102), 0, p102 - 1])
+    for (v, _), x, y in zip(_raw(host29, field, 8, n, [a, b]), a, b):
+        assert v == 4 * p - x - 2 * y
+    # a - b + 2p for b below 1.2p: limbs below 1.5 * 2^30
+    a = pick(2 * p, [_max_limbs(2 * p), 0, 2 * p - 1])
+    b = pick(p12, [0, _max_limbs(p12), p12 - 1])
+    for (v, limbs), x, y in zip(_raw(host29, field, 9, n, [a, b]), a, b):
+        assert v == x - y + 2 * p and max(limbs) < 3 << 29
+    # 2p - a for a below 1.2p, 4p - a for a below 2p: limbs below 2^30
+    a = pick(p12, [0, _max_limbs(p12), p12 - 1])
+    for (v, limbs), x in zip(_raw(host29, field, 10, n, [a]), a):
+        assert v == 2 * p - x and max(limbs) < 1 << 30
+    a = pick(2 * p, [0, _max_limbs(2 * p), 2 * p - 1])
+    for (v, limbs), x in zip(_raw(host29, field, 11, n, [a]), a):
+        assert v == 4 * p - x and max(limbs) < 1 << 30
+    # normalise limbs up to 2.5 * 2^30 back to 29 bits
+    ls = [[rng.randrange(5 << 29) for _ in range(8)] + [rng.randrange(1 << 28)] for _ in range(n - 1)]
+    ls.append([(5 << 29) - 1] * 8 + [(1 << 28) - 1])
+    for (v, limbs), x in zip(_raw(host29, field, 12, n, [ls], limb_input=True), ls):
+        assert v == sum(l << (29 * j) for j, l in enumerate(x)) and max(limbs[:8]) < 1 << 29
+    if p != Q_MOD:
+        return
+    # the finishing step of the additions (over Fq): R < 3.03p, PPP and Q < 1.02p, S1 < 2p
+    p303 = p * 303 // 100
+    R = pick(p303, [_max_limbs(p303), 0, p303 - 1, _max_limbs(p303)])
+    PPP = pick(p102, [0, _max_limbs(p102), _max_limbs(p102), 0])
+    Q = pick(p102, [_max_limbs(p102), 0, _max_limbs(p102), p102 - 1])
+    S1 = pick(2 * p, [_max_limbs(2 * p), 2 * p - 1, 0, _max_limbs(2 * p)])
+    host29.zkhost_xyzz9_finish.argtypes = [ct.c_void_p] * 5 + [ct.c_uint64]
+    out = (ct.c_uint32 * (18 * n))()
+    assert host29.zkhost_xyzz9_finish(_arr(R), _arr(PPP), _arr(Q), _arr(S1), out, n) == 0
+    rinv = pow(R261, -1, p)
+    for i, (r, ppp, q, s1) in enumerate(zip(R, PPP, Q, S1)):
+        xl, yl = out[18 * i:18 * i + 9], out[18 * i + 9:18 * i + 18]
+        assert max(xl[:8]) < 1 << 29 and max(yl[:8]) < 1 << 29
+        x3 = sum(l << (29 * j) for j, l in enumerate(xl))
+        y3 = sum(l << (29 * j) for j, l in enumerate(yl))
+        assert x3 % p == (r * r * rinv + 4 * p - ppp - 2 * q) % p and x3 < p + (p >> 20)
+        t = r * (q - x3 + 2 * p) + s1 * (2 * p - ppp)
+        assert y3 % p == t * rinv % p and y3 * R261 < p * R261 + t
