@@ -5,6 +5,7 @@
 // little-endian words (x, y; 8 x u64, all zero = infinity).
 #include <string.h>
 #include "internal.hpp"
+#include "curve.hpp"
 
 namespace zk {
 namespace g1h {
@@ -59,7 +60,40 @@ static J1 jadd(const J1& p, const J1& q) {
   return r;
 }
 
+static Fe words64(const fe& a) {
+  Fe r;
+  for (int i = 0; i < 4; i++) r.v[i] = (uint64_t)a.w[2 * i] | ((uint64_t)a.w[2 * i + 1] << 32);
+  return r;
+}
+
+// XYZZ (x = X / ZZ, y = Y / ZZZ, ZZ^3 = ZZZ^2) -> Jacobian (X ZZ^2, Y ZZZ^2, ZZZ)
+static J1 from_xyzz(const xyzz& p) {
+  const Fe zz = words64(p.zz), zzz = words64(p.zzz);
+  if (zkh::is_zero(zz)) return {zkh::one(FQ), zkh::one(FQ), zkh::zero()};
+  return {zkh::mul(FQ, words64(p.x), zkh::mul(FQ, zz, zz)), zkh::mul(FQ, words64(p.y), zkh::mul(FQ, zzz, zzz)),
+          zzz};
+}
+
 }  // namespace g1h
+
+// The window combination of an MSM, sum_w 2^(c w) S_w by Horner's rule, on
+// the host in Jacobian coordinates over the 4 x 64-bit field (a doubling is
+// 7 products).  S_w: XYZZ, R = 2^256 Montgomery words.  out: Jacobian X, Y,
+// Z (4 words each), all zero for the identity.
+void g1_window_combine(const xyzz* s, uint32_t W, int c, uint64_t out[12]) {
+  memset(out, 0, 12 * sizeof(uint64_t));
+  if (W == 0) return;
+  g1h::J1 acc = g1h::from_xyzz(s[W - 1]);
+  for (int w = (int)W - 2; w >= 0; w--) {
+    for (int d = 0; d < c; d++) acc = g1h::jdbl(acc);
+    acc = g1h::jadd(acc, g1h::from_xyzz(s[w]));
+  }
+  if (g1h::inf(acc)) return;
+  memcpy(out, acc.X.v, 32);
+  memcpy(out + 4, acc.Y.v, 32);
+  memcpy(out + 8, acc.Z.v, 32);
+}
+
 }  // namespace zk
 
 using namespace zk;

@@ -86,8 +86,55 @@ inline Fe neg(const Mod& M, const Fe& a) {
   return r;
 }
 
-// CIOS Montgomery product with an explicit carry word.
-inline Fe mul(const Mod& M, const Fe& a, const Fe& b) {
+// CIOS Montgomery product with an explicit carry word (any a, b below
+// 2^256 with a b < 2^256 p; output below p for canonical inputs).  The
+// modulus is a template argument, so its words are immediates and the
+// limb loops unroll completely.
+template <u64 P0, u64 P1, u64 P2, u64 P3, u64 PINV>
+inline Fe mul_mod(const Fe& a, const Fe& b) {
+  constexpr u64 p[4] = {P0, P1, P2, P3};
+  u64 t0 = 0, t1 = 0, t2 = 0, t3 = 0, t4 = 0;
+#pragma GCC unroll 4
+  for (int i = 0; i < 4; i++) {
+    const u64 bi = b.v[i];
+    u128 s = (u128)a.v[0] * bi + t0;
+    t0 = (u64)s;
+    u64 c = (u64)(s >> 64);
+    s = (u128)a.v[1] * bi + t1 + c;
+    t1 = (u64)s;
+    c = (u64)(s >> 64);
+    s = (u128)a.v[2] * bi + t2 + c;
+    t2 = (u64)s;
+    c = (u64)(s >> 64);
+    s = (u128)a.v[3] * bi + t3 + c;
+    t3 = (u64)s;
+    c = (u64)(s >> 64);
+    s = (u128)t4 + c;
+    t4 = (u64)s;
+    const u64 t5 = (u64)(s >> 64);
+    const u64 m = t0 * PINV;
+    s = (u128)m * p[0] + t0;
+    c = (u64)(s >> 64);
+    s = (u128)m * p[1] + t1 + c;
+    t0 = (u64)s;
+    c = (u64)(s >> 64);
+    s = (u128)m * p[2] + t2 + c;
+    t1 = (u64)s;
+    c = (u64)(s >> 64);
+    s = (u128)m * p[3] + t3 + c;
+    t2 = (u64)s;
+    c = (u64)(s >> 64);
+    s = (u128)t4 + c;
+    t3 = (u64)s;
+    t4 = t5 + (u64)(s >> 64);
+  }
+  Fe r = {{t0, t1, t2, t3}};
+  if (t4 != 0 || geq(r.v, p)) sub_raw(r.v, r.v, p);
+  return r;
+}
+
+// the same product for a modulus known only at run time
+inline Fe mul_any(const Mod& M, const Fe& a, const Fe& b) {
   u64 t[6] = {0, 0, 0, 0, 0, 0};
   for (int i = 0; i < 4; i++) {
     u64 c = 0;
@@ -115,6 +162,16 @@ inline Fe mul(const Mod& M, const Fe& a, const Fe& b) {
   memcpy(r.v, t, 32);
   if (t[4] != 0 || geq(r.v, M.p)) sub_raw(r.v, r.v, M.p);
   return r;
+}
+
+inline Fe mul(const Mod& M, const Fe& a, const Fe& b) {
+  if (&M == &FQ)
+    return mul_mod<0x3c208c16d87cfd47ull, 0x97816a916871ca8dull, 0xb85045b68181585dull, 0x30644e72e131a029ull,
+                   0x87d20782e4866389ull>(a, b);
+  if (&M == &FR)
+    return mul_mod<0x43e1f593f0000001ull, 0x2833e84879b97091ull, 0xb85045b68181585dull, 0x30644e72e131a029ull,
+                   0xc2e1f593efffffffull>(a, b);
+  return mul_any(M, a, b);
 }
 
 inline Fe one(const Mod& M) { Fe r; memcpy(r.v, M.one, 32); return r; }

@@ -121,6 +121,9 @@ struct FixedBases {
 };
 int fixed_bases_create(Ctx* c, const fe* bases, uint64_t n, int window_bits, FixedBases** out,
                        hipStream_t st);
+// sum_w 2^(c w) S_w over an MSM's XYZZ window sums, on the host (g1_host.hip)
+struct xyzz;
+void g1_window_combine(const xyzz* s, uint32_t W, int c, uint64_t out[12]);
 void fixed_bases_destroy(FixedBases* fb);
 int msm_fixed_batch_run(Ctx* c, const FixedBases* fb, const fe* scalars, const uint64_t* offsets,
                         const uint64_t* sizes, const uint64_t* base_offsets, uint32_t count, uint64_t* out_jac,

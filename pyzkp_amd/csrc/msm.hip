@@ -1163,12 +1163,7 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
     ZK_HIP(c, hipMemcpyAsync(hs.data() + wb[g], b[g].rexp, (size_t)(wb[g + 1] - wb[g]) * sizeof(xyzz),
                              hipMemcpyDeviceToHost, st));
   ZK_HIP(c, hipStreamSynchronize(st));
-  xyzz acc = hs[W - 1];
-  for (int w = (int)W - 2; w >= 0; w--) {
-    for (int d = 0; d < cb; d++) acc = xyzz_dbl(acc);
-    acc = xyzz_add(acc, hs[w]);
-  }
-  export_jacobian(acc, out_jac);
+  g1_window_combine(hs.data(), W, cb, out_jac);
   return ZK_OK;
 }
 
