@@ -228,8 +228,10 @@ struct Mont29 {
     return s;
   }
 
-  // x - q p with q = floor(x_8 / (p_8 + 1)), for any x below 2^261 with
-  // normalised limbs: q p <= x, and the result is below p + p / 2^20
+  // x - q p with q = floor(x_8 / (p_8 + 1)), for any x below 2^261 whose
+  // limbs 0..7 are below 2^32: q p <= x, and the result is normalised and
+  // below p + p / 2^18 (limbs 0..7 past 29 bits add under 8 * 2^232 to x
+  // beyond x_8 2^232, a few millionths of p; normalised limbs give p + p / 2^20)
   static ZK_HD f9 reduce_est(const f9& x) {
     const u32 q = x.l[8] / (F::P[8] + 1);
     f9 r;

@@ -160,7 +160,7 @@ int zkhost_xyzz29(int op, const uint64_t* p, const uint64_t* q, uint32_t k, uint
 // op 0 mul2(a, b, c, d)  1 sqr_plus(a, b)  2 sub_nr(a, b)  3 neg_nr(a)
 //    4 sub_4p_a_2b(a, b)  5 sub_if_ge(a, 4p)  6 is_zero_lt4p(a)  7 reduce_est(a)
 //    8 sub_4p_a_2b_cf(a, b)  9 sub_cf(a, b)  10 neg2p_cf(a)  11 neg_cf(a)  12 normalize(a)
-// Inputs of ops 7 and 12 are 9 x u32 limbs (a below 2^261).  Outputs are 9 x u32 limbs per value (results may exceed 2^256).
+// Inputs of ops 7 and 12 are 9 x u32 limbs (value below 2^261).  Outputs are 9 x u32 limbs per value (results may exceed 2^256).
 int zkhost_field29_lazy(int field, int op, const uint64_t* a, const uint64_t* b, const uint64_t* c,
                         const uint64_t* d, uint32_t* out, uint64_t n) {
   for (uint64_t i = 0; i < n; i++) {
@@ -254,7 +254,8 @@ int zkhost_madd_chain(const uint64_t* p16, const uint64_t* affs, const uint8_t* 
 // <= 1024 elements, global stage 0 first), in the pass kernel's order:
 // caller words reduced below 4p on load, stage 0 with the carrying forms,
 // later stages with the carry-free forms and u normalised at every even
-// stage, then normalise, quotient-estimate reduction and canonical form.
+// stage, then the quotient-estimate reduction (which takes the wide limbs)
+// and canonical form.
 // tw: omega^e (e < n/2), 4 words each, R = 2^261 canonical.  max_limb gets
 // the largest limb any stage wrote.
 int zkhost_ntt_tile(const uint64_t* in, const uint64_t* tw, uint32_t log_n, uint64_t* out, uint32_t* max_limb) {
@@ -288,7 +289,7 @@ int zkhost_ntt_tile(const uint64_t* in, const uint64_t* tw, uint32_t log_n, uint
     }
     for (uint32_t j = 0; j < n; j++) a[j] = b[j];
   }
-  for (uint32_t j = 0; j < n; j++) wr(out + 4 * j, pack9(Fr9::canonical(Fr9::reduce_est(Fr9::normalize(a[j])))));
+  for (uint32_t j = 0; j < n; j++) wr(out + 4 * j, pack9(Fr9::canonical(Fr9::reduce_est(a[j]))));
   *max_limb = mx;
   return 0;
 }

@@ -186,7 +186,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
     } else if (post_mode == SCALE_TABLE) {
       x = Fr9::mul(x, pow_lookup(post_hi, post_lo, pos));
     } else {
-      x = Fr9::reduce_est(Fr9::normalize(x));
+      x = Fr9::reduce_est(x);  // limbs up to 2.5 * 2^30 in, normalised out
     }
     if (last) x = Fr9::canonical(x);
     store_fe(out + pos, pack9(x));

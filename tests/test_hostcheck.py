@@ -424,6 +424,14 @@ def test_field29_carry_free(host29, field, p):
     a = pick(2 * p, [0, _max_limbs(2 * p), 2 * p - 1])
     for (v, limbs), x in zip(_raw(host29, field, 11, n, [a]), a):
         assert v == 4 * p - x and max(limbs) < 1 << 30
+    # quotient-estimate reduction of values whose limbs 0..7 reach 2.5 * 2^30
+    # (the NTT pass store): normalised, below p + p / 2^18
+    top = (24 * p) >> 232
+    ls = [[rng.randrange(5 << 29) for _ in range(8)] + [rng.randrange(top + 1)] for _ in range(n - 2)]
+    ls += [[(5 << 29) - 1] * 8 + [top], [0] * 8 + [top]]
+    for (v, limbs), x in zip(_raw(host29, field, 7, n, [ls], limb_input=True), ls):
+        xv = sum(l << (29 * j) for j, l in enumerate(x))
+        assert v % p == xv % p and v < p + (p >> 18) and max(limbs) < 1 << 29
     # normalise limbs up to 2.5 * 2^30 back to 29 bits
     ls = [[rng.randrange(5 << 29) for _ in range(8)] + [rng.randrange(1 << 28)] for _ in range(n - 1)]
     ls.append([(5 << 29) - 1] * 8 + [(1 << 28) - 1])
