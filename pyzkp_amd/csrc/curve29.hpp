@@ -62,18 +62,28 @@ ZK_HD xyzz9 xyzz9_dbl_aff(const aff9& a) {
   return r;
 }
 
-// dbl-2008-s
+// dbl-2008-s with lazy reductions.  Bounds, in units of p, for coordinates
+// below 2p (normalised limbs):
+//   U = 2Y < 4, limb by limb (limbs below 2^30: squaring columns stay
+//     below 2^63.5, product columns below 2^63)
+//   V = U^2 < 1.1, W = U V < 1.03, S = X V < 1.02, X^2 < 1.03
+//   M = 3 X^2 < 3.1, normalised (it is squared)
+//   X3 = M^2 + (4p - 2S) < 5.1 (bracket carry-free, in the high columns),
+//        then below 1.0001p by one quotient estimate
+//   Y3 = M (S - X3 + 2p) + W (4p - Y), one reduction: < 1.09 (brackets
+//        carry-free, limbs below 1.5 * 2^30 and 2^30: columns under 27 * 2^59)
+//   ZZ3 = V ZZ1, ZZZ3 = W ZZZ1 < 1.02
 ZK_HD xyzz9 xyzz9_dbl(const xyzz9& p) {
   if (xyzz9_is_inf(p)) return p;
-  f9 u = Fq9::dbl(p.y);
-  f9 v = Fq9::sqr(u);
-  f9 w = Fq9::mul(u, v);
-  f9 s = Fq9::mul(p.x, v);
-  f9 x2 = Fq9::sqr(p.x);
-  f9 m = Fq9::add(Fq9::dbl(x2), x2);
+  const f9 u = Fq9::add_cf(p.y, p.y);
+  const f9 v = Fq9::sqr(u);
+  const f9 w = Fq9::mul(u, v);
+  const f9 s = Fq9::mul(p.x, v);
+  const f9 x2 = Fq9::sqr(p.x);
+  const f9 m = Fq9::normalize(Fq9::add_cf(Fq9::add_cf(x2, x2), x2));
   xyzz9 r;
-  r.x = Fq9::sub(Fq9::sqr(m), Fq9::dbl(s));
-  r.y = Fq9::sub(Fq9::mul(m, Fq9::sub(s, r.x)), Fq9::mul(w, p.y));
+  r.x = Fq9::reduce_est(Fq9::sqr_plus(m, Fq9::sub_4p_a_2b_cf(Fq9::zero(), s)));
+  r.y = Fq9::mul2(m, Fq9::sub_cf(s, r.x), w, Fq9::neg_cf(p.y));
   r.zz = Fq9::mul(v, p.zz);
   r.zzz = Fq9::mul(w, p.zzz);
   return r;

@@ -129,7 +129,8 @@ int zkhost_field29(int field, int op, const uint64_t* a, const uint64_t* b, uint
 }
 
 // XYZZ on 29-bit limbs: points given as 16 words of R = 2^261 values
-// op 0: add(p, q)  1: dbl(p)  2: madd(p, affine q)  3: mul_small(p, k)
+// op 0: add(p, q)  1: dbl(p)  2: madd(p, affine q)  3: mul_small(p, k)  4: k doublings of p,
+// failing (2) if a coordinate leaves [0, 2p) on the way
 int zkhost_xyzz29(int op, const uint64_t* p, const uint64_t* q, uint32_t k, uint64_t* out) {
   xyzz9 P, R;
   P.x = unpack9(rd(p)); P.y = unpack9(rd(p + 4)); P.zz = unpack9(rd(p + 8)); P.zzz = unpack9(rd(p + 12));
@@ -141,6 +142,21 @@ int zkhost_xyzz29(int op, const uint64_t* p, const uint64_t* q, uint32_t k, uint
       break;
     }
     case 1: R = xyzz9_dbl(P); break;
+    case 4: {
+      // below 2p with 29-bit limbs: s - 2p would borrow, sub_if_ge keeps s
+      auto ok = [](const f9& c) {
+        const f9 d = Fq9::sub_if_ge(c, Fq29Params::P2);
+        for (int j = 0; j < 9; j++)
+          if (d.l[j] != c.l[j] || (j < 8 && (c.l[j] >> 29))) return false;
+        return true;
+      };
+      R = P;
+      for (uint32_t i = 0; i < k; i++) {
+        R = xyzz9_dbl(R);
+        if (!ok(R.x) || !ok(R.y) || !ok(R.zz) || !ok(R.zzz)) return 2;
+      }
+      break;
+    }
     case 2: {
       aff9 a;
       a.x = unpack9(rd(q));

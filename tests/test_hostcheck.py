@@ -457,3 +457,19 @@ def test_field29_carry_free(host29, field, p):
         assert x3 % p == (r * r * rinv + 4 * p - ppp - 2 * q) % p and x3 < p + (p >> 20)
         t = r * (q - x3 + 2 * p) + s1 * (2 * p - ppp)
         assert y3 % p == t * rinv % p and y3 * R261 < p * R261 + t
+
+
+def test_xyzz29_doubling_chain(host29):
+    """The lazily reduced doubling iterated (as the reduction's small scalar
+    multiples and the fixed-base tables iterate it): 2^k P matches the oracle
+    and every coordinate of every step stays below 2p (op 4 fails with 2
+    otherwise), from inputs anywhere below 2p."""
+    rng = random.Random(1234)
+    G = C.G1_GENERATOR
+    for _ in range(6):
+        P = C.g1_mul_affine(G, rng.randrange(1, R_MOD))
+        lam = rng.randrange(1, Q_MOD)
+        k = rng.choice([1, 2, 17, 64, 255])
+        out = (ct.c_uint64 * 16)()
+        assert host29.zkhost_xyzz29(4, _arr(_pt29(P, lam, rng)), None, k, out) == 0
+        assert _aff29(_vals(out, 4)) == C.g1_mul_affine(P, pow(2, k, R_MOD))
