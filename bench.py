@@ -53,7 +53,7 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=16, help="serve: host threads for G2 MSMs")
     ap.add_argument("--hbm-budget-gb", type=float, default=16.0, help="serve: device budget for working sets")
     ap.add_argument("--load", type=float, default=0.8, help="serve: arrival rate / plain-loop capacity")
-    ap.add_argument("--g2-workers", type=int, default=4, help="serve: concurrent host G2 MSMs")
+    ap.add_argument("--g2-workers", type=int, default=2, help="serve: concurrent host G2 MSMs")
     ap.add_argument("--log-n", type=int, default=None, help="MSM: 22, NTT: 24")
     ap.add_argument("--window-bits", type=int, default=0, help="0: autotune in 13..18")
     ap.add_argument("--skewed", action="store_true", help="skewed scalars + repeated bases")

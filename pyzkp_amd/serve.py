@@ -118,7 +118,7 @@ def working_set_bytes(pk: ProvingKey) -> int:
 
 class ProveServer:
     def __init__(self, keys: Dict[object, ProvingKey], hbm_budget: int, max_batch: int = 16,
-                 g2_workers: int = 4, cpu_threads: int = 16) -> None:
+                 g2_workers: int = 2, cpu_threads: int = 16) -> None:
         self.keys = keys
         self.budget = int(hbm_budget)
         self.max_batch = int(max_batch)
