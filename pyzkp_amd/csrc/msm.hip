@@ -317,17 +317,20 @@ __global__ __launch_bounds__(SCAN_THREADS) void scan_add_kernel(uint32_t* out, c
 // c. scatter of one tile into coarse-key order.  Packed form (PK): the
 //    fine key rides in bits [ps, 31) of the entry (ps = 31 - S, used when
 //    every base index of the launch is below 2^ps), so no separate u16
-//    array is written, staged or read by the fine level.
-template <bool PK>
+//    array is written, staged or read by the fine level.  KT / NCX: the
+//    staged coarse key's type and the key arrays' size; with at most 256
+//    coarse keys (every window up to 22 bits) u8 keys and 256-entry arrays
+//    keep the kernel at 45 KiB of LDS, three workgroups per CU.
+template <bool PK, typename KT, uint32_t NCX>
 __global__ __launch_bounds__(SORT_THREADS) void msm_sort_scatter_kernel(
     const uint32_t* __restrict__ digits, const MsmRow* __restrict__ rows, uint32_t S, uint32_t NC,
     const uint32_t* __restrict__ hoff, uint32_t* __restrict__ items_v, uint16_t* __restrict__ items_f) {
-  __shared__ uint32_t cnt[SORT_MAX_COARSE];
-  __shared__ uint32_t cstart[SORT_MAX_COARSE];
-  __shared__ uint32_t gbase[SORT_MAX_COARSE];
+  __shared__ uint32_t cnt[NCX];
+  __shared__ uint32_t cstart[NCX];
+  __shared__ uint32_t gbase[NCX];
   __shared__ uint32_t stage_v[SORT_TILE];
   __shared__ uint16_t stage_f[PK ? 1 : SORT_TILE];
-  __shared__ uint16_t stage_k[SORT_TILE];
+  __shared__ KT stage_k[SORT_TILE];
   __shared__ uint32_t scan_sh[SORT_THREADS];
   const MsmRow row = rows[blockIdx.y];
   const uint32_t tile = blockIdx.x;
@@ -377,7 +380,7 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_sort_scatter_kernel(
       stage_v[slot] = idx;
       stage_f[slot] = (uint16_t)((mag - 1) & fmask);
     }
-    stage_k[slot] = (uint16_t)k;
+    stage_k[slot] = (KT)k;
   }
   __syncthreads();
   for (uint32_t slot = tid; slot < total; slot += SORT_THREADS) {
@@ -980,9 +983,14 @@ static void scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t len, uint
 template <bool PK>
 static void sort_placement(const MsmShape& sh, const MsmBuffers& b, hipStream_t st) {
   const dim3 g(sh.max_tiles, sh.nrows);
-  if (sh.max_tiles > 0)
-    msm_sort_scatter_kernel<PK><<<g, SORT_THREADS, 0, st>>>(b.digits, b.rows, sh.S, sh.NC, b.hist, b.items_v,
-                                                             b.items_f);
+  if (sh.max_tiles > 0) {
+    if (sh.NC <= 256)
+      msm_sort_scatter_kernel<PK, uint8_t, 256><<<g, SORT_THREADS, 0, st>>>(b.digits, b.rows, sh.S, sh.NC, b.hist,
+                                                                           b.items_v, b.items_f);
+    else
+      msm_sort_scatter_kernel<PK, uint16_t, SORT_MAX_COARSE><<<g, SORT_THREADS, 0, st>>>(
+          b.digits, b.rows, sh.S, sh.NC, b.hist, b.items_v, b.items_f);
+  }
   const uint32_t nseg = sh.nrows * sh.NC, NF = 1u << sh.S;
   msm_fine_chunks_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(b.hist, b.rows, sh.NC, nseg, sh.fch, b.fc0);
   scan_exclusive(b.fc0, b.fc0, nseg, b.sums, st);  // fc0[nseg] = chunks in use (<= sh.fchunks)
