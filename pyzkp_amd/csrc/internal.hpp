@@ -45,7 +45,7 @@ struct Ctx {
   DevBuf scratch[8];  // 0-1 NTT, 2-3 MSM, 4 counters, 5 powers, 6 inversion, 7 PLONK quotient
   // tuning knobs (set through zk_set_tuning)
   int ntt_max_pass_bits = 0;      // 0: planner default
-  uint32_t msm_segment = 64;      // sorted (bucket, point) pairs per accumulation thread
+  uint32_t msm_segment = 80;      // sorted (bucket, point) pairs per accumulation thread
   uint32_t msm_reduce_chunk = 0;  // buckets per reduction thread (0: sized to fill the device)
   uint32_t msm_groups = 0;        // window groups pipelined over two streams (0: chosen per shape)
   // second stream (default priority) and events for pipelined launches, created on first use

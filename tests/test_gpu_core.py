@@ -131,10 +131,10 @@ def test_msm_same_scalar_everywhere(zctx):
     s = rng.randrange(R_MOD)
     bases = upload_bases(zctx, _bases(ks))
     sc = upload_fr_mont(zctx, [s] * n)
-    for seg in (1, 3, 64):
+    for seg in (1, 3, 64, 80):
         zctx.set_tuning("msm_segment", seg)
         assert msm_affine(zctx, bases, sc, n, 10) == M.msm_closed_form(ks, [s] * n)
-    zctx.set_tuning("msm_segment", 64)
+    zctx.set_tuning("msm_segment", 80)  # the library default
 
 
 def test_msm_window_groups(zctx):
