@@ -267,7 +267,8 @@ int zkhost_madd_chain(const uint64_t* p16, const uint64_t* affs, const uint8_t* 
 }
 
 // The arithmetic of one NTT pass holding the whole transform (n = 2^log_n
-// <= 1024 elements, global stage 0 first), in the pass kernel's order:
+// <= 1024 elements, global stage 0 first), stage by stage (the kernel runs
+// the same butterflies two stages at a time):
 // caller words reduced below 4p on load, stage 0 with the carrying forms,
 // later stages with the carry-free forms and u normalised at every even
 // stage, then the quotient-estimate reduction (which takes the wide limbs)

@@ -106,26 +106,67 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
   //      2.5 * 2^30.  v enters the product with such limbs: a column then
   //      sums at most 9 products below 2.5 * 2^59 (v by the twiddle), 9
   //      below 2^58 (m by p) and a carry below 2^35, under 27 * 2^59 < 2^64.
-  const uint32_t halfRT = RT >> 1;
-  // twiddle of butterfly b at pass stage t; loaded one stage ahead so the
-  // gather overlaps the current stage's products
-  auto tw_at = [&](uint32_t b, uint32_t t) -> f9 {
-    const uint32_t c = b & (T - 1), q = b >> log_t;
-    const uint32_t d = q & ((1u << t) - 1);
+  //      Stages run in pairs (t, t + 1), t even, as one radix-4 step: a
+  //      thread takes the rows q, q + R/4, q + R/2, q + 3R/4 of one column
+  //      (q < R/4) through both stages in registers.  Stage t pairs rows
+  //      (q, q + R/2) and (q + R/4, q + 3R/4), both with d = q mod 2^t and
+  //      so one twiddle; its outputs land on rows o, o + 2^t, o + R/2,
+  //      o + R/2 + 2^t (o = 2(q - d) + d), which stage t + 1 pairs as
+  //      (o, o + R/2) with d and (o + 2^t, o + 2^t + R/2) with d + 2^t,
+  //      writing rows o2, o2 + 2^(t+1), o2 + 2^t, o2 + 2^t + 2^(t+1) with
+  //      o2 = 4(q - d) + d.  An odd stage count ends with one radix-2 stage.
+  const uint32_t halfRT = RT >> 1, quarterRT = RT >> 2;
+  auto tw_cd = [&](uint32_t c, uint32_t d, uint32_t t) -> f9 {
     const uint64_t low = (j0 + c) & (Ns - 1);
     return tw9(tw, (low + (uint64_t)d * Ns) << (log_n - 1 - (s0 + t)));
   };
-  f9 twc[NTT_BFLY_PER_THREAD];
-  if (s0 != 0) {
-#pragma unroll
-    for (int i = 0; i < NTT_BFLY_PER_THREAD; i++) {
-      const uint32_t b = tid + i * NTT_THREADS;
-      if (b < halfRT) twc[i] = tw_at(b, 0);
+  uint32_t t = 0;
+  for (; t + 1 < k; t += 2) {
+    const bool act = tid < quarterRT;
+    const uint32_t c = tid & (T - 1), q = tid >> log_t;
+    const uint32_t d = q & ((1u << t) - 1);
+    const uint32_t R4 = R >> 2;
+    f9 x0, x1, x2, x3, w1, w2, w3;
+    if (act) {
+      x0 = lds[(q << log_t) + c];
+      x1 = lds[((q + R4) << log_t) + c];
+      x2 = lds[((q + 2 * R4) << log_t) + c];
+      x3 = lds[((q + 3 * R4) << log_t) + c];
+      if (s0 + t != 0) w1 = tw_cd(c, d, t);
+      w2 = tw_cd(c, d, t + 1);
+      w3 = tw_cd(c, d + (1u << t), t + 1);
     }
+    __syncthreads();
+    if (act) {
+      f9 A, B, C, D;
+      if (s0 + t != 0) {
+        const f9 u0 = (t != 0) ? Fr9::normalize(x0) : x0;
+        const f9 u1 = (t != 0) ? Fr9::normalize(x1) : x1;
+        const f9 v2 = Fr9::mul(x2, w1), v3 = Fr9::mul(x3, w1);
+        A = Fr9::add_cf(u0, v2);
+        B = Fr9::sub_cf(u0, v2);
+        C = Fr9::add_cf(u1, v3);
+        D = Fr9::sub_cf(u1, v3);
+      } else {
+        // global stage 0: v below 2p, not 1.2p: the carrying forms
+        const f9 v2 = Fr9::reduce2p(x2), v3 = Fr9::reduce2p(x3);
+        A = Fr9::add_nr(x0, v2);
+        B = Fr9::sub_nr(x0, v2);
+        C = Fr9::add_nr(x1, v3);
+        D = Fr9::sub_nr(x1, v3);
+      }
+      const f9 vc = Fr9::mul(C, w2), vd = Fr9::mul(D, w3);
+      const uint32_t o2 = 4 * (q - d) + d;
+      lds[(o2 << log_t) + c] = Fr9::add_cf(A, vc);
+      lds[((o2 + (2u << t)) << log_t) + c] = Fr9::sub_cf(A, vc);
+      lds[((o2 + (1u << t)) << log_t) + c] = Fr9::add_cf(B, vd);
+      lds[((o2 + (3u << t)) << log_t) + c] = Fr9::sub_cf(B, vd);
+    }
+    __syncthreads();
   }
-  for (uint32_t t = 0; t < k; t++) {
+  if (t < k) {
     const uint32_t s = s0 + t;
-    f9 u[NTT_BFLY_PER_THREAD], v[NTT_BFLY_PER_THREAD], twn[NTT_BFLY_PER_THREAD];
+    f9 u[NTT_BFLY_PER_THREAD], v[NTT_BFLY_PER_THREAD], w[NTT_BFLY_PER_THREAD];
 #pragma unroll
     for (int i = 0; i < NTT_BFLY_PER_THREAD; i++) {
       const uint32_t b = tid + i * NTT_THREADS;
@@ -133,7 +174,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         const uint32_t c = b & (T - 1), q = b >> log_t;
         u[i] = lds[(q << log_t) + c];
         v[i] = lds[((q + (R >> 1)) << log_t) + c];
-        if (t + 1 < k) twn[i] = tw_at(b, t + 1);
+        if (s != 0) w[i] = tw_cd(c, q & ((1u << t) - 1), t);
       }
     }
     __syncthreads();
@@ -146,12 +187,11 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         const uint32_t o = ((q - d) << 1) + d;
         f9 lo, hi;
         if (s != 0) {
-          const f9 vv = Fr9::mul(v[i], twc[i]);
-          const f9 uu = ((t & 1) == 0 && t != 0) ? Fr9::normalize(u[i]) : u[i];
+          const f9 vv = Fr9::mul(v[i], w[i]);
+          const f9 uu = (t != 0) ? Fr9::normalize(u[i]) : u[i];  // t is even here
           lo = Fr9::add_cf(uu, vv);
           hi = Fr9::sub_cf(uu, vv);
         } else {
-          // v below 2p here, not 1.2p: the carrying forms
           const f9 vv = Fr9::reduce2p(v[i]);
           lo = Fr9::add_nr(u[i], vv);
           hi = Fr9::sub_nr(u[i], vv);
@@ -159,7 +199,6 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         lds[(o << log_t) + c] = lo;
         lds[((o + (1u << t)) << log_t) + c] = hi;
       }
-      twc[i] = twn[i];
     }
     __syncthreads();
   }
