@@ -30,6 +30,7 @@
 // bits leave room for 24p); a pass ends with one quotient-estimate
 // reduction per element (below 1.0001p), and the last pass stores canonical
 // values.
+#include <algorithm>
 #include "internal.hpp"
 #include "field29.hpp"
 
@@ -354,6 +355,27 @@ static void plan_passes(uint32_t log_n, int max_pass_bits, std::vector<uint32_t>
   uint32_t p = (log_n + kmax - 1) / kmax;
   uint32_t base = log_n / p, extra = log_n % p;
   for (uint32_t i = 0; i < p; i++) ks.push_back(base + (i < extra ? 1 : 0));
+  // Passes run their stages two at a time; an odd pass ends with a lone
+  // radix-2 stage that costs about as much as a pair.  With the same pass
+  // count, prefer even sizes (largest first) when they fit under kmax:
+  // 20 = 8 + 6 + 6 rather than 7 + 7 + 6.
+  if (log_n % 2 == 0) {
+    std::vector<uint32_t> ev(p, 0);
+    uint32_t left = log_n / 2;  // pairs to place
+    for (uint32_t i = 0; i < p; i++) {
+      const uint32_t rest = p - 1 - i;
+      uint32_t take = (left + rest) / (rest + 1);   // near-equal split of the pairs
+      if (2 * take > kmax) take = kmax / 2;
+      ev[i] = 2 * take;
+      left -= take;
+    }
+    bool ok = left == 0;
+    for (uint32_t v : ev) ok = ok && v >= 1 && v <= kmax;
+    if (ok) {
+      std::sort(ev.begin(), ev.end(), [](uint32_t a, uint32_t b) { return a > b; });
+      ks = ev;
+    }
+  }
 }
 
 int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint32_t batch,
