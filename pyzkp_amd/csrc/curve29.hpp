@@ -76,16 +76,15 @@ ZK_HD xyzz9 xyzz9_dbl_aff(const aff9& a) {
 ZK_HD xyzz9 xyzz9_dbl(const xyzz9& p) {
   if (xyzz9_is_inf(p)) return p;
   const f9 u = Fq9::add_cf(p.y, p.y);
-  const f9 v = Fq9::sqr(u);
-  const f9 w = Fq9::mul(u, v);
-  const f9 s = Fq9::mul(p.x, v);
-  const f9 x2 = Fq9::sqr(p.x);
+  f9 v, x2;
+  Fq9::sqr_x2(u, p.x, v, x2);
+  f9 w, s;
+  Fq9::mul_x2(u, v, p.x, v, w, s);
   const f9 m = Fq9::normalize(Fq9::add_cf(Fq9::add_cf(x2, x2), x2));
   xyzz9 r;
   r.x = Fq9::reduce_est(Fq9::sqr_plus(m, Fq9::sub_4p_a_2b_cf(Fq9::zero(), s)));
   r.y = Fq9::mul2(m, Fq9::sub_cf(s, r.x), w, Fq9::neg_cf(p.y));
-  r.zz = Fq9::mul(v, p.zz);
-  r.zzz = Fq9::mul(w, p.zzz);
+  Fq9::mul_x2(v, p.zz, w, p.zzz, r.zz, r.zzz);
   return r;
 }
 
@@ -154,23 +153,25 @@ ZK_HD void xyzz9_madd(xyzz9& p, const aff9& a) {
 ZK_HD xyzz9 xyzz9_add(const xyzz9& p, const xyzz9& q) {
   if (xyzz9_is_inf(p)) return q;
   if (xyzz9_is_inf(q)) return p;
-  const f9 u1 = Fq9::mul(p.x, q.zz);
-  const f9 u2 = Fq9::mul(q.x, p.zz);
-  const f9 s1 = Fq9::mul(p.y, q.zzz);
-  const f9 s2 = Fq9::mul(q.y, p.zzz);
+  // independent products in pairs (mul_x2): the reduction's kernels run
+  // few waves, so the latency of one product chain would be exposed
+  f9 u1, u2, s1, s2;
+  Fq9::mul_x2(p.x, q.zz, q.x, p.zz, u1, u2);
+  Fq9::mul_x2(p.y, q.zzz, q.y, p.zzz, s1, s2);
   const f9 P = Fq9::sub_nr(u2, u1);
   const f9 R = Fq9::sub_nr(s2, s1);
   if (Fq9::is_zero_lt4p(P)) {
     if (Fq9::is_zero_lt4p(R)) return xyzz9_dbl(p);
     return xyzz9_inf();
   }
+  f9 zz12, zzz12;
+  Fq9::mul_x2(p.zz, q.zz, p.zzz, q.zzz, zz12, zzz12);
   const f9 pp = Fq9::sqr(P);
-  const f9 ppp = Fq9::mul(P, pp);
-  const f9 qq = Fq9::mul(u1, pp);
+  f9 ppp, qq;
+  Fq9::mul_x2(P, pp, u1, pp, ppp, qq);
   xyzz9 r;
   xyzz9_finish(R, ppp, qq, s1, r.x, r.y);
-  r.zz = Fq9::mul(Fq9::mul(p.zz, q.zz), pp);
-  r.zzz = Fq9::mul(Fq9::mul(p.zzz, q.zzz), ppp);
+  Fq9::mul_x2(zz12, pp, zzz12, ppp, r.zz, r.zzz);
   return r;
 }
 

@@ -137,6 +137,30 @@ struct Mont29 {
     return r;
   }
 
+  // two independent products a*b / R and c*d / R with their columns
+  // interleaved: two accumulator chains, so twice the instruction-level
+  // parallelism of two mul() calls for the same instructions (for callers
+  // with few waves, where one chain's latency is exposed)
+  static ZK_HD void mul_x2(const f9& a, const f9& b, const f9& c, const f9& d, f9& r, f9& s) {
+    u32 m[9], n[9];
+    u64 acc = 0, acc2 = 0;
+#pragma unroll
+    for (int k = 0; k < 17; k++) {
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        const int j = k - i;
+        if (j >= 0 && j < 9) {
+          ZK_MAD(acc, a.l[i], b.l[j]);
+          ZK_MAD(acc2, c.l[i], d.l[j]);
+        }
+      }
+      redc_column(acc, m, r, k);
+      redc_column(acc2, n, s, k);
+    }
+    r.l[8] = (u32)acc;
+    s.l[8] = (u32)acc2;
+  }
+
   // (a*b + c*d) / R with one reduction: a column holds at most 27 products
   // of < 2^58 plus a < 2^35 carry, still below 2^64.
   static ZK_HD f9 mul2(const f9& a, const f9& b, const f9& c, const f9& d) {
@@ -185,6 +209,36 @@ struct Mont29 {
     return r;
   }
   static ZK_HD f9 sqr(const f9& a) { return sqr_plus(a, zero()); }
+  // two independent squarings with their columns interleaved (as mul_x2)
+  static ZK_HD void sqr_x2(const f9& a, const f9& c, f9& r, f9& s) {
+    u32 a2[9], c2[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      a2[i] = a.l[i] << 1;
+      c2[i] = c.l[i] << 1;
+    }
+    u32 m[9], n[9];
+    u64 acc = 0, acc2 = 0;
+#pragma unroll
+    for (int k = 0; k < 17; k++) {
+#pragma unroll
+      for (int i = 0; i < 9; i++) {
+        const int j = k - i;
+        if (j > i && j < 9) {
+          ZK_MAD(acc, a2[i], a.l[j]);
+          ZK_MAD(acc2, c2[i], c.l[j]);
+        }
+      }
+      if ((k & 1) == 0 && k / 2 < 9) {
+        ZK_MAD(acc, a.l[k / 2], a.l[k / 2]);
+        ZK_MAD(acc2, c.l[k / 2], c.l[k / 2]);
+      }
+      redc_column(acc, m, r, k);
+      redc_column(acc2, n, s, k);
+    }
+    r.l[8] = (u32)acc;
+    s.l[8] = (u32)acc2;
+  }
 
   // s - K if s >= K else s (K normalised limbs; s - K must fit below 2^261)
   static ZK_HD f9 sub_if_ge(const f9& s, const u32 (&K)[9]) {
