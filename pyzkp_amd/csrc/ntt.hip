@@ -143,7 +143,8 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
       if (s0 + t != 0) {
         const f9 u0 = (t != 0) ? Fr9::normalize(x0) : x0;
         const f9 u1 = (t != 0) ? Fr9::normalize(x1) : x1;
-        const f9 v2 = Fr9::mul(x2, w1), v3 = Fr9::mul(x3, w1);
+        f9 v2, v3;
+        Fr9::mul_x2(x2, w1, x3, w1, v2, v3);  // two chains interleaved
         A = Fr9::add_cf(u0, v2);
         B = Fr9::sub_cf(u0, v2);
         C = Fr9::add_cf(u1, v3);
