@@ -13,7 +13,11 @@
 // in [0, 2p).  R > 4p, so mul() takes inputs below 2p and returns below 2p
 // without a final subtraction.  add / sub / neg return values in [0, 2p);
 // canonical() maps to [0, p).  unpack / pack convert to and from the
-// 32-byte little-endian wire format.
+// 32-byte little-endian wire format.  The carry-free forms (add_cf, sub_cf,
+// neg2p_cf, neg_cf, sub_4p_a_2b_cf) add and subtract limb by limb, so limbs
+// may pass 29 bits while the value stays exact; their callers state the limb
+// bounds that keep every product column below 2^64.  normalize() and
+// reduce_est() bring limbs back to 29 bits.
 //
 // Domain bookkeeping: a 32-byte buffer holding x * 2^256 (the reference's
 // Montgomery form) read as an R = 2^261 value represents x * 2^-5.
