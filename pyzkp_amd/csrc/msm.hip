@@ -698,26 +698,38 @@ __global__ __launch_bounds__(256) void msm_fixup_final_kernel(const uint32_t* __
 // 6. bucket reduction, chunked: chunk t of window w covers buckets
 //    [lo, lo + K) with lo = 1 + t K; its weighted sum
 //      Sum_b b B_b = Sum_b (b - lo + 1) B_b + (lo - 1) Sum_b B_b
-//    is a running sum from the top plus one small scalar multiple.
-__global__ __launch_bounds__(128) void msm_reduce_chunks_kernel(const pt9* __restrict__ buckets,
-                                                                const uint32_t* __restrict__ row_ptr,
-                                                                uint32_t W, uint32_t B, uint32_t K,
-                                                                uint32_t cpw,
-                                                                pt9* __restrict__ partial) {
-  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= W * cpw) return;
-  const uint32_t w = gid / cpw, t = gid - w * cpw;
-  const uint32_t lo = 1 + t * K;
-  uint32_t hi = lo + K - 1;
-  if (hi > B - 1) hi = B - 1;
-  xyzz9 run = xyzz9_inf(), sum = xyzz9_inf();
-  for (uint32_t b = hi; b >= lo; b--) {
-    const uint32_t key = w * B + b;
-    if (row_ptr[key + 1] > row_ptr[key]) run = xyzz9_add(run, load_xyzz9(buckets + key));
-    sum = xyzz9_add(sum, run);
+//    is a running sum from the top plus one small scalar multiple.  A
+//    workgroup (RED_CHUNK chunks of one window: grid (ceil(cpw / RED_CHUNK),
+//    W)) then sums its chunks in an LDS tree and writes one partial, so the
+//    first level of the per-window tree needs no launch of its own.
+static constexpr uint32_t RED_CHUNK = 128;
+__global__ __launch_bounds__(RED_CHUNK) void msm_reduce_chunks_kernel(const pt9* __restrict__ buckets,
+                                                                      const uint32_t* __restrict__ row_ptr,
+                                                                      uint32_t B, uint32_t K, uint32_t cpw,
+                                                                      pt9* __restrict__ partial) {
+  __shared__ xyzz9 sh[RED_CHUNK];
+  const uint32_t w = blockIdx.y, tid = threadIdx.x;
+  const uint32_t t = blockIdx.x * RED_CHUNK + tid;
+  xyzz9 sum = xyzz9_inf();
+  if (t < cpw) {
+    const uint32_t lo = 1 + t * K;
+    uint32_t hi = lo + K - 1;
+    if (hi > B - 1) hi = B - 1;
+    xyzz9 run = xyzz9_inf();
+    for (uint32_t b = hi; b >= lo; b--) {
+      const uint32_t key = w * B + b;
+      if (row_ptr[key + 1] > row_ptr[key]) run = xyzz9_add(run, load_xyzz9(buckets + key));
+      sum = xyzz9_add(sum, run);
+    }
+    if (lo > 1) sum = xyzz9_add(sum, xyzz9_mul_small(run, lo - 1));
   }
-  if (lo > 1) sum = xyzz9_add(sum, xyzz9_mul_small(run, lo - 1));
-  store_xyzz9(partial + gid, sum);
+  sh[tid] = sum;
+  __syncthreads();
+  for (uint32_t step = RED_CHUNK >> 1; step > 0; step >>= 1) {
+    if (tid < step) sh[tid] = xyzz9_add(sh[tid], sh[tid + step]);
+    __syncthreads();
+  }
+  if (tid == 0) store_xyzz9(partial + (uint64_t)w * gridDim.x + blockIdx.x, sh[0]);
 }
 
 // 7. per-window tree sum of the chunk partials, in levels: group g of window
@@ -1042,13 +1054,14 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
   phase_end(c, "msm_fixup", st, ev);
 
   phase_begin(c, "msm_reduce", st, &ev);
-  msm_reduce_chunks_kernel<<<grid_for((uint64_t)sh.nrows * sh.cpw, 128), 128, 0, st>>>(
-      b.buckets, b.row_ptr, sh.nrows, sh.B, sh.K, sh.cpw, b.partial);
+  const uint32_t g1 = (sh.cpw + RED_CHUNK - 1) / RED_CHUNK;  // partials per window after the chunk kernel
+  msm_reduce_chunks_kernel<<<dim3(g1, sh.nrows), RED_CHUNK, 0, st>>>(b.buckets, b.row_ptr, sh.B, sh.K, sh.cpw,
+                                                                     b.partial);
   {
-    // chunk partials -> one sum per window, 64 to 1 per level
+    // workgroup partials -> one sum per window, 64 to 1 per level
     pt9* src = b.partial;
     pt9* spare[2] = {b.partial2, b.partial};
-    uint32_t m = sh.cpw;
+    uint32_t m = g1;
     for (int lvl = 0;; lvl++) {
       const uint32_t G = (m + RED_GROUP - 1) / RED_GROUP;
       pt9* dst = (G == 1) ? b.rsum : spare[lvl & 1];
