@@ -29,16 +29,111 @@ static inline F2 f2_sub(const F2& a, const F2& b) { return {zkh::sub(FQ, a.c0, b
 static inline F2 f2_dbl(const F2& a) { return f2_add(a, a); }
 static inline F2 f2_neg(const F2& a) { return {zkh::neg(FQ, a.c0), zkh::neg(FQ, a.c1)}; }
 static inline bool f2_zero(const F2& a) { return zkh::is_zero(a.c0) && zkh::is_zero(a.c1); }
-// (a0 + a1 u)(b0 + b1 u) = (a0 b0 - a1 b1) + ((a0 + a1)(b0 + b1) - a0 b0 - a1 b1) u
+// Lazy-reduction Fq2 products.  All F2 coefficients are canonical (< q).
+// The three 256 x 256-bit products of the Karatsuba form are kept as 512-bit
+// integers and combined exactly:
+//   c1 = (a0 + a1)(b0 + b1) - a0 b0 - a1 b1 = a0 b1 + a1 b0   in [0, 2 q^2)
+//   c0 = a0 b0 + q^2 - a1 b1                                  in (0, 2 q^2)
+// and only these two are Montgomery-reduced.  Both are below q 2^256
+// (2 q < 2^256), so a reduction ends below 2 q and one conditional
+// subtraction makes it canonical: two reductions and no modular additions
+// instead of three reductions and five modular additions.
+using zkh::u128;
+using zkh::u64;
+static constexpr u64 QW[4] = {0x3c208c16d87cfd47ull, 0x97816a916871ca8dull, 0xb85045b68181585dull,
+                              0x30644e72e131a029ull};
+static constexpr u64 QINV = 0x87d20782e4866389ull;  // -q^-1 mod 2^64
+static constexpr u64 QSQ[8] = {0x3b5458a2275d69b1ull, 0xa602072d09eac101ull, 0x4a50189c6d96cadcull,
+                               0x04689e957a1242c8ull, 0x26edfa5c34c6b38dull, 0xb00b855116375606ull,
+                               0x599a6f7c0348d21cull, 0x0925c4b8763cbf9cull};  // q^2
+
+// r = a b, 8 words
+static inline void mul_wide(const u64* a, const u64* b, u64* r) {
+  u64 t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma GCC unroll 4
+  for (int i = 0; i < 4; i++) {
+    u64 c = 0;
+#pragma GCC unroll 4
+    for (int j = 0; j < 4; j++) {
+      const u128 s = (u128)a[j] * b[i] + t[i + j] + c;
+      t[i + j] = (u64)s;
+      c = (u64)(s >> 64);
+    }
+    t[i + 4] = c;
+  }
+  memcpy(r, t, 64);
+}
+// a += b (8 words; no carry out by the callers' bounds)
+static inline void add8(u64* a, const u64* b) {
+  u64 c = 0;
+  for (int i = 0; i < 8; i++) {
+    const u128 s = (u128)a[i] + b[i] + c;
+    a[i] = (u64)s;
+    c = (u64)(s >> 64);
+  }
+}
+// a -= b (8 words; a >= b by the callers' bounds)
+static inline void sub8(u64* a, const u64* b) {
+  u64 br = 0;
+  for (int i = 0; i < 8; i++) {
+    const u128 d = (u128)a[i] - b[i] - br;
+    a[i] = (u64)d;
+    br = (u64)(d >> 127);
+  }
+}
+// T 2^-256 mod q for T < q 2^256 (8 words, clobbered), canonical
+static inline Fe redc(u64* t) {
+  u64 hi = 0;  // carry out of word i + 3 of the previous step, owed to word i + 4
+#pragma GCC unroll 4
+  for (int i = 0; i < 4; i++) {
+    const u64 m = t[i] * QINV;
+    u64 c = 0;
+#pragma GCC unroll 4
+    for (int j = 0; j < 4; j++) {
+      const u128 s = (u128)m * QW[j] + t[i + j] + c;
+      t[i + j] = (u64)s;
+      c = (u64)(s >> 64);
+    }
+    const u128 s = (u128)t[i + 4] + c + hi;
+    t[i + 4] = (u64)s;
+    hi = (u64)(s >> 64);
+  }
+  // (T + M q) / 2^256 < 2 q < 2^256, so hi is 0 here
+  Fe r = {{t[4], t[5], t[6], t[7]}};
+  if (zkh::geq(r.v, QW)) zkh::sub_raw(r.v, r.v, QW);
+  return r;
+}
+// a + b without reduction (< 2 q for canonical inputs)
+static inline Fe add_nr(const Fe& a, const Fe& b) {
+  Fe r;
+  u64 c = 0;
+  for (int i = 0; i < 4; i++) {
+    const u128 s = (u128)a.v[i] + b.v[i] + c;
+    r.v[i] = (u64)s;
+    c = (u64)(s >> 64);
+  }
+  return r;
+}
+// (a0 + a1 u)(b0 + b1 u) = (a0 b0 - a1 b1) + (a0 b1 + a1 b0) u
 static inline F2 f2_mul(const F2& a, const F2& b) {
-  Fe t0 = zkh::mul(FQ, a.c0, b.c0), t1 = zkh::mul(FQ, a.c1, b.c1);
-  Fe t2 = zkh::mul(FQ, zkh::add(FQ, a.c0, a.c1), zkh::add(FQ, b.c0, b.c1));
-  return {zkh::sub(FQ, t0, t1), zkh::sub(FQ, zkh::sub(FQ, t2, t0), t1)};
+  u64 t0[8], t1[8], t2[8];
+  const Fe sa = add_nr(a.c0, a.c1), sb = add_nr(b.c0, b.c1);
+  mul_wide(a.c0.v, b.c0.v, t0);
+  mul_wide(a.c1.v, b.c1.v, t1);
+  mul_wide(sa.v, sb.v, t2);
+  sub8(t2, t0);
+  sub8(t2, t1);
+  add8(t0, QSQ);
+  sub8(t0, t1);
+  return {redc(t0), redc(t2)};
 }
 static inline F2 f2_sqr(const F2& a) {
-  // (a0 + a1)(a0 - a1) + 2 a0 a1 u
-  Fe t = zkh::mul(FQ, a.c0, a.c1);
-  return {zkh::mul(FQ, zkh::add(FQ, a.c0, a.c1), zkh::sub(FQ, a.c0, a.c1)), zkh::add(FQ, t, t)};
+  // (a0 + a1)(a0 - a1) + 2 a0 a1 u; the unreduced factors are below 2 q,
+  // their products below 4 q^2 < q 2^256, which the Montgomery product
+  // takes to a canonical result
+  Fe dq;
+  zkh::sub_raw(dq.v, QW, a.c1.v);
+  return {zkh::mul(FQ, add_nr(a.c0, a.c1), add_nr(a.c0, dq)), zkh::mul(FQ, a.c0, add_nr(a.c1, a.c1))};
 }
 static inline F2 f2_inv(const F2& a) {
   // (a0 - a1 u) / (a0^2 + a1^2)
