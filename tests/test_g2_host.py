@@ -92,3 +92,30 @@ def test_g2_msm_batches_and_collisions(c):
     s = rng.randrange(R_MOD)
     expect2 = O.mul(O.G2_GENERATOR, sum(ks) * s % R_MOD)
     assert _aff(G.g2_msm(bases, [s] * n, window_bits=c, threads=4)) == expect2
+
+
+def test_g2_chord_extreme_coordinates():
+    # The Fq2 products reduce lazily (512-bit Karatsuba terms, q^2 added to
+    # keep c0 non-negative); coefficients at 0, 1 and next to q are their
+    # extreme inputs.  The chord rule (x3, y3) = (l^2 - x1 - x2,
+    # l (x1 - x3) - y1), l = (y2 - y1) / (x2 - x1), does not use the curve
+    # equation, so off-curve points with such coordinates check every
+    # product against the oracle's affine formula: through the Jacobian
+    # mixed addition (g2_add) and through the MSM's batched affine bucket
+    # additions (two unit scalars).
+    from pyzkp_amd import g2 as G
+    from oracle.field import Q_MOD
+    rng = random.Random(0x6262)
+    edge = [0, 1, 2, Q_MOD - 1, Q_MOD - 2, (Q_MOD - 1) // 2, (Q_MOD + 1) // 2, (1 << 253) - 1]
+    pick = lambda: rng.choice(edge) if rng.random() < 0.75 else rng.randrange(Q_MOD)
+    checked = 0
+    while checked < 60:
+        p = ((pick(), pick()), (pick(), pick()))
+        q = ((pick(), pick()), (pick(), pick()))
+        if p[0] == q[0] or p == ((0, 0), (0, 0)) or q == ((0, 0), (0, 0)):
+            continue
+        expect = O.add(p, q)
+        P, Q = (p[0], p[1], (1, 0)), (q[0], q[1], (1, 0))
+        assert _aff(G.g2_add(P, Q)) == expect, (p, q)
+        assert _aff(G.g2_msm([P, Q], [1, 1], window_bits=4, threads=1)) == expect, (p, q)
+        checked += 1
