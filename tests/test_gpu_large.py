@@ -1,5 +1,6 @@
 """GPU parity at the benchmark sizes: NTT 2^20 bit-exact against the C
-oracle, 2^24 by round trip and Horner rows; MSM 2^12 (c = 10) against the C
+oracle, 2^24 by round trip and Horner rows, 2^28 (the largest Fr domain) by a
+sparse input's evaluations and a dense round trip; MSM 2^12 (c = 10) against the C
 Pippenger and the closed form; 2^16 / 2^22 uniform and skewed by the closed
 form [sum s_i k_i] G1 (bases B_i = [k_i] G1)."""
 import numpy as np
@@ -104,6 +105,68 @@ def test_ntt_2_24_roundtrip_and_rows(zctx):
     assert np.array_equal(_dev_words(zctx, ev.ptr, n), a)
     for b in (src, ev, back):
         b.free()
+
+
+def test_ntt_2_28_max_size(zctx):
+    """The largest domain Fr has (2-adicity 28): 8 GiB per buffer, so element
+    byte offsets pass 2^32.  A sparse input (48 nonzero coefficients) is
+    checked at 16 outputs against direct evaluation, plain and on the coset,
+    and must come back from the in-place coset inverse with zeros elsewhere;
+    a dense uniform input must round-trip bit-exactly."""
+    log_n = 28
+    n = 1 << log_n
+    mont = 1 << 256
+    rinv = pow(mont, -1, R_MOD)
+    rng = np.random.default_rng(0x2828)
+    src = zctx.alloc(32 * n)
+    ev = zctx.alloc(32 * n)
+
+    def get(buf, i):
+        return int.from_bytes(zctx.download(buf.ptr + 32 * i, 32), "little") * rinv % R_MOD
+
+    pos = sorted(set([0, 1, n // 2, n - 1] + [int(x) for x in rng.integers(0, n, 44)]))
+    coef = {p: int.from_bytes(rng.bytes(32), "little") % R_MOD for p in pos}
+    zctx.memset_zero(src.ptr, 32 * n)
+    staged = [(p, (v * mont % R_MOD).to_bytes(32, "little")) for p, v in coef.items()]
+    for p, b in staged:
+        zctx.h2d(src.ptr + 32 * p, b, 32)
+    zctx.sync()
+
+    def ev_at(x):
+        return sum(v * pow(x, p, R_MOD) for p, v in coef.items()) % R_MOD
+
+    w = N.omega_for_size(n)
+    outs = [0, 1, 2, 3, n // 4, n // 2, n - 1] + [int(x) for x in rng.integers(0, n, 9)]
+    zctx.ntt(src.ptr, n, ev.ptr, log_n, False)
+    zctx.sync()
+    for i in outs:
+        assert get(ev, i) == ev_at(pow(w, i, R_MOD)), i
+    sh = fr_to_mont(FR_GENERATOR)
+    zctx.ntt(src.ptr, n, ev.ptr, log_n, False, shift_mont=sh)
+    zctx.sync()
+    for i in outs[:8]:
+        assert get(ev, i) == ev_at(FR_GENERATOR * pow(w, i, R_MOD) % R_MOD), i
+    zctx.ntt(ev.ptr, n, ev.ptr, log_n, True, shift_mont=sh)
+    zctx.sync()
+    for p in pos:
+        assert get(ev, p) == coef[p], p
+    assert zctx.fr_count_nonzero(ev.ptr, n) == len(pos)
+
+    # dense round trip: canonical words (top word below 2^61, so below r)
+    a = rng.integers(0, np.iinfo(np.uint64).max, size=(n, 4), dtype=np.uint64, endpoint=True)
+    a[:, 3] &= np.uint64((1 << 61) - 1)
+    zctx.h2d(src.ptr, a.ctypes.data, 32 * n)
+    zctx.sync()
+    zctx.fr_to_mont(src.ptr, src.ptr, n)
+    zctx.ntt(src.ptr, n, ev.ptr, log_n, False)
+    zctx.ntt(ev.ptr, n, ev.ptr, log_n, True)
+    zctx.fr_from_mont(ev.ptr, ev.ptr, n)
+    back = np.empty((n, 4), dtype="<u8")
+    zctx.d2h(back.ctypes.data, ev.ptr, 32 * n)
+    zctx.sync()
+    assert np.array_equal(back, a)
+    src.free()
+    ev.free()
 
 
 def test_fixed_base_kernel(zctx):
