@@ -127,6 +127,14 @@ static inline F2 f2_mul(const F2& a, const F2& b) {
   sub8(t0, t1);
   return {redc(t0), redc(t2)};
 }
+// a0^2 + a1^2 (below 2 q^2 before the one reduction)
+static inline Fe f2_norm(const F2& a) {
+  u64 t0[8], t1[8];
+  mul_wide(a.c0.v, a.c0.v, t0);
+  mul_wide(a.c1.v, a.c1.v, t1);
+  add8(t0, t1);
+  return redc(t0);
+}
 static inline F2 f2_sqr(const F2& a) {
   // (a0 + a1)(a0 - a1) + 2 a0 a1 u; the unreduced factors are below 2 q,
   // their products below 4 q^2 < q 2^256, which the Montgomery product
@@ -330,20 +338,26 @@ static J2 bucket_task(const A2* pts, const int32_t* digits, uint64_t n_all, cons
   std::vector<uint8_t> busy(nb, 0);
   uint32_t bidx[BA_BATCH];
   A2 pp[BA_BATCH];
-  F2 den[BA_BATCH], pref[BA_BATCH];
+  F2 den[BA_BATCH];
+  Fe nrm[BA_BATCH], pref[BA_BATCH];
   int cnt = 0;
 
+  // 1 / d = conj(d) / N(d) with the norm N(d) = d0^2 + d1^2 in Fq (nonzero
+  // for d != 0: -1 is not a square mod q), so the simultaneous inversion
+  // runs its prefix chain over the norms in Fq, not over the d in Fq2
   auto flush = [&]() {
     if (cnt == 0) return;
-    F2 acc = f2_one();
+    Fe acc = zkh::one(FQ);
     for (int e = 0; e < cnt; e++) {
+      nrm[e] = f2_norm(den[e]);
       pref[e] = acc;
-      acc = f2_mul(acc, den[e]);
+      acc = zkh::mul(FQ, acc, nrm[e]);
     }
-    F2 inv = f2_inv(acc);
+    Fe inv = zkh::inv(FQ, acc);
     for (int e = cnt - 1; e >= 0; e--) {
-      const F2 ie = f2_mul(inv, pref[e]);
-      inv = f2_mul(inv, den[e]);
+      const Fe ni = zkh::mul(FQ, inv, pref[e]);
+      inv = zkh::mul(FQ, inv, nrm[e]);
+      const F2 ie = {zkh::mul(FQ, den[e].c0, ni), zkh::neg(FQ, zkh::mul(FQ, den[e].c1, ni))};
       A2& B = bk[bidx[e]];
       const F2 lam = f2_mul(f2_sub(pp[e].y, B.y), ie);
       const F2 x3 = f2_sub(f2_sub(f2_sqr(lam), B.x), pp[e].x);
