@@ -106,8 +106,10 @@ def g2_msm_words(bases: "G2Bases", scalar_words: np.ndarray, n: int, window_bits
     """Same, with canonical scalars as (n, 4) uint64 words."""
     if n > bases.n:
         raise ValueError("points and scalars length mismatch")
-    out = np.zeros(16, dtype="<u8")
     s = np.ascontiguousarray(scalar_words, dtype="<u8")
+    if s.ndim != 2 or s.shape[1] != 4 or s.shape[0] < n:
+        raise ValueError(f"scalar words must have shape (>= {n}, 4), got {tuple(s.shape)}")
+    out = np.zeros(16, dtype="<u8")
     rc = load_library().zk_g2_msm_host(bases.words.ctypes.data, s.ctypes.data, n, window_bits, threads,
                                        out.ctypes.data)
     if rc:

@@ -104,6 +104,28 @@ def save_key(path: str, pk: ProvingKey, circ: SyntheticCircuit) -> dict:
     return meta
 
 
+def _check_arrays(meta: dict, arr: dict, circ: SyntheticCircuit) -> int:
+    """Every array the device or the host MSMs will read must have the shape
+    the circuit implies; returns the number of G1 query points."""
+    def need(cond: bool, what: str) -> None:
+        if not cond:
+            raise CacheMismatchError(f"setup cache arrays mismatch: {what}")
+
+    n, nv, na = circ.n, circ.n_vars, len(circ.aux_ids)
+    total = 2 * nv + (n - 1) + na
+    need(int(meta.get("n", -1)) == n and int(meta.get("n_vars", -1)) == nv, "domain or variable count")
+    need(int(meta.get("query_total", -1)) == total, "query_total")
+    need(list(meta.get("query_offsets", [])) == [0, nv, 2 * nv, 2 * nv + n - 1], "query_offsets")
+    need([int(i) for i in meta.get("public_ids", [])] == [int(i) for i in circ.public_ids], "public_ids")
+    for name, shape in (("query_points", (8 * total,)), ("b_query_g2", (nv, 16)), ("g1", (3, 2, 4)),
+                        ("g2", (3, 16)), ("ic", (len(circ.public_ids), 2, 4)), ("trapdoor", (5, 4))):
+        need(name in arr and tuple(arr[name].shape) == shape, name)
+    need("aux_ids" in arr and np.array_equal(arr["aux_ids"].astype(np.int64),
+                                            np.asarray(circ.aux_ids, dtype=np.int64)), "aux_ids")
+    need(len(meta.get("public_names", [])) == len(circ.public_ids), "public_names")
+    return total
+
+
 def load_key(path: str, circ: SyntheticCircuit, ctx, window_bits: int = 0) -> ProvingKey:
     """Reads a key written by save_key for this circuit and rebuilds its
     device state; CacheMismatchError when the file is for something else."""
@@ -122,12 +144,14 @@ def load_key(path: str, circ: SyntheticCircuit, ctx, window_bits: int = 0) -> Pr
         if str(meta.get("ir_fingerprint")) != circuit_fingerprint(circ):
             raise CacheMismatchError("setup cache circuit mismatch")
         arr = {k: z[k] for k in z.files}
-    total = int(meta["query_total"])
+    total = _check_arrays(meta, arr, circ)
     pts = ctx.upload(arr["query_points"].tobytes() or b"\0" * 64)
     fixed = ctx.fixed_bases_create(pts.ptr, total, window_bits)
-    csr_host = {name: (arr[f"{name}_row_ptr"], arr[f"{name}_cols"], arr[f"{name}_coeffs"]) for name in "ABC"}
+    # the matrices come from the circuit the fingerprint was checked against,
+    # never from the file
+    csr_host = {name: (m.row_ptr, m.cols, m.coeffs) for name, m in (("A", circ.A), ("B", circ.B), ("C", circ.C))}
     csr = {name: _upload_csr(ctx, CSR(*v)) for name, v in csr_host.items()}
-    aux = arr["aux_ids"]
+    aux = np.ascontiguousarray(circ.aux_ids, dtype="<u4")
     g1 = [_g1_from(w) for w in arr["g1"]]
     g2 = [G2.point_from_words(w) for w in arr["g2"]]
     vk = VerifyingKey(alpha_g1=g1[0], beta_g2=g2[0], gamma_g2=g2[1], delta_g2=g2[2],

@@ -27,34 +27,41 @@ from .types import Proof, ProvingKey
 
 
 class _Work:
-    """Per-key device buffers for k in-flight proofs."""
+    """Device buffers for k in-flight proofs of one key.  The workspace is
+    stored on the key it was sized for (`ProvingKey.work`), records that
+    key's shape and context, and is freed with the key."""
 
     def __init__(self, pk: ProvingKey, k: int) -> None:
         ctx = pk.ctx
         n, nv, na = pk.n, pk.n_vars, len(pk.aux_ids)
+        self.ctx = ctx
+        self.n = n
         self.seg = nv + n + na              # scalars of one proof: [w | h | l]
         self.k = k
         self.S = ctx.alloc(32 * self.seg * k)
         self.abc = ctx.alloc(32 * 3 * n)
         self.tmp = ctx.alloc(32 * n)
 
+    def fits(self, pk: ProvingKey, k: int) -> bool:
+        return (self.S.ptr != 0 and self.ctx is pk.ctx and self.n == pk.n and self.k >= k
+                and self.seg == pk.n_vars + pk.n + len(pk.aux_ids))
+
     def free(self) -> None:
         for b in (self.S, self.abc, self.tmp):
             b.free()
 
 
-_WORK: dict = {}
-
-
 def _work(pk: ProvingKey, k: int) -> _Work:
-    """Per-key workspace with room for at least k proofs (grown, never shrunk)."""
-    w = _WORK.get(id(pk))
-    if w is None or w.S.ptr == 0 or w.k < k:
+    """The key's workspace with room for at least k proofs (grown, never
+    shrunk)."""
+    w = pk.work
+    if w is None or not w.fits(pk, k):
         if w is not None:
-            pk.ctx.sync()
+            w.ctx.sync()
             w.free()
-        w = _Work(pk, k)
-        _WORK[id(pk)] = w
+            pk.work = None
+        w = _Work(pk, max(k, w.k if w is not None else 0))
+        pk.work = w
     return w
 
 
@@ -64,8 +71,9 @@ def segment_bytes(pk: ProvingKey) -> int:
 
 
 def release_work(pk: ProvingKey) -> None:
-    w = _WORK.pop(id(pk), None)
+    w, pk.work = pk.work, None
     if w is not None:
+        w.ctx.sync()
         w.free()
 
 
@@ -73,8 +81,7 @@ def _device_part(pk: ProvingKey, wk: _Work, slot: int, w_words: np.ndarray) -> N
     """Fills scalars segment `slot` with [w | h | w[aux]] (Montgomery)."""
     ctx = pk.ctx
     n, nv = pk.n, pk.n_vars
-    if w_words.shape[0] != nv:
-        raise ValueError("witness length mismatch")
+    check_witness(pk, w_words)
     S = wk.S.ptr + 32 * wk.seg * slot
     wbuf = np.ascontiguousarray(w_words, dtype="<u8")   # alive until the synchronising check below
     ctx.h2d(S, wbuf.ctypes.data, 32 * nv)
@@ -133,6 +140,14 @@ def _norm(p):
     return from_affine(normalize(p))
 
 
+def check_witness(pk: ProvingKey, w_words: np.ndarray) -> None:
+    """A witness is (n_vars, 4) canonical words; anything else is refused
+    before any host or device work reads it."""
+    shape = getattr(w_words, "shape", None)
+    if shape is None or len(shape) != 2 or shape[1] != 4 or shape[0] != pk.n_vars:
+        raise ValueError(f"witness must have shape ({pk.n_vars}, 4), got {shape}")
+
+
 def g2_part(pk: ProvingKey, w_words: np.ndarray, threads: int = 0):
     """b_query_g2 . w on host threads (the reference's MSM_G2 on CPU)."""
     return G2.g2_msm_words(pk.b_query_g2, w_words, pk.n_vars, threads=threads)
@@ -166,6 +181,8 @@ def prove_many(pk: ProvingKey, witnesses: Sequence[np.ndarray], rs: Sequence[Tup
     k = len(witnesses)
     if k == 0:
         return []
+    for w in witnesses:
+        check_witness(pk, w)
     g2_res: List = [None] * k
 
     def g2_job():

@@ -67,6 +67,10 @@ class ProvingKey:
     hbm_bytes: int = 0
 
     def free(self) -> None:
+        if self.work is not None:         # the prover's per-key workspace
+            self.work.ctx.sync()
+            self.work.free()
+            self.work = None
         if self.fixed is not None:
             self.fixed.free()
             self.fixed = None
@@ -92,3 +96,4 @@ class ProvingKey:
 
     query_total: int = 0
     csr_host: Any = None                  # dict name -> (row_ptr, cols, coeffs) host arrays
+    work: Any = field(default=None, repr=False)   # prover workspace sized for this key

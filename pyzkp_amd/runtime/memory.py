@@ -39,6 +39,7 @@ class MemoryPool:
         self._owned_dev_ids: set = set()
         self.device_stats = PoolStats()
         self._staging: Dict[int, Tuple[int, int]] = {}  # ctx id -> (pinned ptr, bytes)
+        self._staging_ctx: Dict[int, Any] = {}          # ctx id -> ctx (to free the area)
 
     # ---- host lists (reference runtime/memory.py:34-71)
     def alloc_fr(self, n: int) -> List[int]:
@@ -94,14 +95,21 @@ class MemoryPool:
             size = max(int(nbytes), 1 << 20)
             ptr = ctx.pinned_alloc(size)
             self._staging[id(ctx)] = (ptr, size)
+            self._staging_ctx[id(ctx)] = ctx
             return ptr
         return cur[0]
 
     def close(self) -> None:
+        """Frees the pooled device buffers and the pinned staging areas."""
         for lst in self._free_dev.values():
             for b in lst:
                 b.free()
         self._free_dev.clear()
+        for ctx_id, (ptr, _) in list(self._staging.items()):
+            ctx = self._staging_ctx.pop(ctx_id)
+            ctx.sync()
+            ctx.pinned_free(ptr)
+        self._staging.clear()
 
 
 # reference alias (runtime/memory.py:112-113)

@@ -165,3 +165,59 @@ def test_hip_context_requires_device():
         pytest.skip("a HIP device is present")
     with pytest.raises(RuntimeError):
         HipContext.create_default()
+
+
+def test_graph_unchanged_by_runs_and_repeatable_with_transfers():
+    log = []
+    r = _fake_registry(log)
+    g = Graph()
+    g.add_buffer(id="x", device=Device.CPU, dtype=DType.FR, data=[1, 2, 3])
+    g.add_node(op=OpType.ROOTS_EVALS_FROM_COEFFS, inputs=["x"], outputs=["y"])
+    g.add_node(op=OpType.BATCH_INV, inputs=["y"], outputs=["z"])
+    before = g.analyze()
+    nodes = list(g.nodes)
+    ctx = DeviceContext(backend=Backend.HIP, device=Device.HIP)
+
+    def reset(i, graph):
+        graph.reset_to_initial(graph.analyze_cached())
+
+    Executor(registry=r).run_repeated(g, 2, context=ctx, keep=["z"], before_each=reset)
+    assert g.nodes == nodes                          # inputs never rewritten to the copies
+    assert g.buffers["z"].data == [4, 6, 8] and set(g.buffers) == {"x", "z"}
+    assert log.count(("to_device", "x")) == 2
+    after = g.analyze()                              # outputs of a run are not taken for sources
+    assert after.initial == before.initial == ("x",) and after.topo_order == before.topo_order
+
+
+def test_insertion_order_kept_when_legal():
+    g = Graph()
+    g.add_buffer(id="a", device=Device.CPU, dtype=DType.FR, data=[1])
+    g.add_node(op=OpType.POLY_SUB, inputs=["a"], outputs=["p"])
+    g.add_node(op=OpType.POLY_SUB, inputs=["a"], outputs=["q"])
+    g.add_node(op=OpType.POLY_SUB, inputs=["q", "p"], outputs=["r"])
+    g.add_node(op=OpType.POLY_SUB, inputs=["a"], outputs=["s"])
+    assert g.analyze().topo_order == (0, 1, 2, 3)
+    g.add_node(op=OpType.POLY_SUB, inputs=["r"], outputs=["p"])
+    with pytest.raises(ValueError):                  # p defined twice
+        g.analyze()
+
+
+def test_no_kept_list_releases_nothing():
+    log = []
+    r = _fake_registry(log)
+    g = Graph()
+    g.add_buffer(id="x", device=Device.CPU, dtype=DType.FR, data=[1])
+    g.add_node(op=OpType.BATCH_INV, inputs=["x"], outputs=["y"])
+    g.add_node(op=OpType.BATCH_INV, inputs=["y"], outputs=["z"])
+    Executor(registry=r).run(g, context=DeviceContext(backend=Backend.HIP, device=Device.HIP))
+    assert set(g.buffers) == {"x", "y", "z"}
+
+
+def test_kernel_without_outputs_is_an_error():
+    r = KernelRegistry()
+    r.register(OpType.BATCH_INV, Device.CPU, lambda ctx: {}, backend=Backend.CPU)
+    g = Graph()
+    g.add_buffer(id="x", device=Device.CPU, dtype=DType.FR, data=[1])
+    g.add_node(op=OpType.BATCH_INV, inputs=["x"], outputs=["y"])
+    with pytest.raises(ValueError):
+        Executor(registry=r).run(g)
