@@ -128,6 +128,19 @@ int zk_ntt_batch(zk_ctx* ctx, const void* in, void* out, uint32_t log_n, uint32_
 int zk_msm_g1(zk_ctx* ctx, const void* bases, const void* scalars, uint64_t n, int window_bits,
               uint64_t out_jac_mont[12], void* stream);
 
+/* zk_msm_g1 without any device allocation: the same result, on a 64 MiB
+ * arena that is a file-scope device array (one per GPU, held until the
+ * process ends) instead of the context's growable scratch.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Shapes whose arena exceeds 64 MiB are refused with
+ * ZK_ERR_INVALID (n up to 2^15 at the default window, 2^16 at windows up
+ * to 14).  Calls on one
+ * GPU share the arena, so calls that overlap on one GPU race; the caller
+ * serialises them. */
+int zk_msm_g1_noalloc(zk_ctx* ctx, const void* bases, const void* scalars, uint64_t n, int window_bits,
+                      uint64_t out_jac_mont[12], void* stream);
+
 /* Fixed-base scalar multiplication of one G1 point P by n scalars:
  * out[i] = [scalars[i]] P as 64-byte affine Montgomery (x = y = 0 for the
  * identity).  `table` holds T[j][d] = [d * 2^(8j)] P for j < 32, d < 256

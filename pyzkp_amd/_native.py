@@ -48,6 +48,7 @@ SIGNATURES = {
     "zk_ntt": [_vp, _vp, ct.c_uint64, _vp, ct.c_uint32, ct.c_int, _u64p, _u64p, _vp],
     "zk_ntt_batch": [_vp, _vp, _vp, ct.c_uint32, ct.c_uint32, ct.c_int, _u64p, _u64p, _vp],
     "zk_msm_g1": [_vp, _vp, _vp, ct.c_uint64, ct.c_int, _u64p, _vp],
+    "zk_msm_g1_noalloc": [_vp, _vp, _vp, ct.c_uint64, ct.c_int, _u64p, _vp],
     "zk_g1_fixed_base": [_vp, _vp, _vp, ct.c_uint64, _vp, _vp],
     "zk_fixed_bases_create": [_vp, _vp, ct.c_uint64, ct.c_int, ct.POINTER(_vp), _vp],
     "zk_fixed_bases_destroy": [_vp, _vp],

@@ -154,6 +154,13 @@ int zk_init(int ordinal, zk_ctx** out) {
     delete ctx;
     return ZK_ERR_RUNTIME;
   }
+  // device address of the no-allocation MSM arena, looked up once per device
+  if (int rc = msm_noalloc_init(&ctx->c)) {
+    g_init_err = ctx->c.err;
+    hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+    return rc;
+  }
   *out = ctx;
   return ZK_OK;
 }
@@ -317,6 +324,15 @@ int zk_msm_g1(zk_ctx* ctx, const void* bases, const void* scalars, uint64_t n, i
   if (n && (!bases || !scalars)) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
   return msm_g1_run(&ctx->c, (const fe*)bases, (const fe*)scalars, n, window_bits, out_jac_mont,
                     pick(ctx, stream));
+}
+
+int zk_msm_g1_noalloc(zk_ctx* ctx, const void* bases, const void* scalars, uint64_t n, int window_bits,
+                      uint64_t out_jac_mont[12], void* stream) {
+  CTX_CHECK(ctx);
+  if (!out_jac_mont) return set_err(&ctx->c, ZK_ERR_INVALID, "null output");
+  if (n && (!bases || !scalars)) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  return msm_g1_noalloc_run(&ctx->c, (const fe*)bases, (const fe*)scalars, n, window_bits, out_jac_mont,
+                            pick(ctx, stream));
 }
 
 int zk_g1_fixed_base(zk_ctx* ctx, const void* table, const void* scalars, uint64_t n, void* out,

@@ -136,6 +136,10 @@ int batch_run(Ctx* c, BatchPlan* p, uint64_t* out_jac, hipStream_t st);
 void batch_destroy(BatchPlan* p);
 int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
                uint64_t out_jac[12], hipStream_t st);
+// the same MSM on a file-scope device arena: no device allocation
+int msm_noalloc_init(Ctx* c);
+int msm_g1_noalloc_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
+                       uint64_t out_jac[12], hipStream_t st);
 
 // fixed-base scalar multiplication --------------------------------------------
 int g1_fixed_base_run(Ctx* c, const fe* table, const fe* scalars, uint64_t n, fe* out, hipStream_t st);

@@ -15,6 +15,8 @@
 // sorted (bucket, point) list instead of a whole bucket (so skewed buckets
 // do not serialise a thread), and the bucket reduction is split into chunks
 // whose weighted partial sums are recombined in parallel.
+#include <stdio.h>
+#include <mutex>
 #include "internal.hpp"
 #include "curve.hpp"
 #include "curve29.hpp"
@@ -1098,16 +1100,128 @@ static void export_jacobian(const xyzz& acc, uint64_t out[12]) {
 // neighbour.  On MI355X at 2^22 this measured slower than one group (the
 // accumulation occupies every CU, so little overlaps, and every group pays
 // the reduction's latency), so the default is one group.
+// One launch sequence over all W windows (the default shape).  `arena`
+// null: the context's growable scratch; else the caller's arena of
+// `arena_bytes`, which must hold the layout (nothing is allocated then).
+// `bases9` non-null: bases already in the R = 2^261 form (a prepared set),
+// so the per-call conversion is skipped.
+static int msm_g1_single(Ctx* c, const fe* bases, const fe* bases9_ready, const fe* scalars, uint64_t n, int cb,
+                         uint64_t out_jac[12], hipStream_t st, char* arena, size_t arena_bytes) {
+  const uint32_t W = (255 + cb - 1) / cb;
+  const uint32_t nt = (uint32_t)((n + SORT_TILE - 1) / SORT_TILE);
+  std::vector<MsmRow> rows(W);
+  for (uint32_t w = 0; w < W; w++) {
+    MsmRow& r = rows[w];
+    r.off = (uint64_t)w * n;
+    r.len = (uint32_t)n;
+    r.inner = (uint32_t)n;
+    r.stride = 0;
+    r.ntiles = nt;
+    r.tile0 = w * nt;
+    r.boff = 0;
+  }
+  MsmShape sh;
+  int rc = make_shape(c, cb, W, rows, sh);
+  if (rc) return rc;
+  const size_t o_bases = align256((size_t)W * n * 4);
+  const size_t o_rest = bases9_ready ? o_bases : align256(o_bases + (size_t)n * 64);
+  uint32_t dummy = 0;
+  const size_t need = align256(o_rest + layout(sh, nullptr, nullptr, &dummy));
+  char* base = arena;
+  if (!base) {
+    void* p;
+    rc = ensure_scratch(c, 2, need, &p);
+    if (rc) return rc;
+    base = (char*)p;
+  } else if (need > arena_bytes) {
+    char msg[160];
+    snprintf(msg, sizeof(msg), "MSM of n = %llu at window %d needs %zu scratch bytes, above the %zu available",
+             (unsigned long long)n, cb, need, arena_bytes);
+    return set_err(c, ZK_ERR_INVALID, msg);
+  }
+  uint32_t* digits = (uint32_t*)base;
+  const fe* bases9 = bases9_ready ? bases9_ready : (const fe*)(base + o_bases);
+  MsmBuffers b;
+  layout(sh, base + o_rest, &b, digits);
+  ZK_HIP(c, hipMemcpyAsync(b.rows, rows.data(), rows.size() * sizeof(MsmRow), hipMemcpyHostToDevice, st));
+
+  hipEvent_t ev;
+  phase_begin(c, "msm_digits", st, &ev);
+  if (!bases9_ready) msm_bases_convert_kernel<<<grid_for(n, 256), 256, 0, st>>>(bases, n, (fe*)bases9);
+  msm_digits_count_kernel<<<nt, DIGITS_COUNT_THREADS, (size_t)W * sh.NC * 4, st>>>(
+      scalars, n, (uint32_t)cb, W, sh.S, sh.NC, nt, digits, b.hist);
+  ZK_HIP(c, hipGetLastError());
+  phase_end(c, "msm_digits", st, ev);
+  rc = run_rows(c, sh, b, bases9, st, nullptr, true);
+  if (rc) return rc;
+  std::vector<xyzz> hs(W);
+  ZK_HIP(c, hipMemcpyAsync(hs.data(), b.rexp, (size_t)W * sizeof(xyzz), hipMemcpyDeviceToHost, st));
+  ZK_HIP(c, hipStreamSynchronize(st));
+  g1_window_combine(hs.data(), W, cb, out_jac);
+  return ZK_OK;
+}
+
+static int msm_check_args(Ctx* c, uint64_t n, int window_bits, int* cb) {
+  if (n >= (1ull << 31)) return set_err(c, ZK_ERR_INVALID, "MSM size must be below 2^31");
+  *cb = window_bits > 0 ? window_bits : default_window_bits(n);
+  if (*cb < 2 || *cb > 24) return set_err(c, ZK_ERR_INVALID, "window_bits must be in [2, 24]");
+  return ZK_OK;
+}
+
+// ------------------------------------------------------------ no-allocation MSM
+// `scratch` is the MSM arena of zk_msm_g1_noalloc, for callers that cannot
+// allocate device memory (code under HIP graph capture, tight serving loops,
+// harnesses that forbid hipMalloc).  Each GPU the library runs on holds its
+// own copy for the life of the process, at its own address, looked up once
+// per device by msm_noalloc_init.  Calls on one GPU share it and keep the
+// last call's contents; nothing in it needs clearing, because every word a
+// launch sequence reads was written earlier in the same sequence.  Calls
+// that overlap on one GPU race.
+static constexpr size_t MSM_NOALLOC_BYTES = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[MSM_NOALLOC_BYTES];
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static char* g_noalloc_ptr[NOALLOC_MAX_DEVICES];  // device address of `scratch`, per device
+
+int msm_noalloc_init(Ctx* c) {
+  if (c->device < 0 || c->device >= NOALLOC_MAX_DEVICES)
+    return set_err(c, ZK_ERR_INVALID, "device ordinal beyond the no-allocation table");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_ptr[c->device]) return ZK_OK;
+  void* p = nullptr;
+  ZK_HIP(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) return set_err(c, ZK_ERR_RUNTIME, "no device address for the MSM scratch array");
+  g_noalloc_ptr[c->device] = (char*)p;
+  return ZK_OK;
+}
+
+int msm_g1_noalloc_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
+                       uint64_t out_jac[12], hipStream_t st) {
+  for (int i = 0; i < 12; i++) out_jac[i] = 0;
+  if (n == 0) return ZK_OK;
+  int cb;
+  int rc = msm_check_args(c, n, window_bits, &cb);
+  if (rc) return rc;
+  char* arena;
+  {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    arena = (c->device >= 0 && c->device < NOALLOC_MAX_DEVICES) ? g_noalloc_ptr[c->device] : nullptr;
+  }
+  if (!arena) return set_err(c, ZK_ERR_RUNTIME, "MSM scratch array not set up for this device");
+  return msm_g1_single(c, bases, nullptr, scalars, n, cb, out_jac, st, arena, MSM_NOALLOC_BYTES);
+}
+
 int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
                uint64_t out_jac[12], hipStream_t st) {
   for (int i = 0; i < 12; i++) out_jac[i] = 0;
   if (n == 0) return ZK_OK;
-  if (n >= (1ull << 31)) return set_err(c, ZK_ERR_INVALID, "MSM size must be below 2^31");
-  const int cb = window_bits > 0 ? window_bits : default_window_bits(n);
-  if (cb < 2 || cb > 24) return set_err(c, ZK_ERR_INVALID, "window_bits must be in [2, 24]");
+  int cb;
+  int rc0 = msm_check_args(c, n, window_bits, &cb);
+  if (rc0) return rc0;
   const uint32_t W = (255 + cb - 1) / cb;
   uint32_t G = c->msm_groups ? c->msm_groups : 1u;
   if (G > W) G = W;
+  if (G == 1) return msm_g1_single(c, bases, nullptr, scalars, n, cb, out_jac, st, nullptr, 0);
   std::vector<uint32_t> wb(G + 1);
   for (uint32_t g = 0; g <= G; g++) wb[g] = (uint32_t)((uint64_t)g * W / G);
   // rows = windows: row w holds digits[w n .. w n + n), base index = position
