@@ -142,6 +142,33 @@ def prove(pk, A, B, C, w: Sequence[int], r: int, s: int):
     return {"a": a, "b": b, "c": c, "h": h}
 
 
+def exponent_key(A, B, C, n: int, n_vars: int, public_ids, trapdoor):
+    """The part of `setup`'s key that check_in_exponent reads (the QAP
+    columns at tau), without the group elements: u_i(tau) = sum_j A[j][i]
+    L_j(tau) with the Lagrange basis L_j(tau) = w^j (tau^n - 1) / (n (tau -
+    w^j)) of the domain, since u_i interpolates column i on the domain.
+    Pinned against `setup` (iNTT then Horner) on small circuits."""
+    tau, alpha, beta, gamma, delta = trapdoor
+    w = N.omega_for_size(n)
+    zt = (pow(tau, n, R_MOD) - 1) % R_MOD
+    n_inv = pow(n, -1, R_MOD)
+    lag, x = [], 1
+    for _ in range(n):
+        lag.append(x * zt % R_MOD * n_inv % R_MOD * pow((tau - x) % R_MOD, -1, R_MOD) % R_MOD)
+        x = x * w % R_MOD
+    cols = []
+    for rows in (A, B, C):
+        acc = [0] * n_vars
+        for j, row in enumerate(rows):
+            for var, c in row.items():
+                acc[var] = (acc[var] + int(c) * lag[j]) % R_MOD
+        cols.append(acc)
+    a_tau, b_tau, c_tau = cols
+    k_tau = [(beta * a + alpha * b + c) % R_MOD for a, b, c in zip(a_tau, b_tau, c_tau)]
+    return {"trapdoor": trapdoor, "a_tau": a_tau, "b_tau": b_tau, "k_tau": k_tau, "public_ids": list(public_ids),
+            "n": n}
+
+
 def check_in_exponent(pk, w: Sequence[int], r: int, s: int, proof, n: int) -> bool:
     tau, alpha, beta, gamma, delta = pk["trapdoor"]
     a_tau, b_tau, k_tau = pk["a_tau"], pk["b_tau"], pk["k_tau"]

@@ -27,12 +27,12 @@ from .types import Proof, ProvingKey
 
 
 class _Work:
-    """Device buffers for k in-flight proofs of one key.  The workspace is
-    stored on the key it was sized for (`ProvingKey.work`), records that
-    key's shape and context, and is freed with the key."""
+    """Device buffers for k in-flight proofs of one key on one context.  A
+    workspace records the key shape and the context it was sized for, and
+    its owner (the key itself, or a serving slot) frees it."""
 
-    def __init__(self, pk: ProvingKey, k: int) -> None:
-        ctx = pk.ctx
+    def __init__(self, pk: ProvingKey, k: int, ctx=None) -> None:
+        ctx = ctx if ctx is not None else pk.ctx
         n, nv, na = pk.n, pk.n_vars, len(pk.aux_ids)
         self.ctx = ctx
         self.n = n
@@ -42,8 +42,13 @@ class _Work:
         self.abc = ctx.alloc(32 * 3 * n)
         self.tmp = ctx.alloc(32 * n)
 
-    def fits(self, pk: ProvingKey, k: int) -> bool:
-        return (self.S.ptr != 0 and self.ctx is pk.ctx and self.n == pk.n and self.k >= k
+    @property
+    def nbytes(self) -> int:
+        return 32 * (self.seg * self.k + 4 * self.n)
+
+    def fits(self, pk: ProvingKey, k: int, ctx=None) -> bool:
+        ctx = ctx if ctx is not None else pk.ctx
+        return (self.S.ptr != 0 and self.ctx is ctx and self.n == pk.n and self.k >= k
                 and self.seg == pk.n_vars + pk.n + len(pk.aux_ids))
 
     def free(self) -> None:
@@ -51,18 +56,25 @@ class _Work:
             b.free()
 
 
+def grow_work(pk: ProvingKey, w, k: int, ctx=None) -> _Work:
+    """`w` if it holds k proofs of this key on ctx, else a fresh workspace
+    (the old one freed once ctx is idle)."""
+    if w is not None and w.fits(pk, k, ctx):
+        return w
+    kk = max(k, w.k if w is not None else 0)
+    if w is not None:
+        w.ctx.sync()
+        w.free()
+    return _Work(pk, kk, ctx)
+
+
 def _work(pk: ProvingKey, k: int) -> _Work:
-    """The key's workspace with room for at least k proofs (grown, never
+    """The key's own workspace with room for at least k proofs (grown, never
     shrunk)."""
-    w = pk.work
-    if w is None or not w.fits(pk, k):
-        if w is not None:
-            w.ctx.sync()
-            w.free()
-            pk.work = None
-        w = _Work(pk, max(k, w.k if w is not None else 0))
-        pk.work = w
-    return w
+    if pk.work is None or not pk.work.fits(pk, k):
+        old, pk.work = pk.work, None
+        pk.work = grow_work(pk, old, k)
+    return pk.work
 
 
 def segment_bytes(pk: ProvingKey) -> int:
@@ -78,8 +90,9 @@ def release_work(pk: ProvingKey) -> None:
 
 
 def _device_part(pk: ProvingKey, wk: _Work, slot: int, w_words: np.ndarray) -> None:
-    """Fills scalars segment `slot` with [w | h | w[aux]] (Montgomery)."""
-    ctx = pk.ctx
+    """Fills scalars segment `slot` with [w | h | w[aux]] (Montgomery), on
+    the workspace's context (its stream and arenas)."""
+    ctx = wk.ctx
     n, nv = pk.n, pk.n_vars
     check_witness(pk, w_words)
     S = wk.S.ptr + 32 * wk.seg * slot
@@ -153,18 +166,23 @@ def g2_part(pk: ProvingKey, w_words: np.ndarray, threads: int = 0):
     return G2.g2_msm_words(pk.b_query_g2, w_words, pk.n_vars, threads=threads)
 
 
-def device_batch(pk: ProvingKey, witnesses: Sequence[np.ndarray]) -> List[tuple]:
+def device_batch(pk: ProvingKey, witnesses: Sequence[np.ndarray], work: _Work = None) -> List[tuple]:
     """Device part of several proofs of one key: per proof the H pipeline,
-    then one fixed-base batch for all their G1 MSMs.  Returns per proof
-    (a_lin, b_lin_g1, h_acc, l_acc)."""
+    then one fixed-base batch for all their G1 MSMs.  Runs on `work`'s
+    context (default: the key's own workspace on the key's context), so
+    batches of different keys can run concurrently from different host
+    threads on different contexts.  Returns per proof (a_lin, b_lin_g1,
+    h_acc, l_acc)."""
     k = len(witnesses)
     if k == 0:
         return []
-    wk = _work(pk, k)
+    wk = work if work is not None else _work(pk, k)
+    if not wk.fits(pk, k, wk.ctx):
+        raise ValueError("workspace too small for this batch")
     for i, w in enumerate(witnesses):
         _device_part(pk, wk, i, w)
     offs, sizes, boffs = _msm_entries(pk, wk, range(k))
-    res = pk.ctx.msm_g1_fixed_batch(pk.fixed, wk.S.ptr, offs, sizes, boffs)
+    res = wk.ctx.msm_g1_fixed_batch(pk.fixed, wk.S.ptr, offs, sizes, boffs)
     return [tuple(from_jacobian_mont(*res[4 * i + j]) for j in range(4)) for i in range(k)]
 
 

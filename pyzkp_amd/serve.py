@@ -1,36 +1,46 @@
 """Continuous-batching Groth16 prove server on one GPU.
 
 Proving keys of several circuits stay resident on the device (query
-fixed-base tables, R1CS matrices, NTT tables).  Requests (circuit, witness,
-r, s) arrive over time; the scheduler loop
+fixed-base tables, R1CS matrices); each device slot keeps its own NTT tables.
+Requests (circuit, witness, r, s) arrive over time.
 
-  * admits queued requests in arrival order while the device working sets
-    of the batch fit the HBM budget and a batch slot is free (a request
-    whose own working set exceeds the budget is rejected with ValueError
-    before the run starts);
-  * queues each admitted request's G2 MSM on a host pool at once (shortest
-    job first; each MSM uses all host threads, the workers share the cores);
-  * groups the batch by circuit and runs, per group, the device H
-    pipelines back to back and ONE fixed-base launch for all their G1
-    MSMs;
-  * hands each request's G1 results to its pool task, which assembles the
-    proof as soon as its G2 MSM is done, while the scheduler already
-    admits the next batch.
+  * Admission.  A request is checked on arrival (known circuit, witness
+    shape, working set within the HBM budget); its G2 MSM is queued at once
+    on a host pool (shortest job first), since it needs nothing from the
+    device.
+  * Device slots.  The server owns `device_slots` contexts on the GPU, each
+    with its own HIP stream and scratch arenas and one host thread.  A slot
+    runs one group at a time: up to `max_batch` requests of ONE circuit,
+    whose H pipelines run back to back and whose G1 MSMs run as one
+    fixed-base launch.  Different circuits' groups run on different slots
+    at the same time, so their device work overlaps.
+  * When a group is cut.  The G2 MSM on the host is the longest stage of a
+    proof, so a request's device work is not on the critical path until its
+    G2 MSM starts.  The pending requests of a circuit therefore form a group
+    when the first of them reaches a G2 worker, or when max_batch of them
+    wait, and a slot and HBM are free: under light load every group holds
+    one request and starts as soon as its G2 MSM does; under load requests
+    wait for G2 workers anyway, and the device batches them meanwhile.
+  * HBM accounting.  The working sets of the groups on the slots (scalar
+    segments plus the per-group a, b, c and scratch vectors) never exceed
+    the budget; a group waits for memory to free up.
+  * Completion.  When a request's G1 parts (device) and G2 part (host) are
+    both ready, a small assembler pool forms the proof.
 
-So the device work of new requests overlaps the host G2 work of earlier
-ones, and a small request is never queued behind a large one's G2 MSM.
-`serve_plain` is the baseline: one proof at a time in arrival order.  The
-reference has no serving layer; its closest pattern is prove_batch
-(protocols/groth16/prove.py:133-248), sequential re-use of one key.
+A witness that fails its R1CS fails only its own request (its group is
+re-run one request at a time).  `serve_plain` is the baseline: `prove()`
+once per request in arrival order.  The reference has no serving layer;
+its closest pattern is prove_batch (protocols/groth16/prove.py:133-248),
+sequential re-use of one key.
 """
 from __future__ import annotations
 
 import heapq
 import importlib
+import queue
 import threading
 import time
-from collections import deque
-from concurrent.futures import Future
+from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass
 from typing import Dict, List, Optional
 
@@ -57,7 +67,7 @@ class Result:
     id: int
     proof: Optional[Proof]
     arrival: float
-    admitted: float = 0.0
+    admitted: float = 0.0       # when its device group started
     done: float = 0.0
     error: Optional[str] = None
 
@@ -69,7 +79,7 @@ class Result:
 class _HostPool:
     """Host workers for the G2 MSMs, shortest job first: a queued job with
     fewer bases runs before a larger one, so a small proof does not wait for
-    the G2 MSMs of large proofs admitted before it."""
+    the G2 MSMs of large proofs that arrived before it."""
 
     def __init__(self, workers: int) -> None:
         self._heap: list = []
@@ -111,27 +121,94 @@ class _HostPool:
 
 def working_set_bytes(pk: ProvingKey) -> int:
     """Device bytes one in-flight proof of this key needs: its scalar segment
-    [w | h | l] plus the per-batch a, b, c and scratch vectors (counted per
+    [w | h | l] plus the per-group a, b, c and scratch vectors (counted per
     request, an upper bound)."""
     return P.segment_bytes(pk) + 32 * 4 * pk.n
 
 
+class _Slot:
+    """One device context (own stream, arenas and NTT tables) with its host
+    thread and its per-circuit workspaces."""
+
+    def __init__(self, index: int, ctx, owns_ctx: bool) -> None:
+        self.index = index
+        self.ctx = ctx
+        self.owns_ctx = owns_ctx
+        self.work: Dict[object, object] = {}    # circuit -> prover workspace on this context
+        self._q: "queue.Queue" = queue.Queue()
+        self._thread = threading.Thread(target=self._loop, daemon=True)
+        self._thread.start()
+
+    def workspace(self, key, pk: ProvingKey, k: int):
+        self.work[key] = P.grow_work(pk, self.work.get(key), k, self.ctx)
+        return self.work[key]
+
+    def workspace_bytes(self) -> int:
+        return sum(w.nbytes for w in self.work.values())
+
+    def submit(self, fn, *args) -> None:
+        self._q.put((fn, args))
+
+    def _loop(self) -> None:
+        while True:
+            job = self._q.get()
+            if job is None:
+                return
+            fn, args = job
+            fn(self, *args)
+
+    def close(self) -> None:
+        self._q.put(None)
+        self._thread.join()
+        self.ctx.sync()
+        for w in self.work.values():
+            w.free()
+        self.work.clear()
+        if self.owns_ctx:
+            self.ctx.close()
+
+
 class ProveServer:
     def __init__(self, keys: Dict[object, ProvingKey], hbm_budget: int, max_batch: int = 16,
-                 g2_workers: int = 2, cpu_threads: int = 16) -> None:
+                 g2_workers: int = 2, cpu_threads: int = 16, device_slots: int = 2,
+                 assemblers: int = 2, slot_contexts: Optional[list] = None) -> None:
+        """`slot_contexts`: contexts for the device slots (owned by the
+        caller); by default each slot opens a context of its own on the
+        keys' device."""
+        if device_slots < 1 or max_batch < 1:
+            raise ValueError("device_slots and max_batch must be at least 1")
         self.keys = keys
         self.budget = int(hbm_budget)
         self.max_batch = int(max_batch)
-        self.pool = _HostPool(g2_workers)
         self.cpu_threads = cpu_threads
-        self.stats = {"batches": 0, "msm_launches": 0, "max_batch_seen": 0}
+        self.slots: List[_Slot] = []
+        if slot_contexts is not None:
+            self.slots = [_Slot(i, c, False) for i, c in enumerate(slot_contexts)]
+        else:
+            ords = {pk.ctx.ordinal for pk in keys.values()}
+            if len(ords) != 1:
+                raise ValueError("the proving keys must all live on one device")
+            from ._native import Context
+            ordinal = next(iter(ords))
+            self.slots = [_Slot(i, Context(ordinal), True) for i in range(device_slots)]
+        if not self.slots:
+            raise ValueError("the server needs at least one device slot")
+        self.pool = _HostPool(g2_workers)
+        self.assembler = ThreadPoolExecutor(max_workers=assemblers)
+        self.stats = {"groups": 0, "batches": 0, "msm_launches": 0, "max_batch_seen": 0,
+                      "max_concurrent_groups": 0, "peak_inflight_bytes": 0, "device_slots": len(self.slots)}
 
     def close(self) -> None:
         self.pool.shutdown()
+        self.assembler.shutdown(wait=True)
+        for s in self.slots:
+            s.close()
+        self.slots = []
 
     def footprint(self, req: Request) -> int:
         return working_set_bytes(self.keys[req.key])
 
+    # ------------------------------------------------------------------ run
     def run(self, requests: List[Request]) -> List[Result]:
         """Feeds `requests` at their arrival times, serves them and returns
         the results in request order."""
@@ -142,95 +219,179 @@ class ProveServer:
             if self.footprint(q) > self.budget:
                 raise ValueError(f"request {q.id} needs {self.footprint(q)} device bytes, above the HBM budget "
                                  f"{self.budget}")
-        results: Dict[int, Result] = {}
-        res_lock = threading.Lock()
-        queue: deque = deque()
-        cv = threading.Condition()
-        t0 = time.perf_counter()
-        now = lambda: time.perf_counter() - t0
+        return _Run(self, reqs).serve(requests)
 
-        def feeder():
-            for q in reqs:
-                delay = q.arrival - now()
-                if delay > 0:
-                    time.sleep(delay)
-                with cv:
-                    queue.append(q)
-                    cv.notify()
 
-        def finish(q: Request, admitted: float, handoff: dict, ready: threading.Event):
-            pk = self.keys[q.key]
-            err = None
-            proof = None
-            try:
-                b2 = P.g2_part(pk, q.witness, self.cpu_threads)
-                ready.wait()
-                if "error" in handoff:
-                    err = handoff["error"]
-                else:
-                    proof = P.assemble(pk, handoff["g1"], b2, q.r, q.s)
-            except Exception as e:          # recorded per request, the server keeps going
-                err = f"{type(e).__name__}: {e}"
-            with res_lock:
-                results[q.id] = Result(q.id, proof, q.arrival, admitted, now(), err)
+class _Run:
+    """State of one ProveServer.run call."""
 
-        ft = threading.Thread(target=feeder, daemon=True)
-        ft.start()
-        futures = []
-        pending: List[tuple] = []
+    def __init__(self, srv: ProveServer, reqs: List[Request]) -> None:
+        self.srv = srv
+        self.reqs = reqs
+        self.cv = threading.Condition()
+        self.pending: List[Request] = []        # arrived, device group not formed yet
+        self.g2_started: set = set()            # ids whose G2 MSM is running or done
+        self.free_slots = list(srv.slots)
+        self.inflight = 0                       # working-set bytes of the groups on the slots
+        self.busy = 0
+        self.arrived = 0
+        self.results: Dict[int, Result] = {}
+        self.parts: Dict[int, dict] = {}        # id -> {"g1": ..., "g2": ..., "err": ...}
+        self.admitted: Dict[int, float] = {}
+        self.t0 = time.perf_counter()
+
+    def now(self) -> float:
+        return time.perf_counter() - self.t0
+
+    # ---- arrival and host stages
+    def _feed(self) -> None:
+        for q in self.reqs:
+            delay = q.arrival - self.now()
+            if delay > 0:
+                time.sleep(delay)
+            self._arrive(q)
+
+    def _arrive(self, q: Request) -> None:
+        pk = self.srv.keys[q.key]
         try:
-            self._schedule(reqs, queue, cv, now, finish, futures, pending)
-        finally:
-            # an exception in the scheduler must not leave pool tasks waiting
-            for hand, ev in pending:
-                if not ev.is_set():
-                    hand.setdefault("error", "server aborted")
-                    ev.set()
-        for f in futures:
-            f.result()
-        ft.join()
-        return [results[q.id] for q in requests]
+            P.check_witness(pk, q.witness)
+        except ValueError as e:
+            with self.cv:
+                self.arrived += 1
+                self._finish(q, None, str(e))
+                self.cv.notify_all()
+            return
+        with self.cv:
+            self.parts[q.id] = {}
+            self.pending.append(q)
+            self.arrived += 1
+            self.cv.notify_all()
+        self.srv.pool.submit(pk.n_vars, self._g2_job, q)
 
-    def _schedule(self, reqs, queue, cv, now, finish, futures, pending) -> None:
-        admitted_count = 0
-        while admitted_count < len(reqs):
-            with cv:
-                while not queue:
-                    cv.wait()
-                batch, used = [], 0
-                while queue and len(batch) < self.max_batch:
-                    fp = self.footprint(queue[0])
-                    if batch and used + fp > self.budget:
-                        break
-                    batch.append(queue.popleft())
-                    used += fp
-            t_adm = now()
-            admitted_count += len(batch)
-            self.stats["batches"] += 1
-            self.stats["max_batch_seen"] = max(self.stats["max_batch_seen"], len(batch))
-            handoffs = {q.id: ({}, threading.Event()) for q in batch}
-            pending.extend(handoffs.values())
-            for q in batch:
-                futures.append(self.pool.submit(self.keys[q.key].n_vars, finish, q, t_adm, *handoffs[q.id]))
-            groups: Dict[object, List[Request]] = {}
-            for q in batch:
-                groups.setdefault(q.key, []).append(q)
-            for key, qs in groups.items():
-                try:
-                    parts = P.device_batch(self.keys[key], [q.witness for q in qs])
-                    self.stats["msm_launches"] += 1
-                    for q, part in zip(qs, parts):
-                        handoffs[q.id][0]["g1"] = part
-                except ValueError:
-                    # one bad witness fails its whole group launch: redo the group one by one
-                    for q in qs:
-                        try:
-                            handoffs[q.id][0]["g1"] = P.device_batch(self.keys[key], [q.witness])[0]
-                            self.stats["msm_launches"] += 1
-                        except ValueError as e:
-                            handoffs[q.id][0]["error"] = str(e)
-                for q in qs:
-                    handoffs[q.id][1].set()
+    def _g2_job(self, q: Request) -> None:
+        with self.cv:
+            self.g2_started.add(q.id)
+            self.cv.notify_all()
+        try:
+            out = ("g2", P.g2_part(self.srv.keys[q.key], q.witness, self.srv.cpu_threads))
+        except Exception as e:                  # recorded per request
+            out = ("err", f"{type(e).__name__}: {e}")
+        self._deliver(q, *out)
+
+    def _deliver(self, q: Request, kind: str, value) -> None:
+        with self.cv:
+            part = self.parts[q.id]
+            part[kind] = value
+            complete = "err" in part or ("g1" in part and "g2" in part)
+            if complete and not part.get("assembling"):
+                part["assembling"] = True
+            else:
+                return
+        self.srv.assembler.submit(self._assemble, q)
+
+    def _assemble(self, q: Request) -> None:
+        part = self.parts[q.id]
+        proof, err = None, part.get("err")
+        if err is None:
+            try:
+                proof = P.assemble(self.srv.keys[q.key], part["g1"], part["g2"], q.r, q.s)
+            except Exception as e:
+                err = f"{type(e).__name__}: {e}"
+        with self.cv:
+            self._finish(q, proof, err)
+            self.cv.notify_all()
+
+    def _finish(self, q: Request, proof, err) -> None:
+        self.results[q.id] = Result(q.id, proof, q.arrival, self.admitted.get(q.id, 0.0), self.now(), err)
+
+    # ---- device groups
+    def _ready_group(self) -> Optional[List[Request]]:
+        """The next group to start, or None (called with the lock held)."""
+        if not self.free_slots or not self.pending:
+            return None
+        srv = self.srv
+        by_key: Dict[object, List[Request]] = {}
+        for q in self.pending:                  # arrival order within a circuit
+            by_key.setdefault(q.key, []).append(q)
+        for key, qs in sorted(by_key.items(), key=lambda kv: kv[1][0].arrival):
+            due = len(qs) >= srv.max_batch or any(q.id in self.g2_started for q in qs)
+            if not due:
+                continue
+            group, used = [], self.inflight
+            for q in qs[:srv.max_batch]:
+                fp = srv.footprint(q)
+                if used + fp > srv.budget:
+                    break
+                group.append(q)
+                used += fp
+            if group:
+                return group
+        return None
+
+    def _launch(self, group: List[Request]) -> None:
+        """Starts `group` on a free slot (lock held)."""
+        srv = self.srv
+        slot = self.free_slots.pop()
+        ids = {q.id for q in group}
+        self.pending = [q for q in self.pending if q.id not in ids]
+        nbytes = sum(srv.footprint(q) for q in group)
+        self.inflight += nbytes
+        self.busy += 1
+        t = self.now()
+        for q in group:
+            self.admitted[q.id] = t
+        st = srv.stats
+        st["groups"] += 1
+        st["batches"] = st["groups"]
+        st["max_batch_seen"] = max(st["max_batch_seen"], len(group))
+        st["max_concurrent_groups"] = max(st["max_concurrent_groups"], self.busy)
+        st["peak_inflight_bytes"] = max(st["peak_inflight_bytes"], self.inflight)
+        slot.submit(self._device_group, group, nbytes)
+
+    def _device_group(self, slot: _Slot, group: List[Request], nbytes: int) -> None:
+        """Runs on the slot's thread: the group's device work on its context."""
+        key = group[0].key
+        pk = self.srv.keys[key]
+        outcome: Dict[int, tuple] = {}
+        launches = 0
+        try:
+            try:
+                parts = P.device_batch(pk, [q.witness for q in group], slot.workspace(key, pk, len(group)))
+                launches += 1
+                outcome = {q.id: ("g1", part) for q, part in zip(group, parts)}
+            except ValueError:
+                # a bad witness fails the group's check: isolate it
+                for q in group:
+                    try:
+                        part = P.device_batch(pk, [q.witness], slot.workspace(key, pk, 1))[0]
+                        outcome[q.id] = ("g1", part)
+                        launches += 1
+                    except ValueError as e:
+                        outcome[q.id] = ("err", str(e))
+        except Exception as e:                  # device failure: every request of the group fails
+            outcome = {q.id: ("err", f"{type(e).__name__}: {e}") for q in group}
+        with self.cv:
+            self.srv.stats["msm_launches"] += launches
+            self.inflight -= nbytes
+            self.busy -= 1
+            self.free_slots.append(slot)
+            self.cv.notify_all()
+        for q in group:
+            self._deliver(q, *outcome[q.id])
+
+    def serve(self, requests: List[Request]) -> List[Result]:
+        feeder = threading.Thread(target=self._feed, daemon=True)
+        feeder.start()
+        total = len(self.reqs)
+        with self.cv:
+            while len(self.results) < total:
+                group = self._ready_group()
+                if group is not None:
+                    self._launch(group)
+                    continue
+                self.cv.wait(timeout=0.5)
+        feeder.join()
+        return [self.results[q.id] for q in requests]
 
 
 def serve_plain(keys: Dict[object, ProvingKey], requests: List[Request], cpu_threads: int = 16) -> List[Result]:

@@ -120,3 +120,36 @@ def device_bases(ctx, ks_words: np.ndarray):
     ctx.sync()
     kbuf.free()
     return out
+
+
+# ---------------------------------------------------------------- batched MSM + NTT workload
+def batch_workload(ctx, srs_log: int = 18, count: int = 64, log_lo: int = 10, ntt_lo: int = 12,
+                   ntt_hi: int = 18) -> dict:
+    """The batched workload: `count` MSMs with n_j = floor(2^U) points, U ~
+    U[log_lo, srs_log] (seed 0x4001), over one shared 2^srs_log-point SRS
+    (bases [k_i] G1, k seed 0x4003; scalars seed 0x4004, MSM j reads scalars
+    [offs_j, offs_j + n_j)), plus `count` forward NTTs with log2 size uniform
+    in ntt_lo..ntt_hi (seed 0x4002), grouped by size into batched jobs (the
+    inputs of size 2^lg from seed 0x4005 + lg).  Device buffers are in
+    `buffers` (the caller frees them)."""
+    nb = 1 << srs_log
+    u = uniform01(0x4001, count)
+    sizes = [int(2.0 ** (log_lo + (srs_log - log_lo) * x)) for x in u]
+    offs = [int(v) for v in np.cumsum([0] + sizes[:-1])]
+    ks = fr_uniform(0x4003, nb)
+    ss = fr_uniform(0x4004, sum(sizes))
+    bases = device_bases(ctx, ks)
+    scal = device_fr_mont(ctx, ss)
+    span = ntt_hi - ntt_lo + 1
+    logs = [ntt_lo + int(v) % span for v in splitmix_block(0x4002, 0, count)]
+    jobs, inputs, bufs = [], {}, [bases, scal]
+    for lg in sorted(set(logs)):
+        cnt = logs.count(lg)
+        a = fr_uniform(0x4005 + lg, cnt << lg)
+        src = device_fr_mont(ctx, a)
+        dst = ctx.alloc(32 * (cnt << lg))
+        bufs += [src, dst]
+        inputs[lg] = a
+        jobs.append({"in": src.ptr, "out": dst.ptr, "log_n": lg, "batch": cnt, "inverse": False})
+    return {"srs_n": nb, "ks": ks, "ss": ss, "sizes": sizes, "offs": offs, "bases": bases, "scalars": scal,
+            "ntt_logs": logs, "ntt_jobs": jobs, "ntt_inputs": inputs, "buffers": bufs}

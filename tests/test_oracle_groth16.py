@@ -31,3 +31,19 @@ def test_oracle_groth16_exponent_check():
     pf2 = dict(pf)
     pf2["c"] = OG.C1.aff_add(pf["c"], OG.C1.G1_GENERATOR)
     assert not OG.check_in_exponent(pk, w, 11111, 22222, pf2, circ.n)
+
+
+def test_exponent_key_equals_setup_columns():
+    """The Lagrange-at-tau route (exponent_key) and setup's iNTT-then-Horner
+    route give the same QAP columns at tau, and the proof checks against
+    either key."""
+    circ, w = _tiny()
+    A, B, C = circ.A.row_dicts(), circ.B.row_dicts(), circ.C.row_dicts()
+    td = (98765, 4321, 55555, 77777, 99991)
+    full = OG.setup(A, B, C, circ.n, circ.n_vars, circ.public_ids, list(circ.aux_ids), td)
+    ek = OG.exponent_key(A, B, C, circ.n, circ.n_vars, circ.public_ids, td)
+    for f in ("a_tau", "b_tau", "k_tau"):
+        assert ek[f] == full[f], f
+    pf = OG.prove(full, A, B, C, w, 5, 6)
+    assert OG.check_in_exponent(ek, w, 5, 6, pf, circ.n)
+    assert not OG.check_in_exponent(ek, w, 5, 7, pf, circ.n)
