@@ -4,18 +4,25 @@
 Default (``python bench.py``): one step = one BN254 G1 MSM over n = 2^22
 device-resident points (uniform Fr scalars, bases B_i = [k_i] G1 built on
 the device, seeds 0x3001 / 0x3002), result returned to the host.  The
-window size c is autotuned in 13..18 before the warm-up.  The line also
-carries the Fr NTT at 2^24 (forward, inverse, coset forward, coset inverse;
-GB/s = 64 N / t), the per-phase device times, the VALU roofline of the
-dominant kernel (bucket accumulation) against the measured
-v_mad_u64_u32 issue rate, and a CPU baseline: the C restatement of the
-reference's Pippenger, single-threaded, on a 2^18-point prefix of the same
-inputs, checked bit-exact against the GPU result on that prefix.
+window size c is autotuned in 13..18 before the warm-up.  The line carries
+the VALU roofline of the dominant kernel (bucket accumulation) and a CPU
+baseline (the C restatement of the reference's Pippenger, one thread on a
+2^18-point prefix and all host threads on the full input, both checked
+against the GPU result), and in `secondary` the two other legs of the
+metric:
 
-``--workload ntt`` reports the NTT line instead (value = GB/s at 2^24);
+* ``ntt_2^24``: the Fr NTT at 2^24 in four variants (GB/s = 64 N / t), with
+  its VALU and HBM rooflines and the C restatement of the reference NTT as
+  CPU baseline;
+* ``served_groth16``: the served Groth16 prover over 2^10..2^20-constraint
+  requests: capacity, a sweep of Poisson arrival rates served by the
+  continuous-batching server and by the plain one-proof-at-a-time loop, and
+  the highest rate each sustains with p99 latency within the objective;
+  every proof verified and equal to the plain loop's byte for byte.
+
+``--workload ntt`` / ``serve`` print those legs as the line;
 ``--workload batch`` the batched fixed-base MSMs + NTTs replayed as one
-hipGraph; ``--workload serve`` the served Groth16 prover (proofs/s and
-p50/p99 latency against a plain one-proof-at-a-time loop).
+hipGraph; ``plonkq`` the PLONK quotient; ``copy`` the counter calibration.
 Prints exactly one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
@@ -48,7 +55,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["msm", "ntt", "batch", "serve", "copy", "plonkq"], default="msm")
-    ap.add_argument("--requests", type=int, default=48, help="serve: number of proof requests")
+    ap.add_argument("--requests", type=int, default=64, help="serve: number of proof requests")
     ap.add_argument("--serve-log", type=str, default="10,20", help="serve: constraint range 2^lo..2^hi")
     ap.add_argument("--cpu-threads", type=int, default=16, help="serve: host threads for G2 MSMs")
     ap.add_argument("--hbm-budget-gb", type=float, default=16.0, help="serve: device budget for working sets")
@@ -60,6 +67,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-log", type=int, default=18)
     ap.add_argument("--no-ntt", action="store_true")
+    ap.add_argument("--no-serve", action="store_true", help="MSM line: leave out the served-prover leg")
+    ap.add_argument("--device-slots", type=int, default=2, help="serve: concurrent device slots")
+    ap.add_argument("--loads", type=str, default="0.6,0.8,1.0,1.15,1.3,1.5",
+                    help="serve: Poisson arrival rates as fractions of the plain loop's capacity")
+    ap.add_argument("--slo-ms", type=float, default=750.0,
+                    help="serve: p99 latency objective (about twice a 2^20-constraint proof's plain service time)")
     return ap.parse_args()
 
 
@@ -174,57 +187,100 @@ def ntt_block(ctx, log_n, steps, warmup):
     return res
 
 
+def ntt_leg(ctx, log_n: int, steps: int, warmup: int, probe_mads: float, with_cpu: bool,
+            cpu_threads: int = 16) -> dict:
+    """Fr NTT at 2^log_n (config 2): the four variants timed, the VALU and
+    HBM rooflines of the forward transform, and the CPU baseline (the C
+    restatement of crypto/poly/ntt.py, forward at 2^min(22, log_n), one
+    thread and all host threads)."""
+    from pyzkp_amd import workload
+    from pyzkp_amd.field import omega_for_size
+
+    r = ntt_block(ctx, log_n, steps, warmup)
+    n = 1 << log_n
+    fwd = r["fwd"]
+    dev_s = fwd["device_ms"] / 1e3
+    mads = (n // 2) * log_n * FQ_MUL_MADS
+    algo_bytes = 64 * n
+    leg = {
+        "metric": f"Fr NTT GB/s @2^{log_n}", "value": fwd["GBps"], "unit": "GB/s", "ms": fwd["ms"],
+        "workload": f"Fr NTT forward n=2^{log_n}, uniform coefficients seed {'0x2001' if log_n == 20 else '0x2002'}",
+        "variants": r,
+        "roofline": {"bound": "valu", "achieved": round(mads / dev_s / 1e12, 3), "peak": round(VALU_PEAK / 1e12, 3),
+                     "unit": "T lane-mads/s", "frac": round(mads / dev_s / VALU_PEAK, 4),
+                     "peak_source": f"256 CUs x 64 lanes x 2.4 GHz; probe {probe_mads / 1e12:.2f} T/s sustained",
+                     "work_per_launch": f"(n/2) log2 n = {(n // 2) * log_n} butterflies x 136 v_mad_u64_u32",
+                     "kernel": "ntt_pass_kernel (all passes of one transform)",
+                     "hbm": {"achieved": round(algo_bytes / dev_s / 1e9, 1), "peak": 8000.0, "unit": "GB/s",
+                             "frac": round(algo_bytes / dev_s / 8e12, 4),
+                             "algorithmic_bytes": algo_bytes},
+                     "traffic": measured_traffic(f"ntt_2^{log_n}", "ntt_pass_kernel")},
+        "cpu_baseline": None,
+    }
+    if with_cpu:
+        from oracle import cref
+        nc = 1 << min(22, log_n)
+        lgc = nc.bit_length() - 1
+        wds = workload.fr_uniform(0x2001, nc)
+        t0 = time.perf_counter()
+        cref.ntt_words(wds, nc, lgc, omega_for_size(nc), False, None, 1)
+        c1 = time.perf_counter() - t0
+        threads = max(1, min(cpu_threads, host_cpu_info()["usable_cores"] or 1))
+        t0 = time.perf_counter()
+        cref.ntt_words(wds, nc, lgc, omega_for_size(nc), False, None, threads)
+        cm = time.perf_counter() - t0
+        leg["cpu_baseline"] = {"value": round(64 * nc / c1 / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+                               "sample": f"C restatement of crypto/poly/ntt.py (bit-reverse radix-2 DIT), forward "
+                                         f"n=2^{lgc}, 1 thread, {c1:.2f} s",
+                               "all_cores": {"value": round(64 * nc / cm / 1e9, 3), "unit": "GB/s",
+                                             "cores": threads, "sample": f"the same, {threads} threads, {cm:.2f} s"}}
+    return leg
+
+
 def run_batch(args, ctx, peak_mads):
     """Config 4: 64 fixed-base MSMs, n_j = floor(2^U), U ~ U[10, 18] (seed
     0x4001), over one shared 2^18-point SRS, plus 64 forward NTTs with log2
-    size uniform in 12..18 (seed 0x4002); one step = one hipGraph replay of
-    the whole batch.  Compared with the same work issued one call at a time."""
-    import numpy as np
+    size uniform in 12..18 (seed 0x4002) grouped by size into batched jobs;
+    one step = one hipGraph replay of the whole batch.  Compared with the
+    same plan run eagerly and with the work issued one call at a time.  The
+    roofline: the bucket accumulation's VALU rate (its device time from the
+    eager plan's HIP events) and the whole step's."""
     from pyzkp_amd import workload
-    from pyzkp_amd.curve import G1_GENERATOR, from_jacobian_mont, g1_eq, g1_mul
-    from pyzkp_amd.field import FR_MODULUS, words_to_ints
+    from pyzkp_amd.curve import from_jacobian_mont
+    from pyzkp_amd.field import words_to_ints
     from pyzkp_amd.tune import Tuner
+    from oracle import msm as M
 
-    nb = 1 << 18
-    u = workload.uniform01(0x4001, 64)
-    sizes = [int(2.0 ** (10 + 8 * x)) for x in u]
-    offs = [int(v) for v in np.cumsum([0] + sizes[:-1])]
-    tot = sum(sizes)
+    wl = workload.batch_workload(ctx)
+    sizes, offs, jobs = wl["sizes"], wl["offs"], wl["ntt_jobs"]
+    nb, tot, ss = wl["srs_n"], sum(wl["sizes"]), wl["ss"]
     log("batch: SRS 2^18, 64 MSMs (%d points), 64 NTTs" % tot)
-    ks = workload.fr_uniform(0x4003, nb)
-    bases = workload.device_bases(ctx, ks)
-    ss = workload.fr_uniform(0x4004, tot)
-    scal = workload.device_fr_mont(ctx, ss)
-    logs = [12 + int(v) % 7 for v in workload.splitmix_block(0x4002, 0, 64)]
-    groups = {}
-    for lg in logs:
-        groups[lg] = groups.get(lg, 0) + 1
-    jobs, keep = [], []
-    for lg, cnt in sorted(groups.items()):
-        a = workload.fr_uniform(0x4005 + lg, cnt << lg)
-        src = workload.device_fr_mont(ctx, a)
-        dst = ctx.alloc(32 * (cnt << lg))
-        keep += [src, dst]
-        jobs.append({"in": src.ptr, "out": dst.ptr, "log_n": lg, "batch": cnt, "inverse": False})
-    ntt_elems = sum(cnt << lg for lg, cnt in groups.items())
-
+    ntt_elems = sum(j["batch"] << j["log_n"] for j in jobs)
     tuner = Tuner(ctx, use_cache=False)
-    c, fb = tuner.fixed_window(bases.ptr, nb, scal.ptr, offs, sizes, cands=range(11, 18))
+    c, fb = tuner.fixed_window(wl["bases"].ptr, nb, wl["scalars"].ptr, offs, sizes, cands=range(11, 18))
     log(f"  fixed-base window c={c}: {tuner.log}")
     plans = {}
+    acc_ms = None
     for mode in ("graph", "eager"):
-        plan = ctx.batch_create(fb, scal.ptr, offs, sizes, jobs, use_graph=(mode == "graph"))
+        plan = ctx.batch_create(fb, wl["scalars"].ptr, offs, sizes, jobs, use_graph=(mode == "graph"))
         for _ in range(args.warmup):
             res = plan.run()
+        if mode == "eager":
+            ctx.profile_reset()
+            ctx.profile_enable(True)
         t0 = time.perf_counter()
         for _ in range(args.steps):
             res = plan.run()
         plans[mode] = ((time.perf_counter() - t0) / args.steps, res)
+        if mode == "eager":
+            ctx.profile_enable(False)
+            ms, cnt = ctx.profile_get("msm_accumulate")
+            acc_ms = ms / max(args.steps, 1)
         plan.free()
-    # the same work one call at a time (plain per-MSM Pippenger + per-NTT calls)
+
     def plain():
         for o, m in zip(offs, sizes):
-            ctx.msm_g1(bases.ptr, scal.ptr + 32 * o, m, 0)
+            ctx.msm_g1(wl["bases"].ptr, wl["scalars"].ptr + 32 * o, m, 0)
         for j in jobs:
             n = 1 << j["log_n"]
             for b in range(j["batch"]):
@@ -236,36 +292,49 @@ def run_batch(args, ctx, peak_mads):
     t_plain = time.perf_counter() - t0
 
     log("closed-form check of 64 MSMs")
-    kk = words_to_ints(ks)
-    sv = words_to_ints(ss)
+    kk, sv = words_to_ints(wl["ks"]), words_to_ints(ss)
     ok = True
     for j, (o, m) in enumerate(zip(offs, sizes)):
-        t = sum(a * b for a, b in zip(kk[:m], sv[o:o + m])) % FR_MODULUS
-        ok &= g1_eq(from_jacobian_mont(*plans["graph"][1][j]), g1_mul(G1_GENERATOR, t))
+        p = from_jacobian_mont(*plans["graph"][1][j])
+        ok &= (None if p[2] == 0 else (p[0], p[1])) == M.msm_closed_form(kk[:m], sv[o:o + m])
     step = plans["graph"][0]
+    # algorithmic VALU work: every nonzero signed digit of every window is one
+    # mixed addition (fixed-base tables: one bucket set for all windows)
+    pairs = sum(signed_digit_pairs(ss[o:o + m], c) for o, m in zip(offs, sizes))
+    msm_mads = pairs * MADD_FQ_MULS * FQ_MUL_MADS
+    ntt_mads = sum(j["batch"] * ((1 << j["log_n"]) // 2) * j["log_n"] for j in jobs) * FQ_MUL_MADS
+    acc_rate = msm_mads / (acc_ms / 1e3) if acc_ms else None
     line = {
         "metric": "batched fixed-base G1 MSM pts/s (64 MSMs, shared 2^18 SRS) + 64 NTTs, one hipGraph",
         "value": round(tot / step, 1), "unit": "pts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(step * 1e3, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "u32x8 Montgomery Fq/Fr (254-bit integer)", "data": "synthetic",
         "config": {"workload": "64 MSMs n=floor(2^U), U~U[10,18] (seed 0x4001) over one 2^18 SRS (fixed-base "
-                               "tables) + 64 forward NTTs 2^12..2^18 (seed 0x4002)",
-                   "msm_points": tot, "ntt_elements": ntt_elems, "fixed_base_window": c,
+                               "tables) + 64 forward NTTs 2^12..2^18 (seed 0x4002), grouped by size",
+                   "msm_points": tot, "ntt_elements": ntt_elems, "ntt_jobs": len(jobs), "fixed_base_window": c,
                    "tune_ms": tuner.log, "graph_ms": round(plans["graph"][0] * 1e3, 4),
                    "eager_ms": round(plans["eager"][0] * 1e3, 4), "one_call_at_a_time_ms": round(t_plain * 1e3, 4),
                    "speedup_vs_one_at_a_time": round(t_plain / step, 3), "verified_closed_form": bool(ok)},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": {"bound": "valu", "achieved": round(acc_rate / 1e12, 3) if acc_rate else None,
+                     "peak": round(VALU_PEAK / 1e12, 3), "unit": "T lane-mads/s",
+                     "frac": round(acc_rate / VALU_PEAK, 4) if acc_rate else None,
+                     "kernel": "msm_accumulate_kernel (device time from the eager plan's HIP events)",
+                     "accumulate_ms": round(acc_ms, 4) if acc_ms else None,
+                     "work_per_launch": f"nonzero signed digits ({pairs}) * 10 Fq-mul * 136 = {msm_mads:.3e}",
+                     "step_frac": round((msm_mads + ntt_mads) / step / VALU_PEAK, 4),
+                     "step_work": f"MSM {msm_mads:.3e} + NTT {ntt_mads:.3e} v_mad_u64_u32 per replay",
+                     "traffic": measured_traffic("batch", "msm_accumulate_kernel")},
+        "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        m = nb
-        rate, dt, _ = cpu_msm_baseline(ctx, bases.ptr, ss, m)
+        rate, dt, _ = cpu_msm_baseline(ctx, wl["bases"].ptr, ss, nb)
         line["cpu_baseline"] = {"value": round(rate, 1), "unit": "pts/s", "cores": 1, "kind": "port",
                                 "sample": f"C restatement of crypto/msm/pippenger.py, one MSM over the whole "
                                           f"2^18-point SRS with the batch's first 2^18 scalars, 1 thread, "
                                           f"{dt:.2f} s"}
     print(json.dumps(line))
     fb.free()
-    for b in keep + [bases, scal]:
+    for b in wl["buffers"]:
         b.free()
 
 
@@ -360,34 +429,55 @@ def run_plonkq(args, ctx):
         b.free()
 
 
-def run_serve(args, ctx):
-    """Served Groth16: Poisson arrivals (seed 0x5001) of requests whose circuit
-    has floor(2^U) constraints, U ~ U[lo, hi] (seed 0x5002), padded to 2^k;
-    one synthetic circuit and proving key per size class; witness seed
-    0x5100 + id, (r, s) from seed 0x5200 + id.  Capacity: all requests
-    queued at t = 0, served by the plain one-at-a-time loop and by the
-    continuous-batching server (value = the server's proofs/s).  Latency:
-    Poisson arrivals at `load` times the plain loop's capacity, served by
-    both.  Every proof is verified with the pairing and the server's proofs
-    must equal the plain loop's byte for byte."""
-    import numpy as np
-    from pyzkp_amd.groth16 import prove, setup, synthetic_circuit, synthetic_witness, verify
-    from pyzkp_amd.groth16.prove import release_work
+def oracle_key_from_device(pk, circ):
+    """The oracle's proving-key dict for `pk`, read back from the device key
+    (its G1 queries equal the oracle's setup point for point, which the GPU
+    tests check), so the CPU prover can be timed without the oracle's slow
+    setup."""
+    from pyzkp_amd.curve import normalize
+    from pyzkp_amd.g2 import normalize as normalize2, point_from_words
+    vk = pk.vk
+    return {"a_query": [normalize(p) for p in pk.g1_query("a")],
+            "b_query_g1": [normalize(p) for p in pk.g1_query("b_g1")],
+            "b_query_g2": [normalize2(point_from_words(pk.b_query_g2.words[i])) for i in range(circ.n_vars)],
+            "h_query": [normalize(p) for p in pk.g1_query("h")],
+            "l_query": [normalize(p) for p in pk.g1_query("l")],
+            "alpha_g1": normalize(vk.alpha_g1), "beta_g1": normalize(pk.beta_g1),
+            "beta_g2": normalize2(vk.beta_g2), "delta_g1": normalize(pk.delta_g1),
+            "delta_g2": normalize2(vk.delta_g2), "aux_ids": [int(i) for i in circ.aux_ids], "n": circ.n}
+
+
+def serve_leg(args, ctx, with_cpu: bool) -> dict:
+    """Served Groth16 (config 5): Poisson arrivals (seed 0x5001) of requests
+    whose circuit has floor(2^U) constraints, U ~ U[lo, hi] (seed 0x5002),
+    padded to 2^k; one synthetic circuit and proving key per size class;
+    witness seed 0x5100 + id, (r, s) from seed 0x5200 + id.
+
+    * Capacity: all requests queued at t = 0, served by the plain
+      one-at-a-time loop and by the continuous-batching server.
+    * Rate sweep: the same request sequence arriving as a Poisson process at
+      each of `loads` x the plain loop's capacity, served by both loops; the
+      sustained rate of a loop is the highest swept rate whose p99 latency
+      meets the objective (--slo-ms).
+    Every server proof is checked with the pairing and must equal the plain
+    loop's proof of the same request byte for byte (each request proved
+    alone vs in the mix)."""
     from pyzkp_amd.field import words_to_ints
+    from pyzkp_amd.groth16 import prove, setup, synthetic_circuit, synthetic_witness, verify
     from pyzkp_amd.serve import (ProveServer, latency_stats, make_requests, poisson_arrivals, request_classes,
                                  serve_plain, working_set_bytes)
 
     lo, hi = (int(x) for x in args.serve_log.split(","))
-    classes = request_classes(args.requests, lo, hi)
+    R = args.requests
+    classes = request_classes(R, lo, hi)
     ks = sorted(set(classes))
-    log(f"serve: {args.requests} requests, classes {dict((k, classes.count(k)) for k in ks)}")
+    log(f"serve: {R} requests, classes {dict((k, classes.count(k)) for k in ks)}")
     t0 = time.perf_counter()
     circuits = {k: synthetic_circuit(k, 0x5300 + k) for k in ks}
     keys = {k: setup(circuits[k], ctx, seed=0x5400 + k) for k in ks}
     t_setup = time.perf_counter() - t0
     key_bytes = sum(pk.hbm_bytes for pk in keys.values())
     log(f"  keys ready in {t_setup:.1f} s ({key_bytes / 2**30:.2f} GiB resident)")
-    # warm-up: one proof per class (per-class service time of the plain loop)
     svc = {}
     for k in ks:
         w = synthetic_witness(circuits[k], 0x5500 + k)
@@ -395,89 +485,108 @@ def run_serve(args, ctx):
         t1 = time.perf_counter()
         prove(keys[k], w, 3, 4, g2_threads=args.cpu_threads)
         svc[k] = time.perf_counter() - t1
-    log(f"  service ms per class {dict((k, round(v * 1e3, 1)) for k, v in svc.items())}")
+    log(f"  plain service ms per class {dict((k, round(v * 1e3, 1)) for k, v in svc.items())}")
     budget = int(args.hbm_budget_gb * 2**30)
+    checks = {"verified": 0, "identical_to_plain": 0, "proofs": 0}
 
     def server_run(reqs):
         srv = ProveServer(keys, hbm_budget=budget, max_batch=16, g2_workers=args.g2_workers,
-                          cpu_threads=args.cpu_threads)
+                          cpu_threads=args.cpu_threads, device_slots=args.device_slots)
         try:
             return srv.run(reqs), dict(srv.stats)
         finally:
             srv.close()
 
-    # capacity: every request queued at t = 0
-    sat = make_requests(circuits, classes, [0.0] * len(classes))
+    def check(reqs, served, plain):
+        for q, a, b in zip(reqs, served, plain):
+            checks["proofs"] += 1
+            if a.proof is None or b.proof is None:
+                continue
+            pub = words_to_ints(q.witness[circuits[q.key].public_ids])
+            checks["verified"] += bool(verify(keys[q.key].vk, pub, a.proof))
+            checks["identical_to_plain"] += a.proof.to_bytes() == b.proof.to_bytes()
+
+    sat = make_requests(circuits, classes, [0.0] * R)
     log("  capacity: plain loop")
-    cap_plain = latency_stats(serve_plain(keys, sat, cpu_threads=args.cpu_threads))
+    plain_sat = serve_plain(keys, sat, cpu_threads=args.cpu_threads)
     log("  capacity: server")
-    sat_res, sat_stats = server_run(sat)
-    cap_srv = latency_stats(sat_res)
-    rate = args.load * cap_plain["proofs_per_s"]
-    log(f"  capacity plain {cap_plain['proofs_per_s']}/s, server {cap_srv['proofs_per_s']}/s; "
-        f"Poisson rate {rate:.2f}/s")
-    reqs = make_requests(circuits, classes, poisson_arrivals(args.requests, rate))
-    log("  Poisson: plain loop")
-    plain = serve_plain(keys, reqs, cpu_threads=args.cpu_threads)
-    log("  Poisson: server")
-    served, srv_stats = server_run(reqs)
-    log("  verifying")
-    n_ok = n_same = 0
-    for q, a, b in zip(reqs, served, plain):
-        if a.proof is None or b.proof is None:
-            continue
-        pub = words_to_ints(q.witness[circuits[q.key].public_ids])
-        n_ok += bool(verify(keys[q.key].vk, pub, a.proof))
-        n_same += a.proof.to_bytes() == b.proof.to_bytes()
-    st_s, st_p = latency_stats(served), latency_stats(plain)
-    n_same += sum(a.proof is not None and b.proof is not None and a.proof.to_bytes() == b.proof.to_bytes()
-                  for a, b in zip(sat_res, plain))
-    line = {
-        "metric": "served Groth16 proofs/s at capacity (all requests queued), p50/p99 latency under Poisson "
-                  "arrivals, vs plain one-at-a-time loop",
-        "value": cap_srv.get("proofs_per_s"), "unit": "proofs/s", "n_gpus": 1, "steps": args.requests,
-        "warmup": len(ks),
-        "ms_per_step": round(1e3 / cap_srv["proofs_per_s"], 3) if cap_srv.get("proofs_per_s") else None,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "u32 limbs: Fr/Fq Montgomery (254-bit integer)", "data": "synthetic",
-        "config": {"workload": f"served Groth16, {args.requests} Poisson requests at {args.load:.0%} of plain-loop "
-                               f"capacity, synthetic x^5 S-box R1CS 2^{lo}..2^{hi} constraints (log-uniform)",
-                   "classes": {str(k): classes.count(k) for k in ks},
-                   "service_ms_per_class": {str(k): round(v * 1e3, 2) for k, v in svc.items()},
-                   "capacity_server": cap_srv, "capacity_plain": cap_plain,
-                   "capacity_speedup": round(cap_srv["proofs_per_s"] / cap_plain["proofs_per_s"], 3),
-                   "arrival_rate_per_s": round(rate, 4), "poisson_server": st_s, "poisson_plain": st_p,
-                   "p50_speedup": round(st_p["p50_ms"] / st_s["p50_ms"], 3),
-                   "p99_speedup": round(st_p["p99_ms"] / st_s["p99_ms"], 3),
-                   "server_stats_capacity": sat_stats, "server_stats_poisson": srv_stats,
-                   "verified": n_ok, "identical_to_plain": n_same,
-                   "key_setup_s": round(t_setup, 2), "keys_hbm_gib": round(key_bytes / 2**30, 3),
-                   "working_set_max_mib": round(max(working_set_bytes(pk) for pk in keys.values()) / 2**20, 2),
-                   "cpu_threads": args.cpu_threads, "g2_workers": args.g2_workers,
-                   "hbm_budget_gb": args.hbm_budget_gb},
-        "roofline": None, "cpu_baseline": None,
+    srv_sat, sat_stats = server_run(sat)
+    check(sat, srv_sat, plain_sat)
+    cap_plain, cap_srv = latency_stats(plain_sat), latency_stats(srv_sat)
+    base = cap_plain["proofs_per_s"]
+    sweep = []
+    for f in (float(x) for x in args.loads.split(",")):
+        rate = f * base
+        reqs = make_requests(circuits, classes, poisson_arrivals(R, rate))
+        log(f"  Poisson {rate:.2f}/s ({f:.2f} x plain capacity): plain, server")
+        pl = serve_plain(keys, reqs, cpu_threads=args.cpu_threads)
+        sv, st = server_run(reqs)
+        check(reqs, sv, pl)
+        sweep.append({"load": f, "rate_per_s": round(rate, 3), "plain": latency_stats(pl),
+                      "server": latency_stats(sv), "server_stats": st})
+    slo = args.slo_ms
+
+    def sustained(loop):
+        ok = [e["rate_per_s"] for e in sweep if e[loop].get("count") == R and e[loop]["p99_ms"] <= slo]
+        return max(ok) if ok else 0.0
+
+    sus_p, sus_s = sustained("plain"), sustained("server")
+    at = [e for e in sweep if e["rate_per_s"] == sus_p]
+    leg = {
+        "metric": f"served Groth16: highest Poisson arrival rate with p99 <= {slo:.0f} ms (server vs plain "
+                  f"one-at-a-time loop), capacity, p50/p99",
+        "value": sus_s, "unit": "proofs/s", "plain_value": sus_p,
+        "rate_gain_at_equal_p99": round(sus_s / sus_p, 3) if sus_p else None,
+        "slo_p99_ms": slo,
+        "p99_at_plain_sustained_rate": ({"plain": at[0]["plain"]["p99_ms"], "server": at[0]["server"]["p99_ms"]}
+                                        if at else None),
+        "workload": f"{R} requests, synthetic x^5 S-box R1CS 2^{lo}..2^{hi} constraints (log-uniform), "
+                    "Poisson arrivals (seed 0x5001)",
+        "classes": {str(k): classes.count(k) for k in ks},
+        "service_ms_per_class": {str(k): round(v * 1e3, 2) for k, v in svc.items()},
+        "capacity_server": cap_srv, "capacity_plain": cap_plain,
+        "capacity_speedup": round(cap_srv["proofs_per_s"] / cap_plain["proofs_per_s"], 3),
+        "server_stats_capacity": sat_stats, "sweep": sweep,
+        "max_batch_seen_poisson": max(e["server_stats"]["max_batch_seen"] for e in sweep),
+        "verified": checks["verified"], "identical_to_plain": checks["identical_to_plain"],
+        "proofs_checked": checks["proofs"],
+        "key_setup_s": round(t_setup, 2), "keys_hbm_gib": round(key_bytes / 2**30, 3),
+        "working_set_max_mib": round(max(working_set_bytes(pk) for pk in keys.values()) / 2**20, 2),
+        "cpu_threads": args.cpu_threads, "g2_workers": args.g2_workers, "device_slots": args.device_slots,
+        "hbm_budget_gb": args.hbm_budget_gb, "cpu_baseline": None,
     }
-    if not args.no_cpu_baseline:
+    if with_cpu:
         from oracle import groth16 as OG
         k0 = ks[0]
         circ = circuits[k0]
+        opk = oracle_key_from_device(keys[k0], circ)
         w = words_to_ints(synthetic_witness(circ, 0x5100))
         A, B, C = circ.A.row_dicts(), circ.B.row_dicts(), circ.C.row_dicts()
-        pk0 = keys[k0]
-        td = (pk0.trapdoor_tau, pk0.trapdoor_alpha, pk0.trapdoor_beta, pk0.trapdoor_gamma, pk0.trapdoor_delta)
-        opk = OG.setup(A, B, C, circ.n, circ.n_vars, circ.public_ids, list(circ.aux_ids), td)
+        log(f"  CPU baseline: the oracle's Groth16 prover, class 2^{k0}")
         t0 = time.perf_counter()
         OG.prove(opk, A, B, C, w, 3, 4)
         cdt = time.perf_counter() - t0
-        line["cpu_baseline"] = {"value": round(1.0 / cdt, 4), "unit": "proofs/s", "cores": 1, "kind": "port",
-                                "sample": f"the oracle's Groth16 prover (Python restatement of "
-                                          f"protocols/groth16/prove.py), one proof of the smallest class 2^{k0}, "
-                                          f"1 thread, {cdt:.2f} s"}
-    print(json.dumps(line))
+        leg["cpu_baseline"] = {"value": round(1.0 / cdt, 4), "unit": "proofs/s", "cores": 1, "kind": "port",
+                               "sample": f"the oracle's Groth16 prover (Python restatement of "
+                                         f"protocols/groth16/prove.py), one proof of the smallest class 2^{k0}, "
+                                         f"1 thread, {cdt:.2f} s (its key read back from the device key)"}
     for pk in keys.values():
-        release_work(pk)
         pk.free()
-    return n_ok == len(reqs) and n_same == 2 * len(reqs)
+    return leg
+
+
+def run_serve(args, ctx):
+    leg = serve_leg(args, ctx, not args.no_cpu_baseline)
+    line = {
+        "metric": leg["metric"], "value": leg["value"], "unit": "proofs/s", "n_gpus": 1, "steps": args.requests,
+        "warmup": 2 * len(leg["classes"]), "ms_per_step": round(1e3 / leg["value"], 3) if leg["value"] else None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u32 limbs: Fr/Fq Montgomery (254-bit integer)", "data": "synthetic",
+        "config": {k: v for k, v in leg.items() if k not in ("metric", "value", "cpu_baseline")},
+        "roofline": None, "cpu_baseline": leg["cpu_baseline"],
+    }
+    print(json.dumps(line))
+    return leg["verified"] == leg["proofs_checked"] and leg["identical_to_plain"] == leg["proofs_checked"]
 
 
 def main():
@@ -518,39 +627,14 @@ def main():
 
     if args.workload == "ntt":
         log_n = args.log_n or 24
-        r = ntt_block(ctx, log_n, args.steps, args.warmup)
-        n = 1 << log_n
-        fwd = r["fwd"]
-        mads = (n // 2) * log_n * FQ_MUL_MADS
+        leg = ntt_leg(ctx, log_n, args.steps, args.warmup, probe_mads, not args.no_cpu_baseline, args.cpu_threads)
         line = {
-            "metric": "Fr NTT GB/s @2^%d" % log_n, "value": fwd["GBps"], "unit": "GB/s",
-            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": fwd["ms"],
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "u32x8 Montgomery Fr (254-bit integer)", "data": "synthetic",
-            "config": {"workload": f"Fr NTT forward, n=2^{log_n}, uniform coefficients seed "
-                                   f"{'0x2001' if log_n == 20 else '0x2002'}",
-                       "variants": r},
-            "roofline": {"bound": "valu", "achieved": round(mads / (fwd["device_ms"] / 1e3) / 1e12, 3),
-                         "peak": round(peak_mads / 1e12, 3), "unit": "T lane-mads/s",
-                         "peak_source": f"256 CUs x 64 lanes x 2.4 GHz; probe {probe_mads / 1e12:.2f} T/s sustained",
-                         "frac": round(mads / (fwd["device_ms"] / 1e3) / peak_mads, 4),
-                         "traffic": measured_traffic(f"ntt_2^{log_n}", "ntt_pass_kernel"),
-                         "kernel": "ntt_pass_kernel",
-                         "hbm_GBps_algorithmic": fwd["GBps"], "hbm_frac": round(fwd["GBps"] / 8000.0, 4)},
-            "cpu_baseline": None,
+            "metric": leg["metric"], "value": leg["value"], "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": leg["ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u32 limbs: Fr Montgomery (254-bit integer)", "data": "synthetic",
+            "config": {"workload": leg["workload"], "variants": leg["variants"]},
+            "roofline": leg["roofline"], "cpu_baseline": leg["cpu_baseline"],
         }
-        if not args.no_cpu_baseline:
-            from oracle import cref
-            from pyzkp_amd.field import omega_for_size
-            nc = 1 << min(22, log_n)
-            wds = workload.fr_uniform(0x2001, nc)
-            t0 = time.perf_counter()
-            cref.ntt_words(wds, nc, nc.bit_length() - 1, omega_for_size(nc), False, None, 1)
-            cdt = time.perf_counter() - t0
-            line["cpu_baseline"] = {"value": round(64 * nc / cdt / 1e9, 3), "unit": "GB/s", "cores": 1,
-                                    "kind": "port",
-                                    "sample": f"C restatement of crypto/poly/ntt.py (bit-reverse radix-2 DIT), "
-                                              f"forward n=2^{nc.bit_length() - 1}, 1 thread, {cdt:.2f} s"}
         print(json.dumps(line))
         return
 
@@ -629,7 +713,10 @@ def main():
     secondary = {}
     if not args.no_ntt:
         log("NTT 2^24")
-        secondary["ntt_2^24"] = ntt_block(ctx, 24, max(3, args.steps // 2), 2)
+        secondary["ntt_2^24"] = ntt_leg(ctx, 24, max(5, args.steps), 2, probe_mads, not args.no_cpu_baseline,
+                                        args.cpu_threads)
+    if not args.no_serve:
+        secondary["served_groth16"] = serve_leg(args, ctx, not args.no_cpu_baseline)
 
     cpu = None
     if not args.no_cpu_baseline:
