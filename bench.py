@@ -651,6 +651,10 @@ def main():
         ss = workload.fr_uniform(seed_s, n)
     bases = workload.device_bases(ctx, ks)
     scal = workload.device_fr_mont(ctx, ss)
+    # the bases are a resident set (an SRS): converted once, like the MSM_G1
+    # operator's per-points cache does
+    mb = ctx.msm_bases_create(bases.ptr, n)
+    msm = lambda m, c: ctx.msm_g1_prepared(mb, scal.ptr, m, c)
 
     # autotune c in 13..18 (one timed call each after one warm call)
     cands = [args.window_bits] if args.window_bits else list(range(13, 19))
@@ -658,23 +662,23 @@ def main():
     for c in cands:
         for k_red in (0, 8, 16):   # buckets per reduction thread (0: the library's rule)
             ctx.set_tuning("msm_reduce_chunk", k_red)
-            ctx.msm_g1(bases.ptr, scal.ptr, n, c)
+            msm(n, c)
             t0 = time.perf_counter()
-            ctx.msm_g1(bases.ptr, scal.ptr, n, c)
-            ctx.msm_g1(bases.ptr, scal.ptr, n, c)
+            msm(n, c)
+            msm(n, c)
             tune[(c, k_red)] = (time.perf_counter() - t0) / 2
             log(f"  c={c} reduce chunk {k_red}: {tune[(c, k_red)] * 1e3:.2f} ms")
     c, k_red = min(tune, key=tune.get)
     ctx.set_tuning("msm_reduce_chunk", k_red)
 
     for _ in range(args.warmup):
-        res = ctx.msm_g1(bases.ptr, scal.ptr, n, c)
+        res = msm(n, c)
     ctx.sync()
     ctx.profile_reset()
     ctx.profile_enable(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res = ctx.msm_g1(bases.ptr, scal.ptr, n, c)
+        res = msm(n, c)
     ctx.sync()
     wall = time.perf_counter() - t0
     ctx.profile_enable(False)
@@ -735,7 +739,7 @@ def main():
         t0 = time.perf_counter()
         cres = cref.msm_pippenger_words(braw, sraw, m, wcpu, threads=1)
         cpu_s = time.perf_counter() - t0
-        gres = from_jacobian_mont(*ctx.msm_g1(bases.ptr, scal.ptr, m, c))
+        gres = from_jacobian_mont(*msm(m, c))
         gaff = None if gres[2] == 0 else (gres[0], gres[1])
         cpu = {"value": round(m / cpu_s, 1), "unit": "pts/s", "cores": 1, "kind": "port",
                "sample": f"C restatement of crypto/msm/pippenger.py (unsigned {wcpu}-bit windows), "
@@ -765,7 +769,7 @@ def main():
         "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u32x8 Montgomery Fq/Fr (254-bit integer)", "data": "synthetic",
         "config": {"workload": f"BN254 G1 MSM n=2^{log_n} {'skewed' if args.skewed else 'uniform'} scalars, "
-                               "bases [k_i]G1 affine (device-resident)",
+                               "bases [k_i]G1 affine (device-resident, prepared once as a resident set)",
                    "n": n, "window_bits": c, "reduce_chunk": k_red,
                    "autotune_ms": {f"c{k[0]}_k{k[1]}": round(v * 1e3, 3) for k, v in tune.items()},
                    "phases_ms": phases, "verified_closed_form": bool(verified)},
@@ -774,6 +778,7 @@ def main():
         "secondary": secondary,
     }
     print(json.dumps(line))
+    mb.free()
     bases.free()
     scal.free()
 

@@ -38,6 +38,7 @@ extern "C" {
 typedef struct zk_ctx zk_ctx;
 typedef struct zk_fixed_bases zk_fixed_bases;
 typedef struct zk_batch zk_batch;
+typedef struct zk_msm_bases zk_msm_bases;
 
 /* One NTT job of a batch: `batch` transforms of size 2^log_n laid out back
  * to back, out of place (in != out), with the optional root / coset shift
@@ -127,6 +128,19 @@ int zk_ntt_batch(zk_ctx* ctx, const void* in, void* out, uint32_t log_n, uint32_
  * `stream` before returning the result in out_jac_mont. */
 int zk_msm_g1(zk_ctx* ctx, const void* bases, const void* scalars, uint64_t n, int window_bits,
               uint64_t out_jac_mont[12], void* stream);
+
+/* Prepared base sets: `n` bases (the zk_msm_g1 layout) converted once into
+ * the form the MSM kernels gather and kept resident, so repeated MSMs over
+ * the set (or a prefix of it) skip the per-call conversion.  The device
+ * form of the reference's per-points-identity caches (cached_points_tuple,
+ * runtime/warmup.py:17-26; crypto/msm/pippenger.py:225-248), where the
+ * Metal path re-packs the bases on every call (runtime/metal/msm.py:13-34).
+ * zk_msm_g1_prepared computes the same sum as zk_msm_g1 over the first n
+ * bases of the set (n <= the set's size). */
+int zk_msm_bases_create(zk_ctx* ctx, const void* bases, uint64_t n, zk_msm_bases** out, void* stream);
+int zk_msm_bases_destroy(zk_ctx* ctx, zk_msm_bases* b);
+int zk_msm_g1_prepared(zk_ctx* ctx, const zk_msm_bases* b, const void* scalars, uint64_t n, int window_bits,
+                       uint64_t out_jac_mont[12], void* stream);
 
 /* zk_msm_g1 without any device allocation: the same result, on a 64 MiB
  * arena that is a file-scope device array (one per GPU, held until the

@@ -49,6 +49,9 @@ SIGNATURES = {
     "zk_ntt_batch": [_vp, _vp, _vp, ct.c_uint32, ct.c_uint32, ct.c_int, _u64p, _u64p, _vp],
     "zk_msm_g1": [_vp, _vp, _vp, ct.c_uint64, ct.c_int, _u64p, _vp],
     "zk_msm_g1_noalloc": [_vp, _vp, _vp, ct.c_uint64, ct.c_int, _u64p, _vp],
+    "zk_msm_bases_create": [_vp, _vp, ct.c_uint64, ct.POINTER(_vp), _vp],
+    "zk_msm_bases_destroy": [_vp, _vp],
+    "zk_msm_g1_prepared": [_vp, _vp, _vp, ct.c_uint64, ct.c_int, _u64p, _vp],
     "zk_g1_fixed_base": [_vp, _vp, _vp, ct.c_uint64, _vp, _vp],
     "zk_fixed_bases_create": [_vp, _vp, ct.c_uint64, ct.c_int, ct.POINTER(_vp), _vp],
     "zk_fixed_bases_destroy": [_vp, _vp],
@@ -145,6 +148,20 @@ class DeviceBuffer:
 
     def offset(self, nbytes: int) -> int:
         return self.ptr + nbytes
+
+
+class MsmBases:
+    """A base set converted once for repeated MSMs (zk_msm_bases_*)."""
+
+    def __init__(self, ctx: "Context", handle, n: int) -> None:
+        self.ctx = ctx
+        self.handle = handle
+        self.n = n
+
+    def free(self) -> None:
+        if self.handle:
+            self.ctx._check(self.ctx.lib.zk_msm_bases_destroy(self.ctx.handle, self.handle))
+            self.handle = None
 
 
 class FixedBases:
@@ -298,6 +315,21 @@ class Context:
         """Returns Jacobian Montgomery (X, Y, Z) as three ints (Z = 0: identity)."""
         out = (ct.c_uint64 * 12)()
         self._check(self.lib.zk_msm_g1(self.handle, bases, scalars, int(n), int(window_bits), out, stream))
+        w = list(out)
+        f = lambda a: w[a] | (w[a + 1] << 64) | (w[a + 2] << 128) | (w[a + 3] << 192)
+        return f(0), f(4), f(8)
+
+    def msm_bases_create(self, bases: int, n: int, stream=None) -> "MsmBases":
+        h = _vp()
+        self._check(self.lib.zk_msm_bases_create(self.handle, bases, int(n), ct.byref(h), stream))
+        return MsmBases(self, h, int(n))
+
+    def msm_g1_prepared(self, mb: "MsmBases", scalars: int, n: int, window_bits: int = 0,
+                        stream=None) -> Sequence[int]:
+        """zk_msm_g1 over the first n bases of a prepared set."""
+        out = (ct.c_uint64 * 12)()
+        self._check(self.lib.zk_msm_g1_prepared(self.handle, mb.handle, scalars, int(n), int(window_bits), out,
+                                                stream))
         w = list(out)
         f = lambda a: w[a] | (w[a + 1] << 64) | (w[a + 2] << 128) | (w[a + 3] << 192)
         return f(0), f(4), f(8)

@@ -326,6 +326,33 @@ int zk_msm_g1(zk_ctx* ctx, const void* bases, const void* scalars, uint64_t n, i
                     pick(ctx, stream));
 }
 
+int zk_msm_bases_create(zk_ctx* ctx, const void* bases, uint64_t n, zk_msm_bases** out, void* stream) {
+  CTX_CHECK(ctx);
+  if (!out) return set_err(&ctx->c, ZK_ERR_INVALID, "null output");
+  *out = nullptr;
+  if (n && !bases) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  MsmBases* b = nullptr;
+  int rc = msm_bases_create(&ctx->c, (const fe*)bases, n, &b, pick(ctx, stream));
+  *out = (zk_msm_bases*)b;
+  return rc;
+}
+
+int zk_msm_bases_destroy(zk_ctx* ctx, zk_msm_bases* b) {
+  CTX_CHECK(ctx);
+  hipDeviceSynchronize();
+  msm_bases_destroy((MsmBases*)b);
+  return ZK_OK;
+}
+
+int zk_msm_g1_prepared(zk_ctx* ctx, const zk_msm_bases* b, const void* scalars, uint64_t n, int window_bits,
+                       uint64_t out_jac_mont[12], void* stream) {
+  CTX_CHECK(ctx);
+  if (!out_jac_mont || !b) return set_err(&ctx->c, ZK_ERR_INVALID, "null argument");
+  if (n && !scalars) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  return msm_g1_prepared_run(&ctx->c, (const MsmBases*)b, (const fe*)scalars, n, window_bits, out_jac_mont,
+                             pick(ctx, stream));
+}
+
 int zk_msm_g1_noalloc(zk_ctx* ctx, const void* bases, const void* scalars, uint64_t n, int window_bits,
                       uint64_t out_jac_mont[12], void* stream) {
   CTX_CHECK(ctx);

@@ -124,7 +124,10 @@ ZK_HD void xyzz9_madd_t(xyzz9& p, bool& p_inf, const aff9& a, bool neg) {
   const f9 s2 = Fq9::mul(neg ? Fq9::neg_cf(a.y) : a.y, p.zzz);
   const f9 P = Fq9::sub_nr(u2, p.x);
   const f9 R = Fq9::sub_nr(s2, p.y);
-  if (Fq9::is_zero_lt4p(P)) {
+  // P is 0 mod p only as 0, p, 2p or 3p (normalised limbs below 4p): their
+  // top limbs are the only candidates, so the full test runs only when a
+  // lane's top limb matches one of them (almost never for distinct points)
+  if (Fq9::top_may_be_zero_lt4p(P) && Fq9::is_zero_lt4p(P)) {
     if (Fq9::is_zero_lt4p(R)) {
       aff9 b = a;
       if (neg) b.y = Fq9::neg_nr(a.y);
@@ -160,7 +163,7 @@ ZK_HD xyzz9 xyzz9_add(const xyzz9& p, const xyzz9& q) {
   Fq9::mul_x2(p.y, q.zzz, q.y, p.zzz, s1, s2);
   const f9 P = Fq9::sub_nr(u2, u1);
   const f9 R = Fq9::sub_nr(s2, s1);
-  if (Fq9::is_zero_lt4p(P)) {
+  if (Fq9::top_may_be_zero_lt4p(P) && Fq9::is_zero_lt4p(P)) {
     if (Fq9::is_zero_lt4p(R)) return xyzz9_dbl(p);
     return xyzz9_inf();
   }

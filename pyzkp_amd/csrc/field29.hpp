@@ -316,6 +316,12 @@ struct Mont29 {
     return z == 0 || e1 == 0 || e2 == 0 || e3 == 0;
   }
 
+  // necessary condition for is_zero_lt4p: the top limb of 0, p, 2p or 3p
+  static ZK_HD bool top_may_be_zero_lt4p(const f9& a) {
+    const u32 t = a.l[8];
+    return t == 0 || t == F::P[8] || t == F::P2[8] || t == F::P3[8];
+  }
+
   // s -> s - 2p if s >= 2p  (s < 4p, limbs normalised)
   static ZK_HD f9 reduce2p(const f9& s) { return sub_if_ge(s, F::P2); }
 

@@ -136,6 +136,15 @@ int batch_run(Ctx* c, BatchPlan* p, uint64_t* out_jac, hipStream_t st);
 void batch_destroy(BatchPlan* p);
 int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
                uint64_t out_jac[12], hipStream_t st);
+// base sets converted once for repeated MSMs
+struct MsmBases {
+  uint64_t n = 0;
+  DevBuf conv;  // R = 2^261 affine coordinates, 64 bytes per point
+};
+int msm_bases_create(Ctx* c, const fe* bases, uint64_t n, MsmBases** out, hipStream_t st);
+void msm_bases_destroy(MsmBases* b);
+int msm_g1_prepared_run(Ctx* c, const MsmBases* b, const fe* scalars, uint64_t n, int window_bits,
+                        uint64_t out_jac[12], hipStream_t st);
 // the same MSM on a file-scope device arena: no device allocation
 int msm_noalloc_init(Ctx* c);
 int msm_g1_noalloc_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,

@@ -1168,6 +1168,49 @@ static int msm_check_args(Ctx* c, uint64_t n, int window_bits, int* cb) {
   return ZK_OK;
 }
 
+// ------------------------------------------------------------ prepared base sets
+// A base set used by many MSMs (an SRS, a proving key's query vector) is
+// converted once to the R = 2^261 form the accumulation gathers, and stays
+// resident; MSMs over a prefix of it skip the per-call conversion.
+int msm_bases_create(Ctx* c, const fe* bases, uint64_t n, MsmBases** out, hipStream_t st) {
+  *out = nullptr;
+  if (n >= (1ull << 31)) return set_err(c, ZK_ERR_INVALID, "MSM size must be below 2^31");
+  MsmBases* b = new MsmBases();
+  b->n = n;
+  b->conv.bytes = (n ? n : 1) * 64;
+  hipError_t e = hipMalloc(&b->conv.ptr, b->conv.bytes);
+  if (e != hipSuccess) {
+    delete b;
+    return hip_check(c, e, "hipMalloc(prepared bases)");
+  }
+  if (n) msm_bases_convert_kernel<<<grid_for(n, 256), 256, 0, st>>>(bases, n, (fe*)b->conv.ptr);
+  e = hipGetLastError();
+  if (e != hipSuccess) {
+    hipFree(b->conv.ptr);
+    delete b;
+    return hip_check(c, e, "msm_bases_convert_kernel");
+  }
+  *out = b;
+  return ZK_OK;
+}
+
+void msm_bases_destroy(MsmBases* b) {
+  if (!b) return;
+  if (b->conv.ptr) hipFree(b->conv.ptr);
+  delete b;
+}
+
+int msm_g1_prepared_run(Ctx* c, const MsmBases* b, const fe* scalars, uint64_t n, int window_bits,
+                        uint64_t out_jac[12], hipStream_t st) {
+  for (int i = 0; i < 12; i++) out_jac[i] = 0;
+  if (n > b->n) return set_err(c, ZK_ERR_INVALID, "more scalars than prepared bases");
+  if (n == 0) return ZK_OK;
+  int cb;
+  int rc = msm_check_args(c, n, window_bits, &cb);
+  if (rc) return rc;
+  return msm_g1_single(c, nullptr, (const fe*)b->conv.ptr, scalars, n, cb, out_jac, st, nullptr, 0);
+}
+
 // ------------------------------------------------------------ no-allocation MSM
 // `scratch` is the MSM arena of zk_msm_g1_noalloc, for callers that cannot
 // allocate device memory (code under HIP graph capture, tight serving loops,

@@ -55,9 +55,11 @@ class HipBuffer:
 
 @dataclass
 class DeviceBases:
-    """G1 bases resident in HBM as 64-byte affine Montgomery points."""
+    """G1 bases resident in HBM as 64-byte affine Montgomery points, plus
+    their prepared form (converted once, made on the first MSM)."""
     dbuf: Any
     n: int
+    prepared: Any = None
 
     def __len__(self) -> int:
         return int(self.n)
@@ -350,7 +352,10 @@ def _window_bits(ctx) -> int:
 
 
 def msm_device(ctx, bases: DeviceBases, scalars: HipBuffer, n: int):
-    X, Y, Z = _hip(ctx).msm_g1(bases.dbuf.ptr, scalars.ptr, n, _window_bits(ctx))
+    hip = _hip(ctx)
+    if bases.prepared is None or bases.prepared.ctx is not hip or not bases.prepared.handle:
+        bases.prepared = hip.msm_bases_create(bases.dbuf.ptr, bases.n)
+    X, Y, Z = hip.msm_g1_prepared(bases.prepared, scalars.ptr, n, _window_bits(ctx))
     return from_jacobian_mont(X, Y, Z)
 
 

@@ -221,6 +221,17 @@ def test_msm_closed_form_large(zctx, log_n, skewed):
     cs = (13, 16, 18) if log_n == 22 else (8, 13, 16)
     for c in cs:
         assert _msm(zctx, bases, scal, n, c) == expect, c
+    # the same bases as a prepared (converted once) set: full size and a prefix
+    mb = zctx.msm_bases_create(bases.ptr, n)
+    for c in cs[:2]:
+        p = from_jacobian_mont(*zctx.msm_g1_prepared(mb, scal.ptr, n, c))
+        assert (None if p[2] == 0 else (p[0], p[1])) == expect, ("prepared", c)
+    m = n // 3 + 1
+    p = from_jacobian_mont(*zctx.msm_g1_prepared(mb, scal.ptr, m, 0))
+    assert (None if p[2] == 0 else (p[0], p[1])) == _closed(ks[:m], ss[:m])
+    with pytest.raises(ValueError):
+        zctx.msm_g1_prepared(mb, scal.ptr, n + 1, 0)
+    mb.free()
     bases.free()
     scal.free()
 
