@@ -500,8 +500,15 @@ int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value) {
   if (!key) return set_err(&ctx->c, ZK_ERR_INVALID, "null key");
   std::string k(key);
   if (k == "ntt_max_pass_bits") {
-    if (value < 0 || value > 11) return set_err(&ctx->c, ZK_ERR_INVALID, "ntt_max_pass_bits must be in [0, 11]");
+    if (value < 0 || value > 12) return set_err(&ctx->c, ZK_ERR_INVALID, "ntt_max_pass_bits must be in [0, 12]");
     ctx->c.ntt_max_pass_bits = (int)value;
+  } else if (k == "ntt_tile_bits") {
+    if (value != 0 && value != 10 && value != 12)
+      return set_err(&ctx->c, ZK_ERR_INVALID, "ntt_tile_bits must be 0, 10 or 12");
+    ctx->c.ntt_tile_bits = (int)value;
+  } else if (k == "ntt_xcd_group") {
+    if (value < 0 || value > 64) return set_err(&ctx->c, ZK_ERR_INVALID, "ntt_xcd_group must be in [0, 64]");
+    ctx->c.ntt_xcd_group = (int)value;
   } else if (k == "msm_segment") {
     if (value < 1 || value > 4096) return set_err(&ctx->c, ZK_ERR_INVALID, "msm_segment must be in [1, 4096]");
     ctx->c.msm_segment = (uint32_t)value;

@@ -1,6 +1,7 @@
 // CPU build of the device field / curve arithmetic (field.hpp, curve.hpp are
 // __host__ __device__).  Exposes a few entry points so the host-logic tests
 // can check the exact code the kernels run against the oracle without a GPU.
+#include <vector>
 #include <string.h>
 #include "field.hpp"
 #include "curve.hpp"
@@ -267,7 +268,7 @@ int zkhost_madd_chain(const uint64_t* p16, const uint64_t* affs, const uint8_t* 
 }
 
 // The arithmetic of one NTT pass holding the whole transform (n = 2^log_n
-// <= 1024 elements, global stage 0 first), stage by stage (the kernel runs
+// <= 4096 elements, global stage 0 first), stage by stage (the kernel runs
 // the same butterflies two stages at a time):
 // caller words reduced below 4p on load, stage 0 with the carrying forms,
 // later stages with the carry-free forms and u normalised at every even
@@ -276,9 +277,9 @@ int zkhost_madd_chain(const uint64_t* p16, const uint64_t* affs, const uint8_t* 
 // tw: omega^e (e < n/2), 4 words each, R = 2^261 canonical.  max_limb gets
 // the largest limb any stage wrote.
 int zkhost_ntt_tile(const uint64_t* in, const uint64_t* tw, uint32_t log_n, uint64_t* out, uint32_t* max_limb) {
-  if (log_n == 0 || log_n > 10) return 1;
+  if (log_n == 0 || log_n > 12) return 1;
   const uint32_t n = 1u << log_n;
-  f9 a[1024], b[1024];
+  std::vector<f9> a(n), b(n);
   for (uint32_t j = 0; j < n; j++) a[j] = Fr9::reduce2p(unpack9(rd(in + 4 * j)));
   uint32_t mx = 0;
   for (uint32_t t = 0; t < log_n; t++) {

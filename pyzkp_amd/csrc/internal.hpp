@@ -36,7 +36,7 @@ struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
-  // NTT twiddle tables omega^e, e < n/2, keyed by (log_n, omega Montgomery words)
+  // NTT twiddle tables omega^e, e <= n/2 (both directions), keyed by (log_n, omega Montgomery words)
   std::map<std::array<uint64_t, 5>, DevBuf> twiddles;
   // two-level power tables (hi, lo) for coset / inverse scaling,
   // keyed by (log_n, base words, scale words)
@@ -45,6 +45,8 @@ struct Ctx {
   DevBuf scratch[8];  // 0-1 NTT, 2-3 MSM, 4 counters, 5 powers, 6 inversion, 7 PLONK quotient
   // tuning knobs (set through zk_set_tuning)
   int ntt_max_pass_bits = 0;      // 0: planner default
+  int ntt_tile_bits = 0;          // 0: 2^10-element tiles unless a pass has more than 10 stages
+  int ntt_xcd_group = 0;          // 0: group tiles per XCD only for one- or two-column tiles
   uint32_t msm_segment = 80;      // sorted (bucket, point) pairs per accumulation thread
   uint32_t msm_reduce_chunk = 0;  // buckets per reduction thread (0: sized to fill the device)
   uint32_t msm_groups = 0;        // window groups pipelined over two streams (0: chosen per shape)

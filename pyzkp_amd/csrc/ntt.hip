@@ -36,10 +36,18 @@
 
 namespace zk {
 
-static constexpr int NTT_TILE = 1024;   // elements per LDS tile (36 KiB as limbs: four workgroups per CU)
+// Tiles: 2^10 elements (36 KiB as limbs, 256 threads: four workgroups per
+// CU) or 2^12 (144 KiB of the 160 KiB LDS, 1024 threads: one workgroup per
+// CU), so that a pass can hold up to 12 stages.  A thread runs one radix-4
+// group per step, so a tile has four elements per thread.
 static constexpr int NTT_TILE_BITS = 10;
-static constexpr int NTT_THREADS = 256;
-static constexpr int NTT_BFLY_PER_THREAD = NTT_TILE / 2 / NTT_THREADS;
+static constexpr int NTT_BIG_TILE_BITS = 12;
+template <int TB>
+struct NttTile {
+  static constexpr int ELEMS = 1 << TB;
+  static constexpr int THREADS = ELEMS / 4;
+  static constexpr int BFLY_PER_THREAD = ELEMS / 2 / THREADS;
+};
 static constexpr int POW_LO_BITS = POW_TABLE_LO_BITS;  // two-level power tables: lo has 2^10 entries
 
 // tables: R = 2^261 values packed in 32 bytes
@@ -59,12 +67,24 @@ __device__ __forceinline__ f9 pow_lookup(const fe* __restrict__ hi, const fe* __
 // scale modes
 enum { SCALE_NONE = 0, SCALE_CONST = 1, SCALE_TABLE = 2 };
 
-__global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
+// xcd_group > 1: tiles are numbered so that the workgroups one XCD receives
+// (the hardware deals blockIdx.x round robin over the 8 XCDs) take
+// consecutive tiles; with one-column tiles (T = 1) the 32-byte runs of
+// neighbouring columns then share a cache line inside one XCD's L2.
+// INV: the transform with the inverse root, read from the forward table.
+// w^-e = w^(n - e) = -w^(n/2 - e), and the table holds w^e for 0 <= e <=
+// n/2 (w^(n/2) = -1), so the twiddle is the entry n/2 - e negated; negating
+// the twiddle of a butterfly (u + w v, u - w v) only exchanges its two
+// outputs, so those stages write the sum where the difference goes and back.
+template <int TB, bool INV>
+__global__ __launch_bounds__(NttTile<TB>::THREADS) void ntt_pass_kernel(
     const fe* __restrict__ in, fe* __restrict__ out, const uint32_t* __restrict__ tw, uint32_t log_n,
     uint32_t s0, uint32_t k, uint32_t log_t, uint64_t in_len, int pre_mode,
     const fe* __restrict__ pre_hi, const fe* __restrict__ pre_lo, int post_mode,
-    const fe* __restrict__ post_hi, const fe* __restrict__ post_lo, int last) {
-  __shared__ f9 lds[NTT_TILE];
+    const fe* __restrict__ post_hi, const fe* __restrict__ post_lo, int last, uint32_t xcd_group) {
+  constexpr int NTT_THREADS = NttTile<TB>::THREADS;
+  constexpr int NTT_BFLY_PER_THREAD = NttTile<TB>::BFLY_PER_THREAD;
+  __shared__ f9 lds[NttTile<TB>::ELEMS];
   const uint64_t n = 1ull << log_n;
   const uint32_t R = 1u << k;
   const uint32_t T = 1u << log_t;
@@ -73,7 +93,12 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
   const uint64_t boff = (uint64_t)blockIdx.y * n;
   in += boff;
   out += boff;
-  const uint64_t j0 = (uint64_t)blockIdx.x << log_t;
+  uint64_t tile = blockIdx.x;
+  if (xcd_group > 1) {
+    const uint64_t per = gridDim.x / xcd_group;  // the host passes a grid divisible by xcd_group
+    tile = (tile % xcd_group) * per + tile / xcd_group;
+  }
+  const uint64_t j0 = tile << log_t;
   const uint64_t Ns = 1ull << s0;
   const uint32_t tid = threadIdx.x;
 
@@ -119,7 +144,19 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
   const uint32_t halfRT = RT >> 1, quarterRT = RT >> 2;
   auto tw_cd = [&](uint32_t c, uint32_t d, uint32_t t) -> f9 {
     const uint64_t low = (j0 + c) & (Ns - 1);
-    return tw9(tw, (low + (uint64_t)d * Ns) << (log_n - 1 - (s0 + t)));
+    const uint64_t e = (low + (uint64_t)d * Ns) << (log_n - 1 - (s0 + t));
+    return tw9(tw, INV ? (n >> 1) - e : e);
+  };
+  // the two outputs of a table-twiddle butterfly: (u + w v, u - w v + 2p),
+  // exchanged in the inverse (the table gives -w there)
+  auto bfly = [&](const f9& u, const f9& wv, f9& lo, f9& hi) {
+    if (INV) {
+      lo = Fr9::sub_cf(u, wv);
+      hi = Fr9::add_cf(u, wv);
+    } else {
+      lo = Fr9::add_cf(u, wv);
+      hi = Fr9::sub_cf(u, wv);
+    }
   };
   uint32_t t = 0;
   for (; t + 1 < k; t += 2) {
@@ -145,10 +182,8 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         const f9 u1 = (t != 0) ? Fr9::normalize(x1) : x1;
         f9 v2, v3;
         Fr9::mul_x2(x2, w1, x3, w1, v2, v3);  // two chains interleaved
-        A = Fr9::add_cf(u0, v2);
-        B = Fr9::sub_cf(u0, v2);
-        C = Fr9::add_cf(u1, v3);
-        D = Fr9::sub_cf(u1, v3);
+        bfly(u0, v2, A, B);
+        bfly(u1, v3, C, D);
       } else {
         // global stage 0: v below 2p, not 1.2p: the carrying forms
         const f9 v2 = Fr9::reduce2p(x2), v3 = Fr9::reduce2p(x3);
@@ -159,10 +194,13 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
       }
       const f9 vc = Fr9::mul(C, w2), vd = Fr9::mul(D, w3);
       const uint32_t o2 = 4 * (q - d) + d;
-      lds[(o2 << log_t) + c] = Fr9::add_cf(A, vc);
-      lds[((o2 + (2u << t)) << log_t) + c] = Fr9::sub_cf(A, vc);
-      lds[((o2 + (1u << t)) << log_t) + c] = Fr9::add_cf(B, vd);
-      lds[((o2 + (3u << t)) << log_t) + c] = Fr9::sub_cf(B, vd);
+      f9 r0, r1, r2, r3;
+      bfly(A, vc, r0, r1);
+      bfly(B, vd, r2, r3);
+      lds[(o2 << log_t) + c] = r0;
+      lds[((o2 + (2u << t)) << log_t) + c] = r1;
+      lds[((o2 + (1u << t)) << log_t) + c] = r2;
+      lds[((o2 + (3u << t)) << log_t) + c] = r3;
     }
     __syncthreads();
   }
@@ -191,8 +229,7 @@ __global__ __launch_bounds__(NTT_THREADS) void ntt_pass_kernel(
         if (s != 0) {
           const f9 vv = Fr9::mul(v[i], w[i]);
           const f9 uu = (t != 0) ? Fr9::normalize(u[i]) : u[i];  // t is even here
-          lo = Fr9::add_cf(uu, vv);
-          hi = Fr9::sub_cf(uu, vv);
+          bfly(uu, vv, lo, hi);
         } else {
           const f9 vv = Fr9::reduce2p(v[i]);
           lo = Fr9::add_nr(u[i], vv);
@@ -316,7 +353,7 @@ static int get_twiddles(Ctx* c, uint32_t log_n, const zkh::Fe& w, const uint32_t
   std::array<uint64_t, 5> key = {log_n, w.v[0], w.v[1], w.v[2], w.v[3]};
   auto it = c->twiddles.find(key);
   if (it == c->twiddles.end()) {
-    const uint64_t count = (1ull << log_n) / 2;
+    const uint64_t count = (1ull << log_n) / 2 + 1;  // w^e for 0 <= e <= n/2 (the inverse reads it reflected)
     const fe *hi, *lo;
     int rc = get_pow_tables(c, log_n, w, zkh::one(zkh::FR), &hi, &lo, st);
     if (rc) return rc;
@@ -341,13 +378,21 @@ static zkh::Fe standard_root(uint32_t log_n) {
   return w;
 }
 
-// choose stages per pass: as few passes as allow a column run of >= 4
-// elements (128 B) per global access, with near-equal pass sizes
+// choose stages per pass: as few passes of at most 10 stages as the size
+// needs, with near-equal pass sizes.  A 10-stage pass on the 2^10-element
+// tile moves one-column (32-byte) runs on its strided side; at 2^20 two
+// such passes (tiles grouped per XCD) measured 4% faster than three
+// passes with 128-byte runs.  Passes of 11 or 12 stages (max_pass_bits)
+// use the 2^12-element tile; at 2^24 two 12-stage passes are bit-exact but
+// measured 37% slower than three 8-stage passes: one workgroup per CU
+// leaves each tile's load and store phases exposed.
 static void plan_passes(uint32_t log_n, int max_pass_bits, std::vector<uint32_t>& ks) {
   ks.clear();
   if (log_n == 0) return;
-  const uint32_t tile_bits = NTT_TILE_BITS;
-  uint32_t kmax = (max_pass_bits > 0) ? (uint32_t)max_pass_bits : 9;
+  // passes of more than 10 stages (asked for through max_pass_bits) run on
+  // the 2^12-element tiles
+  const uint32_t tile_bits = max_pass_bits > NTT_TILE_BITS ? NTT_BIG_TILE_BITS : NTT_TILE_BITS;
+  uint32_t kmax = (max_pass_bits > 0) ? (uint32_t)max_pass_bits : 10;
   if (kmax > tile_bits) kmax = tile_bits;
   if (log_n <= tile_bits && (max_pass_bits <= 0 || log_n <= kmax)) {
     ks.push_back(log_n);
@@ -401,8 +446,7 @@ int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint
     zkh::Fe m1 = zkh::neg(zkh::FR, zkh::one(zkh::FR));
     if (!zkh::eq(h, m1)) return set_err(c, ZK_ERR_INVALID, "omega is not a primitive n-th root of unity");
   }
-  if (inverse) w = zkh::inv(zkh::FR, w);
-
+  // one table per (n, w) serves both directions
   const uint32_t* tw;
   int rc = get_twiddles(c, log_n, w, &tw, st);
   if (rc) return rc;
@@ -463,15 +507,28 @@ int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint
   for (size_t i = 0; i < P; i++) {
     const uint32_t k = ks[i];
     const uint64_t cols = n >> k;
-    uint32_t log_t = NTT_TILE_BITS - k;
+    const bool big = k > NTT_TILE_BITS || c->ntt_tile_bits == NTT_BIG_TILE_BITS;
+    const uint32_t tb = big ? NTT_BIG_TILE_BITS : NTT_TILE_BITS;
+    uint32_t log_t = tb - k;
     while ((1ull << log_t) > cols) log_t--;
     const uint64_t tiles = cols >> log_t;
+    uint32_t xg = c->ntt_xcd_group ? (uint32_t)c->ntt_xcd_group : (log_t <= 1 ? 8u : 1u);
+    if (xg > 1 && tiles % xg != 0) xg = 1;
     const fe* s = (i == 0) ? in : dst[i - 1];
     const uint64_t len = (i == 0) ? in_len : n;
     dim3 grid((unsigned)tiles, batch);
-    ntt_pass_kernel<<<grid, NTT_THREADS, 0, st>>>(
-        s, dst[i], tw, log_n, s0, k, log_t, len, i == 0 ? pre_mode : 0, pre_hi, pre_lo,
-        i == P - 1 ? post_mode : 0, post_hi, post_lo, i == P - 1 ? 1 : 0);
+#define ZK_NTT_PASS(TBV, INVV)                                                                          \
+  ntt_pass_kernel<TBV, INVV><<<grid, NttTile<TBV>::THREADS, 0, st>>>(                                   \
+      s, dst[i], tw, log_n, s0, k, log_t, len, i == 0 ? pre_mode : 0, pre_hi, pre_lo,                   \
+      i == P - 1 ? post_mode : 0, post_hi, post_lo, i == P - 1 ? 1 : 0, xg)
+    if (big) {
+      if (inverse) ZK_NTT_PASS(NTT_BIG_TILE_BITS, true);
+      else ZK_NTT_PASS(NTT_BIG_TILE_BITS, false);
+    } else {
+      if (inverse) ZK_NTT_PASS(NTT_TILE_BITS, true);
+      else ZK_NTT_PASS(NTT_TILE_BITS, false);
+    }
+#undef ZK_NTT_PASS
     ZK_HIP(c, hipGetLastError());
     s0 += k;
   }
@@ -490,7 +547,6 @@ int ntt_prepare(Ctx* c, uint32_t log_n, bool inverse, const zkh::Fe* omega, cons
   if (log_n > 28) return set_err(c, ZK_ERR_INVALID, "domain too large for BN254 root of unity");
   const uint64_t n = 1ull << log_n;
   zkh::Fe w = omega ? *omega : standard_root(log_n);
-  if (inverse) w = zkh::inv(zkh::FR, w);
   const uint32_t* tw;
   int rc = get_twiddles(c, log_n, w, &tw, st);
   if (rc) return rc;

@@ -44,11 +44,12 @@ def test_ntt_2_20_bit_exact(zctx, variant):
     dst.free()
 
 
-@pytest.mark.parametrize("pass_bits", [0, 10])
-def test_ntt_2_20_extreme_inputs(zctx, pass_bits):
+@pytest.mark.parametrize("pass_bits,tile_bits", [(0, 0), (10, 0), (12, 12)])
+def test_ntt_2_20_extreme_inputs(zctx, pass_bits, tile_bits):
     """Buffers may hold any 256-bit Montgomery word, canonical or not.  The
     largest ones (2^256 - 1, r - 1, v + r) with passes of ten stages give the
-    largest intermediate values the lazily reduced butterflies see."""
+    largest intermediate values the lazily reduced butterflies see (on the
+    2^10- and on the 2^12-element tiles)."""
     log_n = 20
     n = 1 << log_n
     rng = np.random.default_rng(0x2003)
@@ -67,11 +68,37 @@ def test_ntt_2_20_extreme_inputs(zctx, pass_bits):
     src = zctx.upload(raw)
     dst = zctx.alloc(32 * n)
     zctx.set_tuning("ntt_max_pass_bits", pass_bits)
+    zctx.set_tuning("ntt_tile_bits", tile_bits)
     try:
         for inv, shift in ((False, None), (True, FR_GENERATOR)):
             zctx.ntt(src.ptr, n, dst.ptr, log_n, inv, shift_mont=fr_to_mont(shift) if shift else None)
             got = _dev_words(zctx, dst.ptr, n).reshape(-1)
             expect = cref.ntt_words(plain, n, log_n, N.omega_for_size(n), inv, shift, threads=0)
+            assert np.array_equal(got, expect), (inv, pass_bits)
+    finally:
+        zctx.set_tuning("ntt_max_pass_bits", 0)
+        zctx.set_tuning("ntt_tile_bits", 0)
+    src.free()
+    dst.free()
+
+
+@pytest.mark.parametrize("pass_bits", [0, 12])
+def test_ntt_2_24_bit_exact(zctx, pass_bits):
+    """2^24 against the C oracle in full (forward and coset inverse), with the
+    default three passes of eight stages and with two passes of twelve on
+    the 2^12-element tiles."""
+    log_n = 24
+    n = 1 << log_n
+    a = workload.fr_uniform(0x2002, n)
+    src = workload.device_fr_mont(zctx, a)
+    dst = zctx.alloc(32 * n)
+    zctx.set_tuning("ntt_max_pass_bits", pass_bits)
+    try:
+        for inv, shift in ((False, None), (True, FR_GENERATOR)):
+            print(f"2^24 pass_bits={pass_bits} inverse={inv}", flush=True)
+            zctx.ntt(src.ptr, n, dst.ptr, log_n, inv, shift_mont=fr_to_mont(shift) if shift else None)
+            got = _dev_words(zctx, dst.ptr, n).reshape(-1)
+            expect = cref.ntt_words(a.reshape(-1), n, log_n, N.omega_for_size(n), inv, shift, threads=0)
             assert np.array_equal(got, expect), (inv, pass_bits)
     finally:
         zctx.set_tuning("ntt_max_pass_bits", 0)

@@ -340,7 +340,7 @@ def test_madd_chain_lazy(host29):
         assert _aff29(got) == acc
 
 
-@pytest.mark.parametrize("log_n", [1, 2, 3, 5, 8, 9, 10])
+@pytest.mark.parametrize("log_n", [1, 2, 3, 5, 8, 9, 10, 12])
 def test_ntt_tile_carry_free(host29, log_n):
     """The NTT pass arithmetic (carry-free butterflies after the first stage,
     u normalised at every even stage) against the DFT of the input words,
