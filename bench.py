@@ -151,8 +151,10 @@ def measured_traffic(workload: str, kernel: str):
             t = json.load(f)[workload][kernel]
     except (OSError, KeyError, ValueError):
         return None
-    return {"bytes": t["read_bytes"] + t["write_bytes"], "read_bytes": t["read_bytes"],
-            "write_bytes": t["write_bytes"], "source": "profiles/hbm_traffic.json (rocprofv3 --pmc)"}
+    k = int(t.get("launches_per_transform", 1))   # an NTT transform is several pass launches
+    return {"bytes": k * (t["read_bytes"] + t["write_bytes"]), "read_bytes": k * t["read_bytes"],
+            "write_bytes": k * t["write_bytes"], "launches": k,
+            "source": "profiles/hbm_traffic.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, calibrated)"}
 
 
 def ntt_block(ctx, log_n, steps, warmup):
