@@ -278,7 +278,11 @@ int zk_g1_mul_host(const uint64_t p[8], const uint64_t k[4], uint64_t out[8]);
 int zk_pairing_product(const uint64_t* g1, const uint64_t* g2, uint32_t n, uint64_t* out48);
 int zk_pairing_check(const uint64_t* g1, const uint64_t* g2, uint32_t n, int* ok);
 
-/* Tuning knobs: "ntt_max_pass_bits" (stages per NTT pass, 1..11; 0 = auto),
+/* Tuning knobs: "ntt_max_pass_bits" (stages per NTT pass, 1..12, passes of
+ * 11 or 12 stages on 2^12-element tiles; 0 = auto), "ntt_tile_bits" (0, 10
+ * or 12), "ntt_xcd_group" (tile numbering per XCD; 0 = auto),
+ * "ntt_table_cap_mb" (bytes of cached NTT tables a context keeps before
+ * freeing the least recently used unpinned ones; 0 = no cap, default 16384),
  * "msm_segment" (pairs per accumulation thread), "msm_reduce_chunk"
  * (buckets per reduction thread).  No counterpart in the reference. */
 int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value);

@@ -170,10 +170,10 @@ int zk_destroy(zk_ctx* ctx) {
   hipSetDevice(ctx->c.device);
   hipDeviceSynchronize();
   collect(&ctx->c);
-  for (auto& kv : ctx->c.twiddles) hipFree(kv.second.ptr);
+  for (auto& kv : ctx->c.twiddles) hipFree(kv.second.a.ptr);
   for (auto& kv : ctx->c.pow_tables) {
-    hipFree(kv.second.first.ptr);
-    hipFree(kv.second.second.ptr);
+    hipFree(kv.second.a.ptr);
+    hipFree(kv.second.b.ptr);
   }
   for (auto& b : ctx->c.scratch)
     if (b.ptr) hipFree(b.ptr);
@@ -502,6 +502,9 @@ int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value) {
   if (k == "ntt_max_pass_bits") {
     if (value < 0 || value > 12) return set_err(&ctx->c, ZK_ERR_INVALID, "ntt_max_pass_bits must be in [0, 12]");
     ctx->c.ntt_max_pass_bits = (int)value;
+  } else if (k == "ntt_table_cap_mb") {
+    if (value < 0) return set_err(&ctx->c, ZK_ERR_INVALID, "ntt_table_cap_mb must be >= 0");
+    ctx->c.table_cap = (size_t)value << 20;
   } else if (k == "ntt_tile_bits") {
     if (value != 0 && value != 10 && value != 12)
       return set_err(&ctx->c, ZK_ERR_INVALID, "ntt_tile_bits must be 0, 10 or 12");

@@ -32,15 +32,33 @@ struct PhaseTimer {
   uint64_t launches = 0;
 };
 
+// A cached NTT table (twiddles use `a` only; power tables are the pair
+// (hi, lo)).  `last_call` is the table call that used it last; `pinned`
+// tables (built for a batch plan, whose recorded launches keep their
+// addresses) are never evicted.
+struct CachedTable {
+  DevBuf a, b;
+  uint64_t last_call = 0;
+  bool pinned = false;
+  size_t bytes() const { return a.bytes + b.bytes; }
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
   // NTT twiddle tables omega^e, e <= n/2 (both directions), keyed by (log_n, omega Montgomery words)
-  std::map<std::array<uint64_t, 5>, DevBuf> twiddles;
+  std::map<std::array<uint64_t, 5>, CachedTable> twiddles;
   // two-level power tables (hi, lo) for coset / inverse scaling,
   // keyed by (log_n, base words, scale words)
-  std::map<std::array<uint64_t, 9>, std::pair<DevBuf, DevBuf>> pow_tables;
+  std::map<std::array<uint64_t, 9>, CachedTable> pow_tables;
+  // table cache accounting: bytes held, the byte cap (0: none; least
+  // recently used unpinned tables are freed to stay under it) and the
+  // number of the current table call (tables it uses are never freed)
+  size_t table_bytes = 0;
+  size_t table_cap = 16ull << 30;
+  uint64_t table_call = 0;
+  bool table_pin = false;  // tables fetched now are pinned (batch plans)
   // growable scratch arenas
   DevBuf scratch[8];  // 0-1 NTT, 2-3 MSM, 4 counters, 5 powers, 6 inversion, 7 PLONK quotient
   // tuning knobs (set through zk_set_tuning)

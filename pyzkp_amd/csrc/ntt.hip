@@ -319,6 +319,50 @@ static void build_pow_tables(const zkh::Fe& base, const zkh::Fe& scale, uint64_t
   }
 }
 
+// Frees least recently used, unpinned tables that the current table call
+// has not touched until `need` more bytes fit under the cap (or nothing is
+// left to free).  Launches still in flight may read a table, so the device
+// is drained before the first free.
+static int make_table_room(Ctx* c, size_t need) {
+  if (c->table_cap == 0) return ZK_OK;
+  bool drained = false;
+  while (c->table_bytes + need > c->table_cap) {
+    uint64_t oldest = UINT64_MAX;
+    int which = -1;
+    std::array<uint64_t, 5> kt{};
+    std::array<uint64_t, 9> kp{};
+    for (auto& kv : c->twiddles)
+      if (!kv.second.pinned && kv.second.last_call != c->table_call && kv.second.last_call < oldest) {
+        oldest = kv.second.last_call;
+        which = 0;
+        kt = kv.first;
+      }
+    for (auto& kv : c->pow_tables)
+      if (!kv.second.pinned && kv.second.last_call != c->table_call && kv.second.last_call < oldest) {
+        oldest = kv.second.last_call;
+        which = 1;
+        kp = kv.first;
+      }
+    if (which < 0) break;  // everything left is pinned or in use: go over the cap
+    if (!drained) {
+      ZK_HIP(c, hipDeviceSynchronize());
+      drained = true;
+    }
+    CachedTable t = which == 0 ? c->twiddles[kt] : c->pow_tables[kp];
+    if (t.a.ptr) ZK_HIP(c, hipFree(t.a.ptr));
+    if (t.b.ptr) ZK_HIP(c, hipFree(t.b.ptr));
+    c->table_bytes -= t.bytes();
+    if (which == 0) c->twiddles.erase(kt);
+    else c->pow_tables.erase(kp);
+  }
+  return ZK_OK;
+}
+
+static void touch(Ctx* c, CachedTable& t) {
+  t.last_call = c->table_call;
+  if (c->table_pin) t.pinned = true;
+}
+
 static int get_pow_tables(Ctx* c, uint32_t log_n, const zkh::Fe& base, const zkh::Fe& scale,
                           const fe** hi, const fe** lo, hipStream_t st) {
   std::array<uint64_t, 9> key;
@@ -329,23 +373,31 @@ static int get_pow_tables(Ctx* c, uint32_t log_n, const zkh::Fe& base, const zkh
   if (it == c->pow_tables.end()) {
     std::vector<zkh::Fe> h, l;
     build_pow_tables(base, scale, 1ull << log_n, h, l);
-    std::pair<DevBuf, DevBuf> bufs;
-    int rc = upload_table(c, h, &bufs.first, st);
+    int rc = make_table_room(c, (h.size() + l.size()) * 32);
     if (rc) return rc;
-    rc = upload_table(c, l, &bufs.second, st);
+    CachedTable t;
+    rc = upload_table(c, h, &t.a, st);
     if (rc) return rc;
-    table_to261_kernel<<<grid_for(h.size(), 256), 256, 0, st>>>((fe*)bufs.first.ptr, h.size());
-    table_to261_kernel<<<grid_for(l.size(), 256), 256, 0, st>>>((fe*)bufs.second.ptr, l.size());
+    rc = upload_table(c, l, &t.b, st);
+    if (rc) {
+      hipFree(t.a.ptr);
+      return rc;
+    }
+    table_to261_kernel<<<grid_for(h.size(), 256), 256, 0, st>>>((fe*)t.a.ptr, h.size());
+    table_to261_kernel<<<grid_for(l.size(), 256), 256, 0, st>>>((fe*)t.b.ptr, l.size());
     ZK_HIP(c, hipGetLastError());
-    it = c->pow_tables.emplace(key, bufs).first;
+    c->table_bytes += t.bytes();
+    it = c->pow_tables.emplace(key, t).first;
   }
-  *hi = (const fe*)it->second.first.ptr;
-  *lo = (const fe*)it->second.second.ptr;
+  touch(c, it->second);
+  *hi = (const fe*)it->second.a.ptr;
+  *lo = (const fe*)it->second.b.ptr;
   return ZK_OK;
 }
 
 int ntt_pow_tables(Ctx* c, uint32_t log_n, const zkh::Fe& base, const zkh::Fe& scale, const fe** hi,
                    const fe** lo, hipStream_t st) {
+  c->table_call++;
   return get_pow_tables(c, log_n, base, scale, hi, lo, st);
 }
 
@@ -357,14 +409,18 @@ static int get_twiddles(Ctx* c, uint32_t log_n, const zkh::Fe& w, const uint32_t
     const fe *hi, *lo;
     int rc = get_pow_tables(c, log_n, w, zkh::one(zkh::FR), &hi, &lo, st);
     if (rc) return rc;
-    DevBuf b;
-    b.bytes = count * 36;
-    ZK_HIP(c, hipMalloc(&b.ptr, b.bytes));
-    pow_expand_kernel<<<grid_for(count, 256), 256, 0, st>>>(hi, lo, (uint32_t*)b.ptr, count);
+    rc = make_table_room(c, count * 36);
+    if (rc) return rc;
+    CachedTable t;
+    t.a.bytes = count * 36;
+    ZK_HIP(c, hipMalloc(&t.a.ptr, t.a.bytes));
+    pow_expand_kernel<<<grid_for(count, 256), 256, 0, st>>>(hi, lo, (uint32_t*)t.a.ptr, count);
     ZK_HIP(c, hipGetLastError());
-    it = c->twiddles.emplace(key, b).first;
+    c->table_bytes += t.bytes();
+    it = c->twiddles.emplace(key, t).first;
   }
-  *out = (const uint32_t*)it->second.ptr;
+  touch(c, it->second);
+  *out = (const uint32_t*)it->second.a.ptr;
   return ZK_OK;
 }
 
@@ -429,6 +485,7 @@ int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint
             int max_pass_bits, fe* tmp_buf) {
   if (log_n > 28) return set_err(c, ZK_ERR_INVALID, "domain too large for BN254 root of unity");
   if (batch == 0) return ZK_OK;
+  c->table_call++;
   const uint64_t n = 1ull << log_n;
   if (batch > 1 && in_len != n)
     return set_err(c, ZK_ERR_INVALID, "batched NTT needs in_len == n");
@@ -547,13 +604,15 @@ int ntt_prepare(Ctx* c, uint32_t log_n, bool inverse, const zkh::Fe* omega, cons
   if (log_n > 28) return set_err(c, ZK_ERR_INVALID, "domain too large for BN254 root of unity");
   const uint64_t n = 1ull << log_n;
   zkh::Fe w = omega ? *omega : standard_root(log_n);
+  // a plan's recorded launches keep these tables' addresses: pin them
+  c->table_call++;
+  c->table_pin = true;
   const uint32_t* tw;
   int rc = get_twiddles(c, log_n, w, &tw, st);
-  if (rc) return rc;
   const fe *h, *l;
-  if (!inverse) {
+  if (rc == ZK_OK && !inverse) {
     if (shift) rc = get_pow_tables(c, log_n, *shift, zkh::one(zkh::FR), &h, &l, st);
-  } else {
+  } else if (rc == ZK_OK) {
     zkh::Fe ninv = zkh::inv(zkh::FR, zkh::from_u64(zkh::FR, n));
     if (shift) {
       zkh::Fe ginv = zkh::is_zero(*shift) ? zkh::zero() : zkh::inv(zkh::FR, *shift);
@@ -562,6 +621,7 @@ int ntt_prepare(Ctx* c, uint32_t log_n, bool inverse, const zkh::Fe* omega, cons
       rc = get_pow_tables(c, 0, zkh::zero(), ninv, &h, &l, st);
     }
   }
+  c->table_pin = false;
   return rc;
 }
 }  // namespace zk

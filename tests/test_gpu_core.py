@@ -193,3 +193,27 @@ def test_msm_golden_commitments(zctx):
         sc = upload_fr_mont(zctx, coeffs)
         expect = tuple(pk["commits"][name])
         assert msm_affine(zctx, bases, sc, len(coeffs)) == expect, name
+
+
+def test_ntt_table_cap_evicts_and_rebuilds(zctx):
+    """With a 1 MiB cap on cached NTT tables every new size evicts the least
+    recently used tables; transforms stay exact when their tables are
+    rebuilt."""
+    zctx.set_tuning("ntt_table_cap_mb", 1)
+    try:
+        for rep in range(2):
+            for log_n in (14, 12, 15, 13):
+                n = 1 << log_n
+                rng = random.Random(900 + log_n + rep)
+                a = [rng.randrange(R_MOD) for _ in range(n)]
+                w = N.omega_for_size(n)
+                src = upload_fr_mont(zctx, a)
+                dst = zctx.alloc(32 * n)
+                zctx.ntt(src.ptr, n, dst.ptr, log_n, False, shift_mont=_mont(5))
+                assert download_fr(zctx, dst.ptr, n) == N.evals_from_coeffs_on_coset(a, n=n, omega=w, shift=5)
+                zctx.ntt(src.ptr, n, dst.ptr, log_n, True)
+                assert download_fr(zctx, dst.ptr, n) == N.coeffs_from_evals_on_roots(a, w)
+                src.free()
+                dst.free()
+    finally:
+        zctx.set_tuning("ntt_table_cap_mb", 16384)
