@@ -195,8 +195,12 @@ class ProveServer:
             raise ValueError("the server needs at least one device slot")
         self.pool = _HostPool(g2_workers)
         self.assembler = ThreadPoolExecutor(max_workers=assemblers)
+        self._reset_stats()
+
+    def _reset_stats(self) -> None:
         self.stats = {"groups": 0, "batches": 0, "msm_launches": 0, "max_batch_seen": 0,
-                      "max_concurrent_groups": 0, "peak_inflight_bytes": 0, "device_slots": len(self.slots)}
+                      "max_concurrent_groups": 0, "peak_inflight_bytes": 0, "device_slots": len(self.slots),
+                      "workspace_bytes": sum(s.workspace_bytes() for s in self.slots)}
 
     def close(self) -> None:
         self.pool.shutdown()
@@ -211,7 +215,8 @@ class ProveServer:
     # ------------------------------------------------------------------ run
     def run(self, requests: List[Request]) -> List[Result]:
         """Feeds `requests` at their arrival times, serves them and returns
-        the results in request order."""
+        the results in request order.  `stats` describes this run; the
+        slots (contexts, NTT tables, workspaces) persist across runs."""
         reqs = sorted(requests, key=lambda q: q.arrival)
         for q in reqs:
             if q.key not in self.keys:
@@ -219,7 +224,10 @@ class ProveServer:
             if self.footprint(q) > self.budget:
                 raise ValueError(f"request {q.id} needs {self.footprint(q)} device bytes, above the HBM budget "
                                  f"{self.budget}")
-        return _Run(self, reqs).serve(requests)
+        self._reset_stats()
+        out = _Run(self, reqs).serve(requests)
+        self.stats["workspace_bytes"] = sum(s.workspace_bytes() for s in self.slots)
+        return out
 
 
 class _Run:
