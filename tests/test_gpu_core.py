@@ -121,6 +121,15 @@ def test_msm_edge_cases(zctx):
     assert msm_affine(zctx, bases, sc0, 7, 5) is None
     sc2 = upload_fr_mont(zctx, [1, 0, 0, 1, 0, 0, 0])
     assert msm_affine(zctx, bases, sc2, 7, 5) is None
+    # the same cases on a prepared (converted once) base set
+    from pyzkp_amd.curve import from_jacobian_mont
+    mb = zctx.msm_bases_create(bases.ptr, len(pts))
+    aff = lambda r: None if r[2] == 0 else (r[0], r[1])
+    for c in (2, 8, 16):
+        assert aff(from_jacobian_mont(*zctx.msm_g1_prepared(mb, sc.ptr, len(pts), c))) == expect
+    assert aff(from_jacobian_mont(*zctx.msm_g1_prepared(mb, sc2.ptr, 7, 5))) is None
+    assert zctx.msm_g1_prepared(mb, sc.ptr, 0, 0)[2] == 0          # empty MSM: the identity
+    mb.free()
 
 
 def test_msm_same_scalar_everywhere(zctx):
@@ -217,3 +226,40 @@ def test_ntt_table_cap_evicts_and_rebuilds(zctx):
                 dst.free()
     finally:
         zctx.set_tuning("ntt_table_cap_mb", 16384)
+
+
+def test_batch_plan_tables_survive_table_cap(zctx):
+    """Tables a batch plan's recorded launches point at are pinned: a tiny
+    table cap and other sizes' transforms in between do not free them."""
+    nb = 1 << 10
+    rng = random.Random(77)
+    fbk = [rng.randrange(1, R_MOD) for _ in range(nb)]
+    bases = upload_bases(zctx, _bases(fbk))
+    fb = zctx.fixed_bases_create(bases.ptr, nb, 8)
+    ss = [rng.randrange(R_MOD) for _ in range(nb)]
+    sc = upload_fr_mont(zctx, ss)
+    n = 1 << 12
+    a = [rng.randrange(R_MOD) for _ in range(n)]
+    src, dst = upload_fr_mont(zctx, a), zctx.alloc(32 * n)
+    zctx.set_tuning("ntt_table_cap_mb", 1)
+    try:
+        plan = zctx.batch_create(fb, sc.ptr, [0], [nb], [{"in": src.ptr, "out": dst.ptr, "log_n": 12,
+                                                            "batch": 1, "inverse": True}], use_graph=True)
+        for log_n in (15, 14, 13):                                     # other tables, evicting
+            m = 1 << log_n
+            t = upload_fr_mont(zctx, [1] * m)
+            zctx.ntt(t.ptr, m, t.ptr, log_n, False)
+            zctx.sync()
+            t.free()
+        for _ in range(2):
+            res = plan.run()
+            assert download_fr(zctx, dst.ptr, n) == N.coeffs_from_evals_on_roots(a, N.omega_for_size(n))
+        from pyzkp_amd.curve import from_jacobian_mont
+        p = from_jacobian_mont(*res[0])
+        assert (None if p[2] == 0 else (p[0], p[1])) == M.msm_closed_form(fbk, ss)
+        plan.free()
+    finally:
+        zctx.set_tuning("ntt_table_cap_mb", 16384)
+    for b in (bases, sc, src, dst):
+        b.free()
+    fb.free()

@@ -48,6 +48,21 @@ def test_noalloc_matches_msm_g1(zctx, shapes):
         scal.free()
 
 
+def test_noalloc_skewed_and_repeated(zctx):
+    """Skewed scalars (zeros, small values) and repeated bases: large buckets
+    spanning many segments, and the doubling case, on the static arena."""
+    n = 1 << 14
+    ks = workload.repeat_some(0x7204, workload.fr_uniform(0x7202, n), 0.05)
+    ss = workload.scalars_skewed(0x7203, n)
+    bases, scal = workload.device_bases(zctx, ks), workload.device_fr_mont(zctx, ss)
+    for c in (8, 13):
+        rc, out = _noalloc(zctx, bases, scal, n, c)
+        assert rc == 0
+        assert _affine(out) == M.msm_closed_form(words_to_ints(ks), words_to_ints(ss)), c
+    bases.free()
+    scal.free()
+
+
 def test_noalloc_refuses_oversized_shape(zctx):
     n = 1 << 17
     ks, ss, bases, scal = _inputs(zctx, n, 0x7100)
