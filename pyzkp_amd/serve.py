@@ -155,7 +155,10 @@ class _Slot:
             if job is None:
                 return
             fn, args = job
-            fn(self, *args)
+            try:
+                fn(self, *args)
+            except BaseException:       # jobs record their own errors; the slot must keep serving
+                pass
 
     def close(self) -> None:
         self._q.put(None)
@@ -310,7 +313,12 @@ class _Run:
             self.cv.notify_all()
 
     def _finish(self, q: Request, proof, err) -> None:
+        """Records q's result (lock held).  A request that failed before its
+        device group formed leaves the pending list, so no device work is
+        started for it."""
         self.results[q.id] = Result(q.id, proof, q.arrival, self.admitted.get(q.id, 0.0), self.now(), err)
+        if proof is None:
+            self.pending = [p for p in self.pending if p.id != q.id]
 
     # ---- device groups
     def _ready_group(self) -> Optional[List[Request]]:
@@ -392,7 +400,9 @@ class _Run:
         feeder.start()
         total = len(self.reqs)
         with self.cv:
-            while len(self.results) < total:
+            # done when every request has its result and no group is still on
+            # a slot (so no device work of this run outlives it)
+            while len(self.results) < total or self.busy:
                 group = self._ready_group()
                 if group is not None:
                     self._launch(group)

@@ -166,3 +166,22 @@ def test_classes_and_arrivals_deterministic():
     t = S.poisson_arrivals(32, 5.0)
     assert all(t[i] < t[i + 1] for i in range(31))
     assert 2.0 < t[-1] < 20.0
+
+
+def test_host_g2_failure_fails_only_its_request(fake, monkeypatch):
+    real = S.P.g2_part
+
+    def g2_part(pk, w, threads):
+        if int(w[0, 0]) == 1003:
+            raise RuntimeError("host G2 failure")
+        return real(pk, w, threads)
+
+    monkeypatch.setattr(S.P, "g2_part", g2_part)
+    keys = {6: _Key(64, 70)}
+    reqs = _reqs(keys, [6] * 6)
+    srv = _server(keys, 1 << 40, max_batch=2)
+    res = srv.run(reqs)
+    srv.close()
+    assert res[3].proof is None and "host G2 failure" in res[3].error
+    assert all(r.error is None for i, r in enumerate(res) if i != 3)
+    assert srv.stats["groups"] >= 1
