@@ -221,3 +221,29 @@ def test_kernel_without_outputs_is_an_error():
     g.add_node(op=OpType.BATCH_INV, inputs=["x"], outputs=["y"])
     with pytest.raises(ValueError):
         Executor(registry=r).run(g)
+
+
+def test_graph_runs_twice_without_reanalysis_and_cache_invalidates():
+    """The reference's runtime graph tests (tests/correctness/cpu/test_runtime_graph.py,
+    test_runtime_graph_batch.py): out-of-order nodes run in dependency order,
+    a graph runs twice in a row without a reset, and adding a node replaces
+    the cached analysis."""
+    log = []
+    r = _fake_registry(log)
+    g = Graph()
+    g.add_buffer(id="a", device=Device.CPU, dtype=DType.FR, data=[2, 3, 4, 5])
+    g.add_node(op=OpType.BATCH_INV, inputs=["c"], outputs=["d"])
+    g.add_node(op=OpType.BATCH_INV, inputs=["a"], outputs=["c"])
+    exe = Executor(registry=r)
+    ctx = DeviceContext(backend=Backend.HIP, device=Device.HIP)
+    a1 = g.analyze_cached()
+    exe.run(g, context=ctx)
+    assert g.buffers["d"].data == [8, 12, 16, 20]
+    exe.run(g, context=ctx)
+    assert g.buffers["d"].data == [8, 12, 16, 20]
+    assert g.analyze_cached() is a1
+    g.add_node(op=OpType.BATCH_INV, inputs=["d"], outputs=["e"])
+    a2 = g.analyze_cached()
+    assert a2 is not a1 and a2.topo_order != a1.topo_order
+    exe.run(g, context=ctx)
+    assert g.buffers["e"].data == [16, 24, 32, 40]
