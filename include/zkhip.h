@@ -262,6 +262,13 @@ int zk_g2_msm_host(const uint64_t* bases, const uint64_t* scalars, uint64_t n, i
                    uint64_t out[16]);
 int zk_g2_mul_fixed_host(const uint64_t* q, const uint64_t* scalars, uint64_t n, int threads, uint64_t* out);
 int zk_g2_add_host(const uint64_t* p, const uint64_t* q, uint64_t out[16]);
+/* Selects the host G2 MSM's arithmetic: mode 1 (the default) runs its
+ * batched affine bucket additions eight at a time on AVX-512 IFMA where the
+ * CPU has AVX-512F and IFMA, 0 runs them one at a time on 64-bit scalar
+ * words, any other mode only asks.  Returns 1 if the IFMA path is in use
+ * after the call.  Both give the same points.  No counterpart in the
+ * reference (its G2 MSM is Python, crypto/msm/pippenger.py:67-121). */
+int zk_host_simd(int mode);
 
 /* out = [k] P on the host for one G1 point (affine canonical, 8 words, zeros
  * = infinity; k canonical).  Replaces the g1_mul calls of proof assembly

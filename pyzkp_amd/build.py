@@ -41,6 +41,15 @@ def _run(cmd: list[str]) -> None:
         raise RuntimeError(f"command failed: {' '.join(cmd)}\n{p.stdout}\n{p.stderr}")
 
 
+# Host-only translation units, built with the host compiler and their own
+# code generation flags: the CPU feature check (none) and the host G2 MSM's
+# AVX-512 IFMA accumulation, entered only after that check.
+HOST_SRCS = {
+    "host_cpu.cpp": [],
+    "g2_simd.cpp": ["-mavx512f", "-mavx512ifma"],
+}
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     hipcc = _hipcc()
@@ -51,6 +60,10 @@ def build(verbose: bool = False) -> str:
         obj = os.path.join(BUILD, f[:-4] + ".o")
         objs.append(obj)
         cmds.append([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, f), "-o", obj])
+    for f, extra in HOST_SRCS.items():
+        obj = os.path.join(BUILD, f.rsplit(".", 1)[0] + ".o")
+        objs.append(obj)
+        cmds.append(["g++", "-O3", "-std=c++17", "-fPIC", *extra, "-c", os.path.join(CSRC, f), "-o", obj])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
         list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"

@@ -10,8 +10,12 @@
 // with all-zero for infinity, 16 x u64 per point.
 #include <string.h>
 #include <atomic>
+#include <exception>
+#include <mutex>
+#include <new>
 #include <thread>
 #include <vector>
+#include "g2_simd.hpp"
 #include "internal.hpp"
 
 namespace zk {
@@ -100,7 +104,7 @@ static inline Fe redc(u64* t) {
   }
   // (T + M q) / 2^256 < 2 q < 2^256, so hi is 0 here
   Fe r = {{t[4], t[5], t[6], t[7]}};
-  if (zkh::geq(r.v, QW)) zkh::sub_raw(r.v, r.v, QW);
+  zkh::csub(r.v, QW);
   return r;
 }
 // a + b without reduction (< 2 q for canonical inputs)
@@ -267,6 +271,8 @@ static void wr_jac_affine(uint64_t* out, const J2& p) {
   wr_fq(out + 12, y.c1);
 }
 
+// fn(0 .. n-1) on up to `threads` threads; an exception thrown by fn stops
+// the remaining items and is rethrown here
 template <class F>
 static void parallel_for(uint64_t n, int threads, F fn) {
   if (threads <= 1 || n <= 1) {
@@ -275,15 +281,24 @@ static void parallel_for(uint64_t n, int threads, F fn) {
   }
   std::vector<std::thread> ts;
   std::atomic<uint64_t> next(0);
+  std::exception_ptr err;
+  std::mutex err_mu;
   for (int t = 0; t < threads; t++)
     ts.emplace_back([&]() {
       for (;;) {
         uint64_t i = next.fetch_add(1);
         if (i >= n) break;
-        fn(i);
+        try {
+          fn(i);
+        } catch (...) {
+          std::lock_guard<std::mutex> g(err_mu);
+          if (!err) err = std::current_exception();
+          next.store(n);
+        }
       }
     });
   for (auto& t : ts) t.join();
+  if (err) std::rethrow_exception(err);
 }
 
 static uint32_t bits_at(const uint64_t* s, uint32_t off, uint32_t c) {
@@ -406,15 +421,62 @@ static J2 bucket_task(const A2* pts, const int32_t* digits, uint64_t n_all, cons
   return sum;
 }
 
+// The same bucket sums on the AVX-512 IFMA path: the batched affine
+// additions run eight at a time (g2_simd.cpp), and the weighted bucket sum
+// runs as above.
+// An Fq value of the IFMA path (five 52-bit limbs, Montgomery form with
+// R = 2^260, canonical) in this file's form (R = 2^256): x 2^260 times 2^252
+// over 2^256 by one Montgomery product.
+static inline Fe from_l5(const uint64_t* l) {
+  Fe x, s = {{0, 0, 0, 1ull << 60}};  // 2^252 < q
+  x.v[0] = l[0] | (l[1] << 52);
+  x.v[1] = (l[1] >> 12) | (l[2] << 40);
+  x.v[2] = (l[2] >> 24) | (l[3] << 28);
+  x.v[3] = (l[3] >> 36) | (l[4] << 16);
+  return zkh::mul(FQ, x, s);
+}
+
+static J2 bucket_task_simd(const uint64_t* pts20, const uint8_t* pinf, const int32_t* digits, uint64_t n_all,
+                           const BucketTask& t, int c) {
+  const uint32_t nb = 1u << (c - 1);
+  std::vector<uint64_t> bk((size_t)nb * g2simd::PT_WORDS);
+  std::vector<uint8_t> empty(nb, 1);
+  if (g2simd::accumulate(pts20, pinf, digits + (uint64_t)t.w * n_all, t.i0, t.i1, nb, bk.data(), empty.data()))
+    throw std::bad_alloc();
+  J2 run = j_inf(), sum = j_inf();
+  for (int b = (int)nb - 1; b >= 0; b--) {
+    if (!empty[b]) {
+      const uint64_t* B = bk.data() + (size_t)g2simd::PT_WORDS * b;
+      const F2 x = {from_l5(B), from_l5(B + 5)}, y = {from_l5(B + 10), from_l5(B + 15)};
+      run = j_madd(run, x, y);
+    }
+    sum = j_add(sum, run);
+  }
+  return sum;
+}
+
+// 1: the AVX-512 IFMA path where the CPU has it; 0: the scalar path
+static std::atomic<int> g_simd_mode(1);
+static bool simd_on() {
+  static const bool has = g2simd::cpu_has_ifma();
+  return has && g_simd_mode.load() != 0;
+}
+
 static void msm(const uint64_t* bases, const uint64_t* scalars, uint64_t n, int c, int threads,
                 uint64_t out[16]) {
   const uint32_t W = (255 + c - 1) / c;
-  std::vector<A2> pts(n);
+  const bool simd = simd_on();
+  std::vector<A2> pts(simd ? 0 : n);
+  std::vector<uint64_t> pts20(simd ? (size_t)n * g2simd::PT_WORDS : 0);
+  std::vector<uint8_t> pinf(simd ? n : 0);
   std::vector<int32_t> digits((size_t)n * W);
   parallel_for((n + 1023) / 1024, threads, [&](uint64_t blk) {
     const uint64_t e = (blk + 1) * 1024 < n ? (blk + 1) * 1024 : n;
+    if (simd)
+      g2simd::prepare_points(bases + 16 * blk * 1024, e - blk * 1024, pts20.data() + g2simd::PT_WORDS * blk * 1024,
+                             pinf.data() + blk * 1024);
     for (uint64_t i = blk * 1024; i < e; i++) {
-      pts[i] = rd_aff(bases + 16 * i);
+      if (!simd) pts[i] = rd_aff(bases + 16 * i);
       Fe s;
       memcpy(s.v, scalars + 4 * i, 32);
       while (zkh::geq(s.v, zkh::FR.p)) zkh::sub_raw(s.v, s.v, zkh::FR.p);
@@ -445,8 +507,10 @@ static void msm(const uint64_t* bases, const uint64_t* scalars, uint64_t n, int 
   for (uint32_t w = 0; w < W; w++)
     for (uint64_t k = 0; k < S; k++) tasks.push_back({w, n * k / S, n * (k + 1) / S});
   std::vector<J2> part(tasks.size());
-  parallel_for(tasks.size(), threads,
-               [&](uint64_t t) { part[t] = bucket_task(pts.data(), digits.data(), n, tasks[t], c); });
+  parallel_for(tasks.size(), threads, [&](uint64_t t) {
+    part[t] = simd ? bucket_task_simd(pts20.data(), pinf.data(), digits.data(), n, tasks[t], c)
+                   : bucket_task(pts.data(), digits.data(), n, tasks[t], c);
+  });
   J2 acc = j_inf();
   for (int w = (int)W - 1; w >= 0; w--) {
     for (int k = 0; k < c; k++) acc = j_dbl(acc);
@@ -492,7 +556,11 @@ int zk_g2_msm_host(const uint64_t* bases, const uint64_t* scalars, uint64_t n, i
     if (c < 4) c = 4;
   }
   if (c < 2 || c > 20) return ZK_ERR_INVALID;
-  g2::msm(bases, scalars, n, c, threads, out);
+  try {
+    g2::msm(bases, scalars, n, c, threads, out);
+  } catch (const std::bad_alloc&) {
+    return ZK_ERR_RUNTIME;
+  }
   return ZK_OK;
 }
 
@@ -558,6 +626,15 @@ int zk_g2_mul_fixed_host(const uint64_t* q, const uint64_t* scalars, uint64_t n,
     g2::wr_jac_affine(out + 16 * i, acc);
   });
   return ZK_OK;
+}
+
+/* The host G2 MSM's AVX-512 IFMA path: mode 1 uses it where the CPU has
+ * AVX-512F and IFMA (the default), 0 uses the scalar path, any other value
+ * only asks.  Returns 1 if the IFMA path is in use after the call.  Both
+ * paths give the same points. */
+int zk_host_simd(int mode) {
+  if (mode == 0 || mode == 1) g2::g_simd_mode.store(mode);
+  return g2::simd_on() ? 1 : 0;
 }
 
 /* p + q on G2 (affine canonical in / out, 16 words). */

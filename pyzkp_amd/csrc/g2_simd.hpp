@@ -1,0 +1,34 @@
+// Interface between the host G2 MSM (g2_host.hip) and its AVX-512 IFMA
+// bucket accumulation (g2_simd.cpp, the only translation unit built with
+// AVX-512 code generation).  Nothing here names a vector type, so the HIP
+// translation units include it without x86 vector headers; g2_host.hip calls
+// into g2_simd.cpp only after checking that the CPU has AVX-512F and IFMA.
+#pragma once
+#include <stdint.h>
+
+namespace zk {
+namespace g2simd {
+
+// Points in the accumulation's form: affine (x0, x1, y0, y1), each an Fq
+// value in Montgomery form with R = 2^260, canonical (< q), as five 52-bit
+// limbs: 20 words per point.
+constexpr int PT_WORDS = 20;
+
+// The CPU and OS support AVX-512F and AVX-512 IFMA (host_cpu.cpp).
+bool cpu_has_ifma();
+
+// out[20 i ..] <- point i of bases (16 canonical words each, all-zero =
+// infinity), inf[i] <- 1 for the point at infinity (its 20 words are zero).
+void prepare_points(const uint64_t* bases16, uint64_t n, uint64_t* out20, uint8_t* inf);
+
+// Bucket sums of one window over points [i0, i1): digit d = digits[i] adds
+// sign(d) P_i to bucket |d| - 1 of nb.  bk20 (nb x 20 words) and empty (nb)
+// are the buckets, in and out.  Additions are affine, batched so that one
+// inversion serves a whole batch; any digit distribution is handled (a point
+// whose bucket is busy goes to a replica of it, and replicas are added back
+// at the end).  Returns 0, or UINT64_MAX when out of memory.
+uint64_t accumulate(const uint64_t* pts20, const uint8_t* inf, const int32_t* digits, uint64_t i0, uint64_t i1,
+                    uint32_t nb, uint64_t* bk20, uint8_t* empty);
+
+}  // namespace g2simd
+}  // namespace zk

@@ -50,6 +50,25 @@ inline void sub_raw(u64* r, const u64* a, const u64* b) {
 inline bool is_zero(const Fe& a) { return (a.v[0] | a.v[1] | a.v[2] | a.v[3]) == 0; }
 inline bool eq(const Fe& a, const Fe& b) { return memcmp(a.v, b.v, 32) == 0; }
 
+// The reductions below select with masks, not branches: whether a sum or a
+// Montgomery product needs the final subtraction is data-dependent and
+// close to a coin flip, and a mispredicted branch there costs more than the
+// whole product (measured 24.8 vs 3.6 ns per Fq product on 8 independent
+// chains).
+//
+// r <- hi 2^256 + r - p if that is >= 0, else r unchanged (hi is 0 or 1)
+inline void csub(u64* r, const u64* p, u64 hi = 0) {
+  u64 d[4];
+  u64 br = 0;
+  for (int i = 0; i < 4; i++) {
+    const u128 z = (u128)r[i] - p[i] - br;
+    d[i] = (u64)z;
+    br = (u64)(z >> 127);
+  }
+  const u64 keep = (u64)0 - (u64)((hi != 0) | (br == 0));
+  for (int i = 0; i < 4; i++) r[i] = (d[i] & keep) | (r[i] & ~keep);
+}
+
 inline Fe add(const Mod& M, const Fe& a, const Fe& b) {
   Fe r;
   u64 c = 0;
@@ -58,31 +77,34 @@ inline Fe add(const Mod& M, const Fe& a, const Fe& b) {
     r.v[i] = (u64)s;
     c = (u64)(s >> 64);
   }
-  if (geq(r.v, M.p)) sub_raw(r.v, r.v, M.p);
+  csub(r.v, M.p, c);
   return r;
 }
 
 inline Fe sub(const Mod& M, const Fe& a, const Fe& b) {
   Fe r;
-  if (geq(a.v, b.v)) {
-    sub_raw(r.v, a.v, b.v);
-  } else {
-    u64 t[4];
-    sub_raw(t, M.p, b.v);
-    u64 c = 0;
-    for (int i = 0; i < 4; i++) {
-      u128 s = (u128)a.v[i] + t[i] + c;
-      r.v[i] = (u64)s;
-      c = (u64)(s >> 64);
-    }
+  u64 br = 0;
+  for (int i = 0; i < 4; i++) {
+    const u128 z = (u128)a.v[i] - b.v[i] - br;
+    r.v[i] = (u64)z;
+    br = (u64)(z >> 127);
+  }
+  // a < b: add p back (the sum wraps past 2^256 exactly once)
+  const u64 m = (u64)0 - br;
+  u64 c = 0;
+  for (int i = 0; i < 4; i++) {
+    const u128 s = (u128)r.v[i] + (M.p[i] & m) + c;
+    r.v[i] = (u64)s;
+    c = (u64)(s >> 64);
   }
   return r;
 }
 
 inline Fe neg(const Mod& M, const Fe& a) {
-  if (is_zero(a)) return a;
   Fe r;
   sub_raw(r.v, M.p, a.v);
+  const u64 nz = (u64)0 - (u64)!is_zero(a);
+  for (int i = 0; i < 4; i++) r.v[i] &= nz;
   return r;
 }
 
@@ -129,7 +151,7 @@ inline Fe mul_mod(const Fe& a, const Fe& b) {
     t4 = t5 + (u64)(s >> 64);
   }
   Fe r = {{t0, t1, t2, t3}};
-  if (t4 != 0 || geq(r.v, p)) sub_raw(r.v, r.v, p);
+  csub(r.v, p, t4);
   return r;
 }
 
@@ -160,7 +182,7 @@ inline Fe mul_any(const Mod& M, const Fe& a, const Fe& b) {
   }
   Fe r;
   memcpy(r.v, t, 32);
-  if (t[4] != 0 || geq(r.v, M.p)) sub_raw(r.v, r.v, M.p);
+  csub(r.v, M.p, t[4]);
   return r;
 }
 

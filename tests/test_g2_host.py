@@ -75,8 +75,8 @@ def test_g2_msm_edge_cases():
 @pytest.mark.parametrize("c", [3, 5, 8])
 def test_g2_msm_batches_and_collisions(c):
     # more points than one batch of affine additions, few buckets: many
-    # additions wait for a later batch; repeated and negated bases put
-    # P + P and P + (-P) into one bucket
+    # additions meet a bucket that is busy in the current batch; repeated
+    # and negated bases put P + P and P + (-P) into one bucket
     from pyzkp_amd import g2 as G
     rng = random.Random(c)
     n = 2500
@@ -119,3 +119,52 @@ def test_g2_chord_extreme_coordinates():
         assert _aff(G.g2_add(P, Q)) == expect, (p, q)
         assert _aff(G.g2_msm([P, Q], [1, 1], window_bits=4, threads=1)) == expect, (p, q)
         checked += 1
+
+
+def _simd_modes():
+    from pyzkp_amd._native import load_library
+    lib = load_library()
+    modes = [0]
+    prev = lib.zk_host_simd(-1)
+    if lib.zk_host_simd(1):
+        modes.append(1)
+    lib.zk_host_simd(prev)
+    return lib, modes
+
+
+@pytest.mark.parametrize("c", [4, 9, 12])
+def test_g2_msm_ifma_and_scalar_paths_agree(c):
+    # The AVX-512 IFMA accumulation (where the CPU has it) and the 64-bit
+    # scalar one, on uniform scalars and on skewed ones (zeros, small values,
+    # repeated and negated bases, a block of equal scalars that piles one
+    # bucket per window), both against the closed form sum k_i s_i G.
+    from pyzkp_amd import g2 as G
+    lib, modes = _simd_modes()
+    rng = random.Random(0x1F3A + c)
+    n = 3000
+    base_ks = [rng.randrange(1, R_MOD) for _ in range(n // 3)]
+    ks = [base_ks[i % len(base_ks)] * (1 if rng.random() < 0.8 else -1) % R_MOD for i in range(n)]
+    G0 = (O.G2_GENERATOR[0], O.G2_GENERATOR[1], (1, 0))
+    bases = G.from_words(G.g2_mul_many(G0, ks))
+    uniform = [rng.randrange(R_MOD) for _ in range(n)]
+    skewed = []
+    for i in range(n):
+        u = rng.random()
+        skewed.append(0 if u < 0.3 else rng.randrange(1 << 16) if u < 0.5 else 77 if i % 7 == 0
+                      else rng.randrange(R_MOD))
+    try:
+        for ss in (uniform, skewed):
+            expect = O.mul(O.G2_GENERATOR, sum(k * s for k, s in zip(ks, ss)) % R_MOD)
+            for mode in modes:
+                lib.zk_host_simd(mode)
+                assert _aff(G.g2_msm(bases, ss, window_bits=c, threads=3)) == expect, (mode, c)
+    finally:
+        lib.zk_host_simd(1)
+
+
+def test_host_simd_switch():
+    lib, modes = _simd_modes()
+    assert lib.zk_host_simd(0) == 0
+    assert lib.zk_host_simd(-1) == 0
+    assert lib.zk_host_simd(1) == (1 if 1 in modes else 0)
+    assert lib.zk_host_simd(7) == lib.zk_host_simd(-1)
