@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["msm", "ntt", "batch", "serve", "copy", "plonkq"], default="msm")
-    ap.add_argument("--requests", type=int, default=64, help="serve: number of proof requests")
+    ap.add_argument("--requests", type=int, default=128, help="serve: number of proof requests")
     ap.add_argument("--serve-log", type=str, default="10,20", help="serve: constraint range 2^lo..2^hi")
     ap.add_argument("--cpu-threads", type=int, default=16, help="serve: host threads for G2 MSMs")
     ap.add_argument("--hbm-budget-gb", type=float, default=16.0, help="serve: device budget for working sets")
@@ -69,10 +69,10 @@ def parse():
     ap.add_argument("--no-ntt", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="MSM line: leave out the served-prover leg")
     ap.add_argument("--device-slots", type=int, default=2, help="serve: concurrent device slots")
-    ap.add_argument("--loads", type=str, default="0.6,0.8,1.0,1.15,1.3,1.5",
+    ap.add_argument("--loads", type=str, default="0.8,1.0,1.15,1.3,1.5,1.75,2.0",
                     help="serve: Poisson arrival rates as fractions of the plain loop's capacity")
-    ap.add_argument("--slo-ms", type=float, default=750.0,
-                    help="serve: p99 latency objective (about twice a 2^20-constraint proof's plain service time)")
+    ap.add_argument("--slo-ms", type=float, default=400.0,
+                    help="serve: p99 latency objective (about 2.5x a 2^20-constraint proof's plain service time)")
     return ap.parse_args()
 
 

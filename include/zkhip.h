@@ -39,6 +39,7 @@ typedef struct zk_ctx zk_ctx;
 typedef struct zk_fixed_bases zk_fixed_bases;
 typedef struct zk_batch zk_batch;
 typedef struct zk_msm_bases zk_msm_bases;
+typedef struct zk_g2_bases zk_g2_bases;
 
 /* One NTT job of a batch: `batch` transforms of size 2^log_n laid out back
  * to back, out of place (in != out), with the optional root / coset shift
@@ -262,6 +263,17 @@ int zk_g2_msm_host(const uint64_t* bases, const uint64_t* scalars, uint64_t n, i
                    uint64_t out[16]);
 int zk_g2_mul_fixed_host(const uint64_t* q, const uint64_t* scalars, uint64_t n, int threads, uint64_t* out);
 int zk_g2_add_host(const uint64_t* p, const uint64_t* q, uint64_t out[16]);
+/* A G2 base set prepared once for many host MSMs (the proving key's
+ * b_query_g2, reused by every proof): zk_g2_bases_create copies the points
+ * (affine canonical, 16 words, zeros = infinity) and converts them for the
+ * IFMA path where the CPU has it; zk_g2_msm_prepared is zk_g2_msm_host over
+ * its first n <= size points.  The host counterpart of zk_msm_bases_create
+ * for the G2 MSM of the reference's prover (protocols/groth16/prove.py:
+ * 76-90 runs an MSM_G2 node over b_query_g2 in every proof). */
+int zk_g2_bases_create(const uint64_t* bases, uint64_t n, int threads, zk_g2_bases** out);
+int zk_g2_bases_destroy(zk_g2_bases* b);
+int zk_g2_msm_prepared(const zk_g2_bases* b, const uint64_t* scalars, uint64_t n, int window_bits, int threads,
+                       uint64_t out[16]);
 /* Selects the host G2 MSM's arithmetic: mode 1 (the default) runs its
  * batched affine bucket additions eight at a time on AVX-512 IFMA where the
  * CPU has AVX-512F and IFMA, 0 runs them one at a time on 64-bit scalar

@@ -73,6 +73,9 @@ SIGNATURES = {
     "zk_g2_mul_fixed_host": [_vp, _vp, ct.c_uint64, ct.c_int, _vp],
     "zk_g2_add_host": [_vp, _vp, _vp],
     "zk_host_simd": [ct.c_int],
+    "zk_g2_bases_create": [_vp, ct.c_uint64, ct.c_int, ct.POINTER(_vp)],
+    "zk_g2_bases_destroy": [_vp],
+    "zk_g2_msm_prepared": [_vp, _vp, ct.c_uint64, ct.c_int, ct.c_int, _vp],
     "zk_g1_mul_host": [_vp, _vp, _vp],
     "zk_pairing_product": [_vp, _vp, ct.c_uint32, _vp],
     "zk_pairing_check": [_vp, _vp, ct.c_uint32, ct.POINTER(ct.c_int)],

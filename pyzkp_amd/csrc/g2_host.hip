@@ -11,12 +11,14 @@
 #include <string.h>
 #include <atomic>
 #include <exception>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <thread>
 #include <vector>
 #include "g2_simd.hpp"
 #include "internal.hpp"
+#include "../../include/zkhip.h"
 
 namespace zk {
 namespace g2 {
@@ -436,12 +438,12 @@ static inline Fe from_l5(const uint64_t* l) {
   return zkh::mul(FQ, x, s);
 }
 
-static J2 bucket_task_simd(const uint64_t* pts20, const uint8_t* pinf, const int32_t* digits, uint64_t n_all,
+static J2 bucket_task_simd(const uint64_t* pts20, const uint8_t* pinf, const uint64_t* scalars, const uint64_t* K,
                            const BucketTask& t, int c) {
   const uint32_t nb = 1u << (c - 1);
   std::vector<uint64_t> bk((size_t)nb * g2simd::PT_WORDS);
   std::vector<uint8_t> empty(nb, 1);
-  if (g2simd::accumulate(pts20, pinf, digits + (uint64_t)t.w * n_all, t.i0, t.i1, nb, bk.data(), empty.data()))
+  if (g2simd::accumulate(pts20, pinf, scalars, K, t.w, (uint32_t)c, t.i0, t.i1, nb, bk.data(), empty.data()))
     throw std::bad_alloc();
   J2 run = j_inf(), sum = j_inf();
   for (int b = (int)nb - 1; b >= 0; b--) {
@@ -457,47 +459,65 @@ static J2 bucket_task_simd(const uint64_t* pts20, const uint8_t* pinf, const int
 
 // 1: the AVX-512 IFMA path where the CPU has it; 0: the scalar path
 static std::atomic<int> g_simd_mode(1);
-static bool simd_on() {
+static bool cpu_ifma() {
   static const bool has = g2simd::cpu_has_ifma();
-  return has && g_simd_mode.load() != 0;
+  return has;
+}
+static bool simd_on() { return cpu_ifma() && g_simd_mode.load() != 0; }
+
+// bases in the IFMA path's form, converted in parallel (each worker first
+// touches the pages it writes)
+static void prepare(const uint64_t* bases, uint64_t n, int threads, uint64_t* pts20, uint8_t* inf) {
+  parallel_for((n + 1023) / 1024, threads, [&](uint64_t blk) {
+    const uint64_t i0 = blk * 1024, e = i0 + 1024 < n ? i0 + 1024 : n;
+    g2simd::prepare_points(bases + 16 * i0, e - i0, pts20 + (uint64_t)g2simd::PT_WORDS * i0, inf + i0);
+  });
 }
 
-static void msm(const uint64_t* bases, const uint64_t* scalars, uint64_t n, int c, int threads,
-                uint64_t out[16]) {
+// sum_i s_i Q_i.  pts20 / pinf: the bases in the IFMA form (then that path
+// runs; its digits come from the scalars on the fly), or null.
+static void msm(const uint64_t* bases, const uint64_t* pts20, const uint8_t* pinf, const uint64_t* scalars,
+                uint64_t n, int c, int threads, uint64_t out[16]) {
   const uint32_t W = (255 + c - 1) / c;
-  const bool simd = simd_on();
+  const bool simd = pts20 != nullptr;
   std::vector<A2> pts(simd ? 0 : n);
-  std::vector<uint64_t> pts20(simd ? (size_t)n * g2simd::PT_WORDS : 0);
-  std::vector<uint8_t> pinf(simd ? n : 0);
-  std::vector<int32_t> digits((size_t)n * W);
-  parallel_for((n + 1023) / 1024, threads, [&](uint64_t blk) {
-    const uint64_t e = (blk + 1) * 1024 < n ? (blk + 1) * 1024 : n;
-    if (simd)
-      g2simd::prepare_points(bases + 16 * blk * 1024, e - blk * 1024, pts20.data() + g2simd::PT_WORDS * blk * 1024,
-                             pinf.data() + blk * 1024);
-    for (uint64_t i = blk * 1024; i < e; i++) {
-      if (!simd) pts[i] = rd_aff(bases + 16 * i);
-      Fe s;
-      memcpy(s.v, scalars + 4 * i, 32);
-      while (zkh::geq(s.v, zkh::FR.p)) zkh::sub_raw(s.v, s.v, zkh::FR.p);
-      // signed digits: v = window + carry; v >= 2^(c-1) -> v - 2^c, carry 1
-      const int64_t half = 1ll << (c - 1), full = 1ll << c;
-      int64_t carry = 0;
-      for (uint32_t w = 0; w < W; w++) {
-        const uint32_t off = w * (uint32_t)c, wi = off >> 6, sh = off & 63;
-        unsigned __int128 v = wi < 4 ? ((unsigned __int128)s.v[wi] >> sh) : 0;
-        if (wi + 1 < 4 && sh) v |= (unsigned __int128)s.v[wi + 1] << (64 - sh);
-        int64_t dv = (int64_t)(uint64_t)(v & (uint64_t)(full - 1)) + carry;
-        if (dv >= half) {
-          dv -= full;
-          carry = 1;
-        } else {
-          carry = 0;
+  std::vector<int32_t> digits(simd ? 0 : (size_t)n * W);
+  // K = sum_w 2^(w c + c - 1): window w of (s mod r) + K, minus 2^(c-1), is
+  // the signed digit that the carry recoding below gives (the carry out of
+  // the lower windows is the carry of the addition); W c >= 255 keeps
+  // r + K below 2^(W c)
+  uint64_t K[5] = {0, 0, 0, 0, 0};
+  for (uint32_t w = 0; w < W; w++) {
+    const uint32_t bit = w * (uint32_t)c + (uint32_t)c - 1;
+    K[bit >> 6] |= 1ull << (bit & 63);
+  }
+  if (!simd) {
+    parallel_for((n + 1023) / 1024, threads, [&](uint64_t blk) {
+      const uint64_t e = (blk + 1) * 1024 < n ? (blk + 1) * 1024 : n;
+      for (uint64_t i = blk * 1024; i < e; i++) {
+        pts[i] = rd_aff(bases + 16 * i);
+        Fe s;
+        memcpy(s.v, scalars + 4 * i, 32);
+        while (zkh::geq(s.v, zkh::FR.p)) zkh::sub_raw(s.v, s.v, zkh::FR.p);
+        // signed digits: v = window + carry; v >= 2^(c-1) -> v - 2^c, carry 1
+        const int64_t half = 1ll << (c - 1), full = 1ll << c;
+        int64_t carry = 0;
+        for (uint32_t w = 0; w < W; w++) {
+          const uint32_t off = w * (uint32_t)c, wi = off >> 6, sh = off & 63;
+          unsigned __int128 v = wi < 4 ? ((unsigned __int128)s.v[wi] >> sh) : 0;
+          if (wi + 1 < 4 && sh) v |= (unsigned __int128)s.v[wi + 1] << (64 - sh);
+          int64_t dv = (int64_t)(uint64_t)(v & (uint64_t)(full - 1)) + carry;
+          if (dv >= half) {
+            dv -= full;
+            carry = 1;
+          } else {
+            carry = 0;
+          }
+          digits[(uint64_t)w * n + i] = (int32_t)dv;
         }
-        digits[(uint64_t)w * n + i] = (int32_t)dv;
       }
-    }
-  });
+    });
+  }
   // tasks: every window split into S point ranges so that all threads have work
   uint64_t S = ((uint64_t)threads * 2 + W - 1) / W;
   const uint64_t min_pts = 1ull << 12;
@@ -508,7 +528,7 @@ static void msm(const uint64_t* bases, const uint64_t* scalars, uint64_t n, int 
     for (uint64_t k = 0; k < S; k++) tasks.push_back({w, n * k / S, n * (k + 1) / S});
   std::vector<J2> part(tasks.size());
   parallel_for(tasks.size(), threads, [&](uint64_t t) {
-    part[t] = simd ? bucket_task_simd(pts20.data(), pinf.data(), digits.data(), n, tasks[t], c)
+    part[t] = simd ? bucket_task_simd(pts20, pinf, scalars, K, tasks[t], c)
                    : bucket_task(pts.data(), digits.data(), n, tasks[t], c);
   });
   J2 acc = j_inf();
@@ -519,12 +539,52 @@ static void msm(const uint64_t* bases, const uint64_t* scalars, uint64_t n, int 
   wr_jac_affine(out, acc);
 }
 
+// window for n points on `threads` threads
+static int default_window(uint64_t n, int threads, bool simd) {
+  // the points one task sees: a window is split over about 2 threads / W
+  // tasks (W ~ 18..22)
+  const uint64_t split = ((uint64_t)threads * 2 + 17) / 18;
+  const uint64_t per = n / (split ? split : 1);
+  int bl = 0;
+  while ((per >> bl) != 0) bl++;
+  int c;
+  if (simd) {
+    // batched IFMA additions are ~4x cheaper than the scalar path's, so the
+    // Jacobian bucket sum weighs more: 2^19 points per task -> c = 12
+    // (c = 11..15 measured at 2^20 points on 16 cores: 12 best, 13 +5%,
+    // 14 +12%)
+    c = bl - 8;
+    if (c > 13) c = 13;
+  } else {
+    // about 2^5 points per bucket; at most 2^13 buckets (1.1 MB of affine
+    // points) per task, so a task's buckets stay in a core's L2 (c = 15, 16
+    // measured 3-10% slower at 2^19..2^20 points)
+    c = bl - 5;
+    if (c > 14) c = 14;
+  }
+  return c < 4 ? 4 : c;
+}
+
+// A G2 base set in both forms: the canonical words (scalar path) and, where
+// the CPU has IFMA, that path's form, converted once.
+struct Bases {
+  uint64_t n = 0;
+  std::unique_ptr<uint64_t[]> words;
+  std::unique_ptr<uint64_t[]> pts20;
+  std::unique_ptr<uint8_t[]> inf;
+};
+
 }  // namespace g2
 }  // namespace zk
 
 using namespace zk;
 
 extern "C" {
+
+static int norm_threads(int threads) {
+  if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+  return threads < 1 ? 1 : threads;
+}
 
 /* G2 MSM on the host: bases affine canonical (16 words each, zeros =
  * infinity), scalars canonical (4 words each), window c (0: default rule),
@@ -537,27 +597,71 @@ int zk_g2_msm_host(const uint64_t* bases, const uint64_t* scalars, uint64_t n, i
     memset(out, 0, 128);
     return ZK_OK;
   }
-  if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
-  if (threads < 1) threads = 1;
-  int c = window_bits;
-  if (c <= 0) {
-    // sized for the points one task sees (a window is split over about
-    // 2 threads / W tasks, W ~ 18): about 2^5 points per bucket keeps the
-    // bucket reduction small against the additions and few collisions in a
-    // batch of affine additions; at most 2^13 buckets (1.1 MB of affine
-    // points) per task, so a task's buckets stay in a core's L2 (c = 15, 16
-    // measured 3-10% slower at 2^19..2^20 points)
-    const uint64_t split = ((uint64_t)threads * 2 + 17) / 18;
-    const uint64_t per = n / (split ? split : 1);
-    int bl = 0;
-    while ((per >> bl) != 0) bl++;
-    c = bl - 5;
-    if (c > 14) c = 14;
-    if (c < 4) c = 4;
-  }
+  threads = norm_threads(threads);
+  const bool simd = g2::simd_on();
+  const int c = window_bits > 0 ? window_bits : g2::default_window(n, threads, simd);
   if (c < 2 || c > 20) return ZK_ERR_INVALID;
   try {
-    g2::msm(bases, scalars, n, c, threads, out);
+    std::unique_ptr<uint64_t[]> pts20;
+    std::unique_ptr<uint8_t[]> inf;
+    if (simd) {  // uninitialised: the conversion's workers write every word
+      pts20.reset(new uint64_t[(size_t)n * g2simd::PT_WORDS]);
+      inf.reset(new uint8_t[n]);
+      g2::prepare(bases, n, threads, pts20.get(), inf.get());
+    }
+    g2::msm(bases, pts20.get(), inf.get(), scalars, n, c, threads, out);
+  } catch (const std::bad_alloc&) {
+    return ZK_ERR_RUNTIME;
+  }
+  return ZK_OK;
+}
+
+/* A G2 base set prepared once for many MSMs (the proving key's
+ * b_query_g2): the points are copied, and converted to the IFMA path's
+ * form where the CPU has it.  Bases as for zk_g2_msm_host. */
+int zk_g2_bases_create(const uint64_t* bases, uint64_t n, int threads, zk_g2_bases** out) {
+  if (!out || (n && !bases)) return ZK_ERR_INVALID;
+  *out = nullptr;
+  threads = norm_threads(threads);
+  try {
+    std::unique_ptr<g2::Bases> h(new g2::Bases);
+    h->n = n;
+    h->words.reset(new uint64_t[(size_t)(n ? n : 1) * 16]);
+    if (n) memcpy(h->words.get(), bases, (size_t)n * 128);
+    if (g2::cpu_ifma() && n) {
+      h->pts20.reset(new uint64_t[(size_t)n * g2simd::PT_WORDS]);
+      h->inf.reset(new uint8_t[n]);
+      g2::prepare(bases, n, threads, h->pts20.get(), h->inf.get());
+    }
+    *out = reinterpret_cast<zk_g2_bases*>(h.release());
+  } catch (const std::bad_alloc&) {
+    return ZK_ERR_RUNTIME;
+  }
+  return ZK_OK;
+}
+
+int zk_g2_bases_destroy(zk_g2_bases* h) {
+  delete reinterpret_cast<g2::Bases*>(h);
+  return ZK_OK;
+}
+
+/* sum_i s_i Q_i over the first n points of a prepared set (n <= its size),
+ * otherwise as zk_g2_msm_host. */
+int zk_g2_msm_prepared(const zk_g2_bases* h, const uint64_t* scalars, uint64_t n, int window_bits, int threads,
+                       uint64_t out[16]) {
+  const g2::Bases* B = reinterpret_cast<const g2::Bases*>(h);
+  if (!B || !out || (n && !scalars) || n > B->n) return ZK_ERR_INVALID;
+  if (n == 0) {
+    memset(out, 0, 128);
+    return ZK_OK;
+  }
+  threads = norm_threads(threads);
+  const bool simd = g2::simd_on() && B->pts20;
+  const int c = window_bits > 0 ? window_bits : g2::default_window(n, threads, simd);
+  if (c < 2 || c > 20) return ZK_ERR_INVALID;
+  try {
+    g2::msm(B->words.get(), simd ? B->pts20.get() : nullptr, simd ? B->inf.get() : nullptr, scalars, n, c, threads,
+            out);
   } catch (const std::bad_alloc&) {
     return ZK_ERR_RUNTIME;
   }
@@ -568,8 +672,7 @@ int zk_g2_msm_host(const uint64_t* bases, const uint64_t* scalars, uint64_t n, i
  * canonical scalars, affine canonical outputs (16 words each). */
 int zk_g2_mul_fixed_host(const uint64_t* q, const uint64_t* scalars, uint64_t n, int threads, uint64_t* out) {
   if (!q || (n && (!scalars || !out))) return ZK_ERR_INVALID;
-  if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
-  if (threads < 1) threads = 1;
+  threads = norm_threads(threads);
   g2::A2 base = g2::rd_aff(q);
   if (n < 64) {
     // few scalars: 4-bit fixed windows over 16 multiples, no table set-up

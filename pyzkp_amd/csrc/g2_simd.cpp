@@ -499,10 +499,48 @@ struct Slots {
   uint64_t* at(uint32_t s) const { return s < nb ? bk + (uint64_t)PT_WORDS * s : rp + (uint64_t)PT_WORDS * (s - nb); }
 };
 
+// Fr modulus: scalars at or above it are reduced first
+static const uint64_t RW64[4] = {0x43e1f593f0000001ull, 0x2833e84879b97091ull, 0xb85045b68181585dull,
+                                 0x30644e72e131a029ull};
+
+// signed digit of window w (bit offset off = w c): bits [off, off + c) of
+// (s mod r) + K, minus 2^(c-1)
+static inline int32_t digit(const uint64_t* sp, const uint64_t* K, uint32_t off, uint32_t c) {
+  uint64_t s[4];
+  memcpy(s, sp, 32);
+  for (;;) {  // s >= r happens only for non-canonical input
+    bool ge = true;
+    for (int j = 3; j >= 0; j--)
+      if (s[j] != RW64[j]) {
+        ge = s[j] > RW64[j];
+        break;
+      }
+    if (!ge) break;
+    uint64_t br = 0;
+    for (int j = 0; j < 4; j++) {
+      const unsigned __int128 v = (unsigned __int128)s[j] - RW64[j] - br;
+      s[j] = (uint64_t)v;
+      br = (uint64_t)(v >> 127);
+    }
+  }
+  uint64_t t[5], cy = 0;
+  for (int j = 0; j < 4; j++) {
+    const unsigned __int128 v = (unsigned __int128)s[j] + K[j] + cy;
+    t[j] = (uint64_t)v;
+    cy = (uint64_t)(v >> 64);
+  }
+  t[4] = K[4] + cy;
+  const uint32_t wi = off >> 6, sh = off & 63;
+  uint64_t v = t[wi] >> sh;
+  if (sh && wi < 4) v |= t[wi + 1] << (64 - sh);
+  return (int32_t)(v & ((1ull << c) - 1)) - (int32_t)(1u << (c - 1));
+}
+
 }  // namespace
 
-uint64_t accumulate(const uint64_t* pts20, const uint8_t* inf, const int32_t* digits, uint64_t i0, uint64_t i1,
-                    uint32_t nb, uint64_t* bk20, uint8_t* empty) {
+uint64_t accumulate(const uint64_t* pts20, const uint8_t* inf, const uint64_t* scalars, const uint64_t* K,
+                    uint32_t win, uint32_t cbits, uint64_t i0, uint64_t i1, uint32_t nb, uint64_t* bk20,
+                    uint8_t* empty) {
   VC c;
   make_vc(c);
   const uint32_t ns = nb + REPLICAS;
@@ -662,9 +700,11 @@ uint64_t accumulate(const uint64_t* pts20, const uint8_t* inf, const int32_t* di
     place(P, (uint32_t)s, neg);
   };
 
+  const uint32_t off = win * cbits;
   for (uint64_t i = i0; i < i1; i++) {
-    const int32_t d = digits[i];
-    if (d == 0 || inf[i]) continue;
+    if (inf[i]) continue;
+    const int32_t d = digit(scalars + 4 * i, K, off, cbits);
+    if (d == 0) continue;
     push(pts20 + (uint64_t)PT_WORDS * i, (uint32_t)(d < 0 ? -d : d) - 1, d < 0);
   }
   do_flush();

@@ -21,14 +21,16 @@ bool cpu_has_ifma();
 // infinity), inf[i] <- 1 for the point at infinity (its 20 words are zero).
 void prepare_points(const uint64_t* bases16, uint64_t n, uint64_t* out20, uint8_t* inf);
 
-// Bucket sums of one window over points [i0, i1): digit d = digits[i] adds
+// Bucket sums of window w (c bits) over points [i0, i1): the signed digit d
+// of scalar i (4 words; bits [w c, w c + c) of (s mod r) + K, minus
+// 2^(c-1), K = sum over the windows of 2^(w c + c - 1), 5 words) adds
 // sign(d) P_i to bucket |d| - 1 of nb.  bk20 (nb x 20 words) and empty (nb)
 // are the buckets, in and out.  Additions are affine, batched so that one
 // inversion serves a whole batch; any digit distribution is handled (a point
 // whose bucket is busy goes to a replica of it, and replicas are added back
 // at the end).  Returns 0, or UINT64_MAX when out of memory.
-uint64_t accumulate(const uint64_t* pts20, const uint8_t* inf, const int32_t* digits, uint64_t i0, uint64_t i1,
-                    uint32_t nb, uint64_t* bk20, uint8_t* empty);
+uint64_t accumulate(const uint64_t* pts20, const uint8_t* inf, const uint64_t* scalars, const uint64_t* K,
+                    uint32_t w, uint32_t c, uint64_t i0, uint64_t i1, uint32_t nb, uint64_t* bk20, uint8_t* empty);
 
 }  // namespace g2simd
 }  // namespace zk

@@ -7,6 +7,7 @@ normalised points have Z = (1, 0); the identity is ((1,0), (1,0), (0,0)).
 from __future__ import annotations
 
 import ctypes as ct
+import threading
 from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -79,42 +80,77 @@ def _scal(scalars) -> np.ndarray:
 
 
 class G2Bases:
-    """Packed affine host copy of a G2 point sequence (reused across MSMs)."""
+    """Packed affine host copy of a G2 point sequence (reused across MSMs).
+
+    MSMs over it run on a prepared base set (zk_g2_bases_create: the points
+    converted once for the IFMA path), made at the first MSM and kept until
+    free() or garbage collection."""
 
     def __init__(self, points) -> None:
         self.words = _pack(points)
         self.n = len(points)
+        self._prep = None
+        self._lock = threading.Lock()
+
+    def prepared(self, threads: int = 0):
+        """The native prepared set (a handle), created on first use."""
+        if self._prep is None:
+            with self._lock:
+                if self._prep is None:
+                    h = ct.c_void_p()
+                    w = np.ascontiguousarray(self.words, dtype="<u8")
+                    rc = load_library().zk_g2_bases_create(w.ctypes.data, self.n, threads, ct.byref(h))
+                    if rc:
+                        raise RuntimeError(f"zk_g2_bases_create failed ({rc})")
+                    self._prep = h
+        return self._prep
+
+    def free(self) -> None:
+        with self._lock:
+            if self._prep is not None and self._prep.value:
+                load_library().zk_g2_bases_destroy(self._prep)
+            self._prep = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
-def g2_msm(points, scalars: Sequence[int], window_bits: int = 0, threads: int = 0) -> G2Point:
-    """sum_i s_i Q_i (crypto/msm/pippenger.py:67-121 semantics)."""
-    bases = points if isinstance(points, G2Bases) else G2Bases(points)
-    n = len(scalars)
-    if n > bases.n:
-        raise ValueError("points and scalars length mismatch")
+def _msm(bases: "G2Bases", s: np.ndarray, n: int, window_bits: int, threads: int, prepared: bool) -> G2Point:
     out = np.zeros(16, dtype="<u8")
-    s = _scal(scalars)
-    rc = load_library().zk_g2_msm_host(bases.words.ctypes.data, s.ctypes.data, n, window_bits, threads,
-                                       out.ctypes.data)
+    lib = load_library()
+    if prepared:
+        rc = lib.zk_g2_msm_prepared(bases.prepared(threads), s.ctypes.data, n, window_bits, threads, out.ctypes.data)
+    else:
+        rc = lib.zk_g2_msm_host(bases.words.ctypes.data, s.ctypes.data, n, window_bits, threads, out.ctypes.data)
     if rc:
         raise ValueError("g2 msm failed")
     return _unpack(out)
 
 
+def g2_msm(points, scalars: Sequence[int], window_bits: int = 0, threads: int = 0) -> G2Point:
+    """sum_i s_i Q_i (crypto/msm/pippenger.py:67-121 semantics).  A G2Bases
+    runs on its prepared set; a point list is converted for this call."""
+    keep = isinstance(points, G2Bases)
+    bases = points if keep else G2Bases(points)
+    n = len(scalars)
+    if n > bases.n:
+        raise ValueError("points and scalars length mismatch")
+    return _msm(bases, _scal(scalars), n, window_bits, threads, prepared=keep)
+
+
 def g2_msm_words(bases: "G2Bases", scalar_words: np.ndarray, n: int, window_bits: int = 0,
-                 threads: int = 0) -> G2Point:
-    """Same, with canonical scalars as (n, 4) uint64 words."""
+                 threads: int = 0, prepared: bool = True) -> G2Point:
+    """Same, with canonical scalars as (n, 4) uint64 words; on the prepared
+    set unless prepared=False."""
     if n > bases.n:
         raise ValueError("points and scalars length mismatch")
     s = np.ascontiguousarray(scalar_words, dtype="<u8")
     if s.ndim != 2 or s.shape[1] != 4 or s.shape[0] < n:
         raise ValueError(f"scalar words must have shape (>= {n}, 4), got {tuple(s.shape)}")
-    out = np.zeros(16, dtype="<u8")
-    rc = load_library().zk_g2_msm_host(bases.words.ctypes.data, s.ctypes.data, n, window_bits, threads,
-                                       out.ctypes.data)
-    if rc:
-        raise ValueError("g2 msm failed")
-    return _unpack(out)
+    return _msm(bases, s, n, window_bits, threads, prepared)
 
 
 def g2_mul_many(q, scalars, threads: int = 0) -> np.ndarray:
@@ -155,6 +191,7 @@ def from_words(words: np.ndarray) -> G2Bases:
     b = G2Bases([])
     b.words = np.ascontiguousarray(words, dtype="<u8")
     b.n = words.shape[0]
+    b._prep = None
     return b
 
 

@@ -79,6 +79,8 @@ class ProvingKey:
                 b.free()
         self.query_points = self.aux_dev = None
         self.csr = None
+        if self.b_query_g2 is not None and hasattr(self.b_query_g2, "free"):
+            self.b_query_g2.free()        # its prepared host set; the words stay
 
     def g1_query(self, which: str) -> List[Any]:
         """Download one query vector as host points (for small keys / tests)."""
