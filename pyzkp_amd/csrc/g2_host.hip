@@ -9,7 +9,10 @@
 // canonical (non-Montgomery) little-endian words: affine (x0, x1, y0, y1)
 // with all-zero for infinity, 16 x u64 per point.
 #include <string.h>
+#include <algorithm>
 #include <atomic>
+#include <condition_variable>
+#include <functional>
 #include <exception>
 #include <memory>
 #include <mutex>
@@ -273,34 +276,111 @@ static void wr_jac_affine(uint64_t* out, const J2& p) {
   wr_fq(out + 12, y.c1);
 }
 
-// fn(0 .. n-1) on up to `threads` threads; an exception thrown by fn stops
-// the remaining items and is rethrown here
-template <class F>
-static void parallel_for(uint64_t n, int threads, F fn) {
-  if (threads <= 1 || n <= 1) {
-    for (uint64_t i = 0; i < n; i++) fn(i);
-    return;
+// Host worker threads shared by every parallel loop of this file, started
+// once.  A loop posts a job; idle workers join the oldest job that still has
+// items (up to the job's thread cap) and the calling thread works on its own
+// job too, so a job always progresses even when every worker is busy
+// elsewhere.  Concurrent MSMs (the prove server runs two) thus share the
+// cores instead of starting 16 threads each, and a small MSM pays no thread
+// start-up.
+class HostPool {
+ public:
+  static HostPool& get() {
+    static HostPool* p = new HostPool();  // never destroyed: workers may be parked at exit
+    return *p;
   }
-  std::vector<std::thread> ts;
-  std::atomic<uint64_t> next(0);
-  std::exception_ptr err;
-  std::mutex err_mu;
-  for (int t = 0; t < threads; t++)
-    ts.emplace_back([&]() {
-      for (;;) {
-        uint64_t i = next.fetch_add(1);
-        if (i >= n) break;
+  // fn(0 .. n-1) on at most `threads` threads (the caller is one of them);
+  // an exception thrown by fn skips the items not yet started and is
+  // rethrown here
+  void run(uint64_t n, int threads, const std::function<void(uint64_t)>& fn) {
+    if (threads <= 1 || n <= 1 || workers_.empty()) {
+      for (uint64_t i = 0; i < n; i++) fn(i);
+      return;
+    }
+    Job j;
+    j.fn = &fn;
+    j.n = n;
+    j.max_helpers = std::min<int>(threads - 1, (int)workers_.size());
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      jobs_.push_back(&j);
+    }
+    cv_.notify_all();
+    work(&j);
+    {
+      std::unique_lock<std::mutex> lk(mu_);
+      done_cv_.wait(lk, [&] { return j.done.load() == j.n && j.helpers == 0; });
+      jobs_.erase(std::find(jobs_.begin(), jobs_.end(), &j));
+    }
+    if (j.err) std::rethrow_exception(j.err);
+  }
+
+ private:
+  struct Job {
+    const std::function<void(uint64_t)>* fn = nullptr;
+    uint64_t n = 0;
+    std::atomic<uint64_t> next{0}, done{0};
+    std::atomic<bool> stop{false};
+    int max_helpers = 0, helpers = 0;  // helpers: under mu_
+    std::exception_ptr err;
+    std::mutex err_mu;
+  };
+
+  HostPool() {
+    int hw = (int)std::thread::hardware_concurrency();
+    if (hw < 1) hw = 1;
+    for (int t = 0; t < hw; t++) workers_.emplace_back([this] { loop(); });
+  }
+  static void work(Job* j) {
+    for (;;) {
+      const uint64_t i = j->next.fetch_add(1);
+      if (i >= j->n) break;
+      if (!j->stop.load()) {
         try {
-          fn(i);
+          (*j->fn)(i);
         } catch (...) {
-          std::lock_guard<std::mutex> g(err_mu);
-          if (!err) err = std::current_exception();
-          next.store(n);
+          std::lock_guard<std::mutex> g(j->err_mu);
+          if (!j->err) j->err = std::current_exception();
+          j->stop.store(true);
         }
       }
-    });
-  for (auto& t : ts) t.join();
-  if (err) std::rethrow_exception(err);
+      j->done.fetch_add(1);
+    }
+  }
+  void loop() {
+    for (;;) {
+      Job* j = nullptr;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] {
+          for (Job* c : jobs_)
+            if (c->next.load() < c->n && c->helpers < c->max_helpers) {
+              j = c;
+              return true;
+            }
+          return false;
+        });
+        j->helpers++;
+      }
+      work(j);
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        j->helpers--;
+      }
+      done_cv_.notify_all();
+    }
+  }
+
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  std::vector<Job*> jobs_;
+  std::vector<std::thread> workers_;
+};
+
+template <class F>
+static void parallel_for(uint64_t n, int threads, F fn) {
+  const std::function<void(uint64_t)> f(std::ref(fn));
+  HostPool::get().run(n, threads, f);
 }
 
 static uint32_t bits_at(const uint64_t* s, uint32_t off, uint32_t c) {
