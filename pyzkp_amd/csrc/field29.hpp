@@ -384,9 +384,10 @@ struct Mont29 {
     for (int i = 0; i < 9; i++) s.l[i] = a.l[i] + b.l[i];
     return s;
   }
-  // a - b + 2p, limb by limb, for b normalised with value below 1.2p (then
-  // b's limbs are at most K2P's, so no limb goes negative); limbs grow by
-  // less than 2^30
+  // a - b + 2p, limb by limb, for b normalised with value below 1.99p (then
+  // b's limbs are at most K2P's: its lower limbs are below 2^29, K2P's above,
+  // and its top limb is at most K2P's, 2 p_8 - 1, so no limb goes negative);
+  // limbs grow by less than 2^30
   static ZK_HD f9 sub_cf(const f9& a, const f9& b) {
     f9 s;
 #pragma unroll
