@@ -168,3 +168,47 @@ def test_host_simd_switch():
     assert lib.zk_host_simd(-1) == 0
     assert lib.zk_host_simd(1) == (1 if 1 in modes else 0)
     assert lib.zk_host_simd(7) == lib.zk_host_simd(-1)
+
+
+def test_g2_prepared_sets():
+    # A prepared set (zk_g2_bases_create) gives the same points as the
+    # per-call conversion, on either arithmetic path, for prefixes of the
+    # set; n beyond the set and null handles are refused; an empty set
+    # gives the identity.
+    import ctypes as ct
+    import numpy as np
+    from pyzkp_amd import g2 as G
+    lib, modes = _simd_modes()
+    rng = random.Random(0x77A)
+    n = 700
+    ks = [rng.randrange(1, R_MOD) for _ in range(n)]
+    ss = [rng.randrange(R_MOD) for _ in range(n)]
+    G0 = (O.G2_GENERATOR[0], O.G2_GENERATOR[1], (1, 0))
+    words = G.g2_mul_many(G0, ks)
+    words[5] = 0                                    # a point at infinity inside the set
+    bases = G.from_words(words)
+    sw = np.frombuffer(b"".join(s.to_bytes(32, "little") for s in ss), dtype="<u8").reshape(n, 4).copy()
+    try:
+        for mode in modes:
+            lib.zk_host_simd(mode)
+            for m in (1, 333, n):
+                a = G.g2_msm_words(bases, sw, m, window_bits=7, threads=3)
+                b = G.g2_msm_words(bases, sw, m, window_bits=7, threads=3, prepared=False)
+                assert _aff(a) == _aff(b), (mode, m)
+            expect = O.mul(O.G2_GENERATOR, sum(k * s for i, (k, s) in enumerate(zip(ks, ss)) if i != 5) % R_MOD)
+            assert _aff(G.g2_msm_words(bases, sw, n, threads=4)) == expect
+    finally:
+        lib.zk_host_simd(1)
+    out = np.zeros(16, dtype="<u8")
+    h = bases.prepared()
+    assert lib.zk_g2_msm_prepared(h, sw.ctypes.data, n + 1, 0, 2, out.ctypes.data) == 1
+    assert lib.zk_g2_msm_prepared(None, sw.ctypes.data, 1, 0, 2, out.ctypes.data) == 1
+    assert lib.zk_g2_msm_prepared(h, sw.ctypes.data, 0, 0, 2, out.ctypes.data) == 0 and not out.any()
+    e = ct.c_void_p()
+    assert lib.zk_g2_bases_create(None, 0, 1, ct.byref(e)) == 0 and e.value
+    assert lib.zk_g2_msm_prepared(e, sw.ctypes.data, 0, 0, 1, out.ctypes.data) == 0 and not out.any()
+    assert lib.zk_g2_bases_destroy(e) == 0
+    assert lib.zk_g2_bases_destroy(None) == 0
+    bases.free()
+    bases.free()                                    # idempotent; the next MSM prepares again
+    assert _aff(G.g2_msm_words(bases, sw, n, threads=2)) == expect
