@@ -17,6 +17,7 @@
 #include <memory>
 #include <mutex>
 #include <new>
+#include <utility>
 #include <thread>
 #include <vector>
 #include "g2_simd.hpp"
@@ -506,18 +507,6 @@ static J2 bucket_task(const A2* pts, const int32_t* digits, uint64_t n_all, cons
 // The same bucket sums on the AVX-512 IFMA path: the batched affine
 // additions run eight at a time (g2_simd.cpp), and the weighted bucket sum
 // runs as above.
-// An Fq value of the IFMA path (five 52-bit limbs, Montgomery form with
-// R = 2^260, canonical) in this file's form (R = 2^256): x 2^260 times 2^252
-// over 2^256 by one Montgomery product.
-static inline Fe from_l5(const uint64_t* l) {
-  Fe x, s = {{0, 0, 0, 1ull << 60}};  // 2^252 < q
-  x.v[0] = l[0] | (l[1] << 52);
-  x.v[1] = (l[1] >> 12) | (l[2] << 40);
-  x.v[2] = (l[2] >> 24) | (l[3] << 28);
-  x.v[3] = (l[3] >> 36) | (l[4] << 16);
-  return zkh::mul(FQ, x, s);
-}
-
 static J2 bucket_task_simd(const uint64_t* pts20, const uint8_t* pinf, const uint64_t* scalars, const uint64_t* K,
                            const BucketTask& t, int c) {
   const uint32_t nb = 1u << (c - 1);
@@ -525,16 +514,34 @@ static J2 bucket_task_simd(const uint64_t* pts20, const uint8_t* pinf, const uin
   std::vector<uint8_t> empty(nb, 1);
   if (g2simd::accumulate(pts20, pinf, scalars, K, t.w, (uint32_t)c, t.i0, t.i1, nb, bk.data(), empty.data()))
     throw std::bad_alloc();
-  J2 run = j_inf(), sum = j_inf();
-  for (int b = (int)nb - 1; b >= 0; b--) {
-    if (!empty[b]) {
-      const uint64_t* B = bk.data() + (size_t)g2simd::PT_WORDS * b;
-      const F2 x = {from_l5(B), from_l5(B + 5)}, y = {from_l5(B + 10), from_l5(B + 15)};
-      run = j_madd(run, x, y);
-    }
-    sum = j_add(sum, run);
+  // sum_b (b + 1) B_b: eight running-sum segments on the vector lanes, then
+  // sum_l sum_l + L sum_l l run_l here
+  uint64_t sw[8 * 24], rw[8 * 24];
+  uint8_t si[8], ri[8];
+  const uint32_t L = g2simd::reduce_segments(bk.data(), empty.data(), nb, sw, rw, si, ri);
+  auto jac = [](const uint64_t* w) {
+    J2 p;
+    memcpy(p.X.c0.v, w, 32);
+    memcpy(p.X.c1.v, w + 4, 32);
+    memcpy(p.Y.c0.v, w + 8, 32);
+    memcpy(p.Y.c1.v, w + 12, 32);
+    memcpy(p.Z.c0.v, w + 16, 32);
+    memcpy(p.Z.c1.v, w + 20, 32);
+    return p;
+  };
+  J2 sum = j_inf(), run = j_inf(), u = j_inf();
+  for (int l = 7; l >= 0; l--) {
+    if (!si[l]) sum = j_add(sum, jac(sw + 24 * l));
+    if (l == 0) break;
+    if (!ri[l]) run = j_add(run, jac(rw + 24 * l));
+    u = j_add(u, run);  // u = sum_(l >= 1) l run_l
   }
-  return sum;
+  J2 m = j_inf();
+  for (int bit = 31; bit >= 0; bit--) {
+    m = j_dbl(m);
+    if ((L >> bit) & 1) m = j_add(m, u);
+  }
+  return j_add(sum, m);
 }
 
 // 1: the AVX-512 IFMA path where the CPU has it; 0: the scalar path
@@ -598,23 +605,62 @@ static void msm(const uint64_t* bases, const uint64_t* pts20, const uint8_t* pin
       }
     });
   }
-  // tasks: every window split into S point ranges so that all threads have work
-  uint64_t S = ((uint64_t)threads * 2 + W - 1) / W;
-  const uint64_t min_pts = 1ull << 12;
-  if (S > (n + min_pts - 1) / min_pts) S = (n + min_pts - 1) / min_pts;
-  if (S < 1) S = 1;
+  // tasks: pieces (window, point range), each with its own buckets
   std::vector<BucketTask> tasks;
-  for (uint32_t w = 0; w < W; w++)
-    for (uint64_t k = 0; k < S; k++) tasks.push_back({w, n * k / S, n * (k + 1) / S});
+  std::vector<std::vector<uint32_t>> chunks;  // the pieces a worker takes together (IFMA path)
+  if (simd) {
+    // The W n (window, point) pairs are cut into one equal chunk per
+    // thread; a chunk may hold the end of one window and the start of the
+    // next.  Cuts fall on a grid of n / 32 points, so no piece is a sliver
+    // that pays a whole bucket sum for a few points.  (Equal pieces per
+    // window balance only when their count fills whole rounds of threads.)
+    const uint64_t g = n >= 32 ? n / 32 : 1;
+    auto cut = [&](uint64_t k) {  // chunk boundary k as (window, point)
+      const unsigned __int128 u = (unsigned __int128)W * n * k / (uint64_t)threads;
+      uint64_t w = (uint64_t)(u / n), i = (uint64_t)(u % n);
+      i = (i + g / 2) / g * g;
+      if (i >= n) {
+        i = 0;
+        w++;
+      }
+      return std::make_pair(w, i);
+    };
+    for (int k = 0; k < threads; k++) {
+      auto a = cut((uint64_t)k), b = cut((uint64_t)k + 1);
+      std::vector<uint32_t> ids;
+      for (uint64_t w = a.first; w <= b.first && w < W; w++) {
+        const uint64_t i0 = (w == a.first) ? a.second : 0;
+        const uint64_t i1 = (w == b.first) ? b.second : n;
+        if (i1 > i0) {
+          ids.push_back((uint32_t)tasks.size());
+          tasks.push_back({(uint32_t)w, i0, i1});
+        }
+      }
+      if (!ids.empty()) chunks.push_back(ids);
+    }
+  } else {
+    const uint64_t min_pts = 1ull << 12;
+    uint64_t S = ((uint64_t)threads * 2 + W - 1) / W;
+    if (S > (n + min_pts - 1) / min_pts) S = (n + min_pts - 1) / min_pts;
+    if (S < 1) S = 1;
+    for (uint32_t w = 0; w < W; w++)
+      for (uint64_t k = 0; k < S; k++) tasks.push_back({w, n * k / S, n * (k + 1) / S});
+  }
   std::vector<J2> part(tasks.size());
-  parallel_for(tasks.size(), threads, [&](uint64_t t) {
-    part[t] = simd ? bucket_task_simd(pts20, pinf, scalars, K, tasks[t], c)
-                   : bucket_task(pts.data(), digits.data(), n, tasks[t], c);
-  });
+  if (simd) {
+    parallel_for(chunks.size(), threads, [&](uint64_t k) {
+      for (uint32_t t : chunks[k]) part[t] = bucket_task_simd(pts20, pinf, scalars, K, tasks[t], c);
+    });
+  } else {
+    parallel_for(tasks.size(), threads,
+                 [&](uint64_t t) { part[t] = bucket_task(pts.data(), digits.data(), n, tasks[t], c); });
+  }
+  std::vector<J2> win(W, j_inf());
+  for (size_t t = 0; t < tasks.size(); t++) win[tasks[t].w] = j_add(win[tasks[t].w], part[t]);
   J2 acc = j_inf();
   for (int w = (int)W - 1; w >= 0; w--) {
     for (int k = 0; k < c; k++) acc = j_dbl(acc);
-    for (uint64_t k = 0; k < S; k++) acc = j_add(acc, part[(uint64_t)w * S + k]);
+    acc = j_add(acc, win[w]);
   }
   wr_jac_affine(out, acc);
 }
@@ -629,12 +675,17 @@ static int default_window(uint64_t n, int threads, bool simd) {
   while ((per >> bl) != 0) bl++;
   int c;
   if (simd) {
-    // batched IFMA additions are ~4x cheaper than the scalar path's, so the
-    // Jacobian bucket sum weighs more: 2^19 points per task -> c = 12
-    // (c = 11..15 measured at 2^20 points on 16 cores: 12 best, 13 +5%,
-    // 14 +12%)
-    c = bl - 8;
-    if (c > 13) c = 13;
+    // additions and bucket sums both run on the vector lanes and the work
+    // is cut into equal chunks per thread, so the window follows the whole
+    // MSM: c = log2(n) - 5 for large n (about 2^6 points per bucket), up to
+    // log2(n) - 3 (at most 13) for small n, where fewer windows matter more
+    // than bucket sums (measured on 16 cores: best c 8-9 at 2^12, 11 at
+    // 2^14, 12-13 at 2^16, 13 at 2^18, 15 at 2^20)
+    int ln = 0;
+    while ((n >> ln) > 1) ln++;
+    c = ln - 5;
+    if (c < ln - 3 && c < 13) c = ln - 3 < 13 ? ln - 3 : 13;
+    if (c > 16) c = 16;
   } else {
     // about 2^5 points per bucket; at most 2^13 buckets (1.1 MB of affine
     // points) per task, so a task's buckets stay in a core's L2 (c = 15, 16

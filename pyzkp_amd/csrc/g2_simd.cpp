@@ -56,6 +56,7 @@ struct Consts {
   uint64_t kq[9][5];   // k q, limbs normalised (k <= 8: below 2^257)
   uint64_t r2[5];      // 2^520 mod q: mont(x, r2) = x R
   uint64_t one[5];     // R mod q = 2^260 mod q
+  uint64_t to256[5];   // 2^256 mod q: mont(x R, to256) = x 2^256 (the 4 x 64-bit form)
   uint64_t e[4];       // q - 2
 };
 
@@ -97,6 +98,7 @@ static Consts make_consts() {
   uint64_t t[4] = {1, 0, 0, 0};
   for (int i = 1; i <= 520; i++) {
     dbl_mod(t);  // t = 2^i mod q
+    if (i == 256) to52(t, c.to256);
     if (i == 260) to52(t, c.one);
   }
   to52(t, c.r2);
@@ -122,6 +124,7 @@ struct VC {  // broadcast constants
   V q[5];
   V kq[9][5];
   F one;
+  F to256;
 };
 static void make_vc(VC& v) {
   const Consts& c = consts();
@@ -131,6 +134,7 @@ static void make_vc(VC& v) {
   for (int j = 0; j < 5; j++) {
     v.q[j] = _mm512_set1_epi64((long long)c.q[j]);
     v.one.l[j] = _mm512_set1_epi64((long long)c.one[j]);
+    v.to256.l[j] = _mm512_set1_epi64((long long)c.to256[j]);
     for (int k = 0; k <= 8; k++) v.kq[k][j] = _mm512_set1_epi64((long long)c.kq[k][j]);
   }
 }
@@ -712,6 +716,319 @@ uint64_t accumulate(const uint64_t* pts20, const uint8_t* inf, const uint64_t* s
   memcpy(empty, S.empty, nb);
   release();
   return 0;
+}
+
+namespace {
+
+// ---- the weighted bucket sum: Fq2 and XYZZ points on eight lanes ----
+// Fq2 components stay below 2q in and out of every operation.
+struct V2 {
+  F c0, c1;
+};
+struct P4 {  // XYZZ: x = X / ZZ, y = Y / ZZZ; infinity is tracked in lane masks
+  V2 X, Y, ZZ, ZZZ;
+};
+
+static inline void add2(V2& r, const V2& a, const V2& b, const VC& c) {
+  F r0, r1;
+  add(r0, a.c0, b.c0, c);
+  csubk<2>(r0, c);
+  add(r1, a.c1, b.c1, c);
+  csubk<2>(r1, c);
+  r.c0 = r0;
+  r.c1 = r1;
+}
+static inline void sub2(V2& r, const V2& a, const V2& b, const VC& c) {
+  F r0, r1;
+  subk<2>(r0, a.c0, b.c0, c);
+  csubk<2>(r0, c);
+  subk<2>(r1, a.c1, b.c1, c);
+  csubk<2>(r1, c);
+  r.c0 = r0;
+  r.c1 = r1;
+}
+// Karatsuba: (a0 + a1)(b0 + b1) < 16q^2, so every product ends below 2q
+static inline void mul2(V2& r, const V2& a, const V2& b, const VC& c) {
+  F m0, m1, m2, sa, sb, t, r0, r1;
+  mul(m0, a.c0, b.c0, c);
+  mul(m1, a.c1, b.c1, c);
+  add(sa, a.c0, a.c1, c);
+  add(sb, b.c0, b.c1, c);
+  mul(m2, sa, sb, c);
+  subk<2>(r0, m0, m1, c);
+  csubk<2>(r0, c);
+  add(t, m0, m1, c);
+  subk<4>(r1, m2, t, c);
+  csubk<4>(r1, c);
+  csubk<2>(r1, c);
+  r.c0 = r0;
+  r.c1 = r1;
+}
+// (a0 + a1)(a0 - a1 + 2q) and 2 a0 a1
+static inline void sqr2(V2& r, const V2& a, const VC& c) {
+  F s, d, t, r0, r1;
+  add(s, a.c0, a.c1, c);
+  subk<2>(d, a.c0, a.c1, c);
+  mul(r0, s, d, c);
+  add(t, a.c0, a.c0, c);
+  mul(r1, t, a.c1, c);
+  r.c0 = r0;
+  r.c1 = r1;
+}
+static inline __mmask8 zero_lt2q(const F& a, const VC& c) {
+  F t = a;
+  csubk<1>(t, c);
+  const V o = _mm512_or_si512(_mm512_or_si512(_mm512_or_si512(t.l[0], t.l[1]), _mm512_or_si512(t.l[2], t.l[3])),
+                              t.l[4]);
+  return _mm512_cmpeq_epi64_mask(o, c.zero);
+}
+static inline __mmask8 zero2(const V2& a, const VC& c) { return zero_lt2q(a.c0, c) & zero_lt2q(a.c1, c); }
+static inline void blend2(V2& a, const V2& b, __mmask8 m) {
+  blend(a.c0, b.c0, m);
+  blend(a.c1, b.c1, m);
+}
+static inline void blendP(P4& a, const P4& b, __mmask8 m) {
+  blend2(a.X, b.X, m);
+  blend2(a.Y, b.Y, m);
+  blend2(a.ZZ, b.ZZ, m);
+  blend2(a.ZZZ, b.ZZZ, m);
+}
+
+// r = 2 (x, y) for an affine point (mdbl-2008-s-1, a = 0); lanes with y = 0
+// are returned in yz (their double is infinity)
+static void aff_dbl(P4& r, const V2& x, const V2& y, __mmask8& yz, const VC& c) {
+  V2 U, V, W, S, x2, M, t, X3, Y3;
+  yz = zero2(y, c);
+  add2(U, y, y, c);
+  sqr2(V, U, c);
+  mul2(W, U, V, c);
+  mul2(S, x, V, c);
+  sqr2(x2, x, c);
+  add2(M, x2, x2, c);
+  add2(M, M, x2, c);
+  sqr2(X3, M, c);
+  add2(t, S, S, c);
+  sub2(X3, X3, t, c);
+  sub2(t, S, X3, c);
+  mul2(Y3, M, t, c);
+  mul2(t, W, y, c);
+  sub2(Y3, Y3, t, c);
+  r.X = X3;
+  r.Y = Y3;
+  r.ZZ = V;
+  r.ZZZ = W;
+}
+// r = 2p (dbl-2008-s-1, a = 0); lanes with y = 0 in yz
+static void xyzz_dbl(P4& r, const P4& p, __mmask8& yz, const VC& c) {
+  V2 U, V, W, S, x2, M, t, X3, Y3, ZZ3, ZZZ3;
+  yz = zero2(p.Y, c);
+  add2(U, p.Y, p.Y, c);
+  sqr2(V, U, c);
+  mul2(W, U, V, c);
+  mul2(S, p.X, V, c);
+  sqr2(x2, p.X, c);
+  add2(M, x2, x2, c);
+  add2(M, M, x2, c);
+  sqr2(X3, M, c);
+  add2(t, S, S, c);
+  sub2(X3, X3, t, c);
+  sub2(t, S, X3, c);
+  mul2(Y3, M, t, c);
+  mul2(t, W, p.Y, c);
+  sub2(Y3, Y3, t, c);
+  mul2(ZZ3, V, p.ZZ, c);
+  mul2(ZZZ3, W, p.ZZZ, c);
+  r.X = X3;
+  r.Y = Y3;
+  r.ZZ = ZZ3;
+  r.ZZZ = ZZZ3;
+}
+
+// p += (x, y) on the lanes of m (affine points, not infinity)
+// (madd-2008-s); pinf: p's infinity lanes, updated
+static void xyzz_madd(P4& p, __mmask8& pinf, const V2& x, const V2& y, __mmask8 m, const VC& c) {
+  const __mmask8 was_inf = pinf;
+  const __mmask8 fin = m & (__mmask8)~was_inf;
+  if (fin) {
+    V2 U2, S2, Pd, R, PP, PPP, Q, t, X3, Y3;
+    P4 r;
+    mul2(U2, x, p.ZZ, c);
+    mul2(S2, y, p.ZZZ, c);
+    sub2(Pd, U2, p.X, c);
+    sub2(R, S2, p.Y, c);
+    sqr2(PP, Pd, c);
+    mul2(PPP, Pd, PP, c);
+    mul2(Q, p.X, PP, c);
+    sqr2(X3, R, c);
+    sub2(X3, X3, PPP, c);
+    add2(t, Q, Q, c);
+    sub2(X3, X3, t, c);
+    sub2(t, Q, X3, c);
+    mul2(Y3, R, t, c);
+    mul2(t, p.Y, PPP, c);
+    sub2(Y3, Y3, t, c);
+    mul2(r.ZZ, p.ZZ, PP, c);
+    mul2(r.ZZZ, p.ZZZ, PPP, c);
+    r.X = X3;
+    r.Y = Y3;
+    __mmask8 ok = fin;
+    const __mmask8 pz = fin & zero2(Pd, c);
+    if (pz) {  // same x: p = (x, y) or p = -(x, y)
+      const __mmask8 dm = pz & zero2(R, c);
+      __mmask8 cancel = pz & (__mmask8)~dm;
+      if (dm) {
+        P4 d;
+        __mmask8 yz;
+        aff_dbl(d, x, y, yz, c);
+        blendP(r, d, dm);
+        cancel |= dm & yz;
+      }
+      pinf |= cancel;
+      ok &= (__mmask8)~cancel;
+    }
+    blendP(p, r, ok);
+  }
+  const __mmask8 fi = m & was_inf;
+  if (fi) {
+    P4 a;
+    a.X = x;
+    a.Y = y;
+    a.ZZ.c0 = c.one;
+    a.ZZ.c1 = F{{c.zero, c.zero, c.zero, c.zero, c.zero}};
+    a.ZZZ = a.ZZ;
+    blendP(p, a, fi);
+    pinf &= (__mmask8)~fi;
+  }
+}
+
+// p += q (add-2008-s); pinf, qinf: infinity lanes
+static void xyzz_add(P4& p, __mmask8& pinf, const P4& q, __mmask8 qinf, const VC& c) {
+  const __mmask8 both = (__mmask8)(~pinf & ~qinf);
+  const __mmask8 take_q = pinf & (__mmask8)~qinf;
+  if (both) {
+    V2 U1, U2, S1, S2, Pd, R, PP, PPP, Q, t, X3, Y3;
+    P4 r;
+    mul2(U1, p.X, q.ZZ, c);
+    mul2(U2, q.X, p.ZZ, c);
+    mul2(S1, p.Y, q.ZZZ, c);
+    mul2(S2, q.Y, p.ZZZ, c);
+    sub2(Pd, U2, U1, c);
+    sub2(R, S2, S1, c);
+    sqr2(PP, Pd, c);
+    mul2(PPP, Pd, PP, c);
+    mul2(Q, U1, PP, c);
+    sqr2(X3, R, c);
+    sub2(X3, X3, PPP, c);
+    add2(t, Q, Q, c);
+    sub2(X3, X3, t, c);
+    sub2(t, Q, X3, c);
+    mul2(Y3, R, t, c);
+    mul2(t, S1, PPP, c);
+    sub2(Y3, Y3, t, c);
+    mul2(t, p.ZZ, q.ZZ, c);
+    mul2(r.ZZ, t, PP, c);
+    mul2(t, p.ZZZ, q.ZZZ, c);
+    mul2(r.ZZZ, t, PPP, c);
+    r.X = X3;
+    r.Y = Y3;
+    __mmask8 ok = both;
+    const __mmask8 pz = both & zero2(Pd, c);
+    if (pz) {  // same x: p = q or p = -q
+      const __mmask8 dm = pz & zero2(R, c);
+      __mmask8 cancel = pz & (__mmask8)~dm;
+      if (dm) {
+        P4 d;
+        __mmask8 yz;
+        xyzz_dbl(d, q, yz, c);
+        blendP(r, d, dm);
+        cancel |= dm & yz;
+      }
+      pinf |= cancel;
+      ok &= (__mmask8)~cancel;
+    }
+    blendP(p, r, ok);
+  }
+  if (take_q) {
+    blendP(p, q, take_q);
+    pinf &= (__mmask8)~take_q;
+  }
+}
+
+// lane values of one Fq component as four canonical words in R = 2^256 form
+static inline void export_f(const F& a, uint64_t* base, int stride, const VC& c) {
+  F t;
+  mul(t, a, c.to256, c);
+  csubk<1>(t, c);
+  alignas(64) uint64_t l5[5][8];
+  for (int j = 0; j < 5; j++) _mm512_store_si512(l5[j], t.l[j]);
+  for (int lane = 0; lane < 8; lane++) {
+    uint64_t* w = base + (size_t)stride * lane;
+    w[0] = l5[0][lane] | (l5[1][lane] << 52);
+    w[1] = (l5[1][lane] >> 12) | (l5[2][lane] << 40);
+    w[2] = (l5[2][lane] >> 24) | (l5[3][lane] << 28);
+    w[3] = (l5[3][lane] >> 36) | (l5[4][lane] << 16);
+  }
+}
+// Jacobian (X ZZ^2, Y ZZZ^2, ZZZ) of each lane: 24 words per lane
+static void export_jac(const P4& p, uint64_t* out, const VC& c) {
+  V2 t, Xj, Yj;
+  sqr2(t, p.ZZ, c);
+  mul2(Xj, p.X, t, c);
+  sqr2(t, p.ZZZ, c);
+  mul2(Yj, p.Y, t, c);
+  export_f(Xj.c0, out + 0, 24, c);
+  export_f(Xj.c1, out + 4, 24, c);
+  export_f(Yj.c0, out + 8, 24, c);
+  export_f(Yj.c1, out + 12, 24, c);
+  export_f(p.ZZZ.c0, out + 16, 24, c);
+  export_f(p.ZZZ.c1, out + 20, 24, c);
+}
+
+}  // namespace
+
+uint32_t reduce_segments(const uint64_t* bk20, const uint8_t* empty, uint32_t nb, uint64_t* out_sum,
+                         uint64_t* out_run, uint8_t* sum_inf, uint8_t* run_inf) {
+  VC c;
+  make_vc(c);
+  const uint32_t L = (nb + 7) / 8;
+  P4 run, sum;
+  const F z5 = {{c.zero, c.zero, c.zero, c.zero, c.zero}};
+  run.X.c0 = run.X.c1 = z5;
+  run.Y = run.ZZ = run.ZZZ = run.X;
+  sum = run;
+  __mmask8 rinf = 0xff, sinf = 0xff;
+  alignas(64) uint64_t tb[PT_WORDS][8];
+  for (int64_t j = (int64_t)L - 1; j >= 0; j--) {
+    __mmask8 present = 0;
+    for (int l = 0; l < 8; l++) {
+      const uint64_t b = (uint64_t)l * L + (uint64_t)j;
+      if (b < nb && !empty[b]) {
+        present |= (__mmask8)(1u << l);
+        const uint64_t* B = bk20 + (uint64_t)PT_WORDS * b;
+        for (int t = 0; t < PT_WORDS; t++) tb[t][l] = B[t];
+      } else {
+        for (int t = 0; t < PT_WORDS; t++) tb[t][l] = 0;
+      }
+    }
+    if (present) {
+      V2 x, y;
+      for (int k = 0; k < 5; k++) {
+        x.c0.l[k] = _mm512_load_si512(tb[k]);
+        x.c1.l[k] = _mm512_load_si512(tb[5 + k]);
+        y.c0.l[k] = _mm512_load_si512(tb[10 + k]);
+        y.c1.l[k] = _mm512_load_si512(tb[15 + k]);
+      }
+      xyzz_madd(run, rinf, x, y, present, c);
+    }
+    xyzz_add(sum, sinf, run, rinf, c);
+  }
+  export_jac(sum, out_sum, c);
+  export_jac(run, out_run, c);
+  for (int l = 0; l < 8; l++) {
+    sum_inf[l] = (sinf >> l) & 1;
+    run_inf[l] = (rinf >> l) & 1;
+  }
+  return L;
 }
 
 }  // namespace g2simd

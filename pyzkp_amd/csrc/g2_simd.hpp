@@ -32,5 +32,15 @@ void prepare_points(const uint64_t* bases16, uint64_t n, uint64_t* out20, uint8_
 uint64_t accumulate(const uint64_t* pts20, const uint8_t* inf, const uint64_t* scalars, const uint64_t* K,
                     uint32_t w, uint32_t c, uint64_t i0, uint64_t i1, uint32_t nb, uint64_t* bk20, uint8_t* empty);
 
+// The weighted sum sum_b (b + 1) B_b of nb affine buckets (bk20 / empty as
+// accumulate leaves them), eight segments at a time: lane l runs XYZZ running
+// sums over buckets [l L, (l + 1) L), L = ceil(nb / 8), and returns
+//   sum_l = sum_j (j + 1) B_(l L + j)  and  run_l = sum_j B_(l L + j)
+// as Jacobian points (X ZZ^2, Y ZZZ^2, ZZZ): 24 words per lane (X0 X1 Y0 Y1
+// Z0 Z1, canonical, Montgomery with R = 2^256), with their infinity flags.
+// The full sum is sum_l sum_l + L sum_l l run_l.  Returns L.
+uint32_t reduce_segments(const uint64_t* bk20, const uint8_t* empty, uint32_t nb, uint64_t* out_sum,
+                         uint64_t* out_run, uint8_t* sum_inf, uint8_t* run_inf);
+
 }  // namespace g2simd
 }  // namespace zk
