@@ -1,4 +1,4 @@
-// Host G2 MSM bucket accumulation on AVX-512 IFMA.
+// Host G2 MSM bucket accumulation and bucket sums on AVX-512 IFMA.
 //
 // Eight Fq values per vector, each as five 52-bit limbs in Montgomery form
 // with R = 2^260.  A product is operand scanning over b's limbs: per limb,
@@ -17,9 +17,13 @@
 // keeps its own product chain of the denominators' norms (1 / d = conj(d) /
 // N(d), N(d) = d0^2 + d1^2 in Fq, nonzero for d != 0 because -1 is not a
 // square mod q), and one vector exponentiation by q - 2 inverts all eight
-// chains.  A point whose bucket already has an addition pending waits for
-// the next batch; one that keeps waiting (many points in one bucket) is
-// handed back to the caller, which adds it with Jacobian arithmetic.
+// chains.  A point whose bucket already has an addition queued goes to a
+// replica slot of that bucket, and the replicas are folded back pairwise
+// through the same batched additions, so any digit distribution stays on
+// the batched path.
+//
+// The weighted bucket sum sum_b (b + 1) B_b then runs on the lanes too:
+// each lane keeps XYZZ running sums over one eighth of the buckets.
 //
 // This file is the only one built with AVX-512 code generation and it is
 // entered only after the caller checked the CPU.  It shares no inline
