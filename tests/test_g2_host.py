@@ -212,3 +212,28 @@ def test_g2_prepared_sets():
     bases.free()
     bases.free()                                    # idempotent; the next MSM prepares again
     assert _aff(G.g2_msm_words(bases, sw, n, threads=2)) == expect
+
+
+@pytest.mark.parametrize("c", [3, 15])
+def test_g2_msm_2_15_both_paths(c):
+    # 2^15 points: with c = 3 (4 buckets a window) every window's additions
+    # pile into few buckets, so the IFMA path's replica pool fills and is
+    # folded back many times; c = 15 has more buckets than points per window
+    # and most buckets hold one point.  Both paths against the closed form.
+    import numpy as np
+    from pyzkp_amd import g2 as G
+    lib, modes = _simd_modes()
+    rng = random.Random(0x215 + c)
+    n = 1 << 15
+    ks = [rng.randrange(1, R_MOD) for _ in range(n)]
+    ss = [rng.randrange(R_MOD) for _ in range(n)]
+    G0 = (O.G2_GENERATOR[0], O.G2_GENERATOR[1], (1, 0))
+    bases = G.from_words(G.g2_mul_many(G0, ks))
+    sw = np.frombuffer(b"".join(s.to_bytes(32, "little") for s in ss), dtype="<u8").reshape(n, 4).copy()
+    expect = O.mul(O.G2_GENERATOR, sum(k * s for k, s in zip(ks, ss)) % R_MOD)
+    try:
+        for mode in modes:
+            lib.zk_host_simd(mode)
+            assert _aff(G.g2_msm_words(bases, sw, n, window_bits=c, threads=8)) == expect, mode
+    finally:
+        lib.zk_host_simd(1)
