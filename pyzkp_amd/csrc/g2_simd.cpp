@@ -208,6 +208,11 @@ static inline void add(F& r, const F& a, const F& b, const VC& c) {
   snorm(r, _mm512_add_epi64(a.l[0], b.l[0]), _mm512_add_epi64(a.l[1], b.l[1]), _mm512_add_epi64(a.l[2], b.l[2]),
         _mm512_add_epi64(a.l[3], b.l[3]), _mm512_add_epi64(a.l[4], b.l[4]), c);
 }
+// a + b limb by limb, carries left in the limbs (below 2^53): only for a
+// value that feeds a difference (subk normalises), never a product
+static inline void add_raw(F& r, const F& a, const F& b) {
+  for (int j = 0; j < 5; j++) r.l[j] = _mm512_add_epi64(a.l[j], b.l[j]);
+}
 // r = a + k q - b, for b < k q
 template <int K>
 static inline void subk(F& r, const F& a, const F& b, const VC& c) {
@@ -415,7 +420,7 @@ static void flush(Work& w, int cnt, const VC& c) {
       blend(num.c0, s0, dm);
       blend(num.c1, s1, dm);
     }
-    // lam = num / den (Karatsuba): lam0 < 4q, lam1 < 6q
+    // lam = num / den (Karatsuba): lam0 < 4q, and lam1 < 6q brought below 4q
     F2 lam;
     {
       F m0, m1, m2, sa, sb, m01;
@@ -425,24 +430,28 @@ static void flush(Work& w, int cnt, const VC& c) {
       add(sb, ie.c0, ie.c1, c);
       mul(m2, sa, sb, c);
       subk<2>(lam.c0, m0, m1, c);
-      add(m01, m0, m1, c);
+      add_raw(m01, m0, m1);
       subk<4>(lam.c1, m2, m01, c);
+      csubk<2>(lam.c1, c);
     }
-    // x3 = lam^2 - x_B - x_P
+    // x3 = lam^2 - x_B - x_P: (lam0 + lam1)(lam0 - lam1 + 4q) and 2 lam0 lam1
+    // (products below 64q^2 and 32q^2), so c0, c1 < 2q and x3 < 4q
     F2 x3;
     {
-      F s0, s1, s2, c0, c1, bp;
-      mul(s0, lam.c0, lam.c0, c);
-      mul(s1, lam.c1, lam.c1, c);
-      mul(s2, lam.c0, lam.c1, c);
-      subk<2>(c0, s0, s1, c);
-      add(c1, s2, s2, c);
-      add(bp, bx.c0, px.c0, c);
+      F s, d, t, c0, c1, bp;
+      add(s, lam.c0, lam.c1, c);
+      subk<4>(d, lam.c0, lam.c1, c);
+      mul(c0, s, d, c);
+      add(t, lam.c0, lam.c0, c);
+      mul(c1, t, lam.c1, c);
+      add_raw(bp, bx.c0, px.c0);
       subk<2>(x3.c0, c0, bp, c);
-      add(bp, bx.c1, px.c1, c);
+      add_raw(bp, bx.c1, px.c1);
       subk<2>(x3.c1, c1, bp, c);
-      canon8(x3.c0, c);
-      canon8(x3.c1, c);
+      csubk<2>(x3.c0, c);
+      csubk<1>(x3.c0, c);
+      csubk<2>(x3.c1, c);
+      csubk<1>(x3.c1, c);
     }
     // y3 = lam (x_B - x3) - y_B
     F2 y3;
@@ -456,7 +465,7 @@ static void flush(Work& w, int cnt, const VC& c) {
       add(sb, d0, d1, c);
       mul(m2, sa, sb, c);
       subk<2>(t0, m0, m1, c);
-      add(m01, m0, m1, c);
+      add_raw(m01, m0, m1);
       subk<4>(t1, m2, m01, c);
       subk<1>(y3.c0, t0, by.c0, c);
       subk<1>(y3.c1, t1, by.c1, c);
