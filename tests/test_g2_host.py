@@ -237,3 +237,34 @@ def test_g2_msm_2_15_both_paths(c):
             assert _aff(G.g2_msm_words(bases, sw, n, window_bits=c, threads=8)) == expect, mode
     finally:
         lib.zk_host_simd(1)
+
+
+def test_g2_bucket_sum_exceptional_cases():
+    # With c = 6 a window has 32 buckets, eight per segment of the IFMA
+    # path's lane-parallel bucket sum, so small digits place points in
+    # neighbouring buckets of one segment and drive each exceptional case of
+    # its running sums: run + B with B = run (doubling) and B = -run
+    # (cancellation), sum + run with sum = run (doubling) and sum = -run
+    # (cancellation).  Both paths against the closed form.
+    from pyzkp_amd import g2 as G
+    lib, modes = _simd_modes()
+    G0 = (O.G2_GENERATOR[0], O.G2_GENERATOR[1], (1, 0))
+    k = 0x1234567
+    P = G.g2_mul(G0, k)
+    neg = lambda p: (p[0], ((-p[1][0]) % O.Q, (-p[1][1]) % O.Q), (1, 0))
+    cases = [
+        ([P, P], [1, 2], [k, k]),                          # bucket 1 = P, bucket 0 = P: run doubles
+        ([P, neg(P)], [2, 1], [k, -k]),                    # bucket 0 = -P cancels run = P
+        ([P], [2], [k]),                                   # empty bucket 0: sum = run doubles
+        ([P, G.g2_mul(G0, (-2 * k) % R_MOD)], [3, 2], [k, -2 * k]),  # run = -P meets sum = P
+        ([P, P, neg(P), G.g2_mul(G0, 3 * k)], [5, 4, 3, 1], [k, k, -k, 3 * k]),
+    ]
+    try:
+        for pts, ss, ks in cases:
+            expect = O.mul(O.G2_GENERATOR, sum(a * b for a, b in zip(ks, ss)) % R_MOD)
+            for mode in modes:
+                lib.zk_host_simd(mode)
+                for threads in (1, 4):
+                    assert _aff(G.g2_msm(pts, ss, window_bits=6, threads=threads)) == expect, (ss, mode)
+    finally:
+        lib.zk_host_simd(1)
