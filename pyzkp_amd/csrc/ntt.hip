@@ -170,13 +170,16 @@ __global__ __launch_bounds__(NttTile<TB>::THREADS) void ntt_pass_kernel(
       x1 = lds[((q + R4) << log_t) + c];
       x2 = lds[((q + 2 * R4) << log_t) + c];
       x3 = lds[((q + 3 * R4) << log_t) + c];
-      if (s0 + t != 0) w1 = tw_cd(c, d, t);
-      w2 = tw_cd(c, d, t + 1);
+      if (s0 + t != 0) {
+        w1 = tw_cd(c, d, t);
+        w2 = tw_cd(c, d, t + 1);
+      }
       w3 = tw_cd(c, d + (1u << t), t + 1);
     }
     __syncthreads();
     if (act) {
       f9 A, B, C, D;
+      f9 r0, r1, r2, r3;
       if (s0 + t != 0) {
         const f9 u0 = (t != 0) ? Fr9::normalize(x0) : x0;
         const f9 u1 = (t != 0) ? Fr9::normalize(x1) : x1;
@@ -184,6 +187,9 @@ __global__ __launch_bounds__(NttTile<TB>::THREADS) void ntt_pass_kernel(
         Fr9::mul_x2(x2, w1, x3, w1, v2, v3);  // two chains interleaved
         bfly(u0, v2, A, B);
         bfly(u1, v3, C, D);
+        const f9 vc = Fr9::mul(C, w2), vd = Fr9::mul(D, w3);
+        bfly(A, vc, r0, r1);
+        bfly(B, vd, r2, r3);
       } else {
         // global stage 0: v below 2p, not 1.2p: the carrying forms
         const f9 v2 = Fr9::reduce2p(x2), v3 = Fr9::reduce2p(x3);
@@ -191,12 +197,15 @@ __global__ __launch_bounds__(NttTile<TB>::THREADS) void ntt_pass_kernel(
         B = Fr9::sub_nr(x0, v2);
         C = Fr9::add_nr(x1, v3);
         D = Fr9::sub_nr(x1, v3);
+        // global stage 1: the pair (A, C) has twiddle w^0 = 1 (the inverse's
+        // table gives -1 there with the outputs exchanged: the same pair), so
+        // C (< 6p) is only reduced, not multiplied
+        const f9 vc = Fr9::reduce_est(C), vd = Fr9::mul(D, w3);
+        r0 = Fr9::add_cf(A, vc);
+        r1 = Fr9::sub_cf(A, vc);
+        bfly(B, vd, r2, r3);
       }
-      const f9 vc = Fr9::mul(C, w2), vd = Fr9::mul(D, w3);
       const uint32_t o2 = 4 * (q - d) + d;
-      f9 r0, r1, r2, r3;
-      bfly(A, vc, r0, r1);
-      bfly(B, vd, r2, r3);
       lds[(o2 << log_t) + c] = r0;
       lds[((o2 + (2u << t)) << log_t) + c] = r1;
       lds[((o2 + (1u << t)) << log_t) + c] = r2;
