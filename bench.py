@@ -675,16 +675,22 @@ def main():
     # autotune c in 13..18 (one timed call each after one warm call)
     cands = [args.window_bits] if args.window_bits else list(range(13, 19))
     tune = {}
+    # bucket reduction: digit split ("split"), or chunked running sums with
+    # 0 (the library's rule), 8 or 16 buckets per thread
+    reducers = (("split", 1, 0), ("chunk0", 0, 0), ("chunk8", 0, 8), ("chunk16", 0, 16))
     for c in cands:
-        for k_red in (0, 8, 16):   # buckets per reduction thread (0: the library's rule)
+        for name, split, k_red in reducers:
+            ctx.set_tuning("msm_reduce_split", split)
             ctx.set_tuning("msm_reduce_chunk", k_red)
             msm(n, c)
             t0 = time.perf_counter()
             msm(n, c)
             msm(n, c)
-            tune[(c, k_red)] = (time.perf_counter() - t0) / 2
-            log(f"  c={c} reduce chunk {k_red}: {tune[(c, k_red)] * 1e3:.2f} ms")
-    c, k_red = min(tune, key=tune.get)
+            tune[(c, name)] = (time.perf_counter() - t0) / 2
+            log(f"  c={c} reduce {name}: {tune[(c, name)] * 1e3:.2f} ms")
+    c, red = min(tune, key=tune.get)
+    _, split, k_red = next(r for r in reducers if r[0] == red)
+    ctx.set_tuning("msm_reduce_split", split)
     ctx.set_tuning("msm_reduce_chunk", k_red)
 
     for _ in range(args.warmup):
@@ -786,8 +792,8 @@ def main():
         "vs_baseline": None, "dtype": "u32x8 Montgomery Fq/Fr (254-bit integer)", "data": "synthetic",
         "config": {"workload": f"BN254 G1 MSM n=2^{log_n} {'skewed' if args.skewed else 'uniform'} scalars, "
                                "bases [k_i]G1 affine (device-resident, prepared once as a resident set)",
-                   "n": n, "window_bits": c, "reduce_chunk": k_red,
-                   "autotune_ms": {f"c{k[0]}_k{k[1]}": round(v * 1e3, 3) for k, v in tune.items()},
+                   "n": n, "window_bits": c, "reduce": red,
+                   "autotune_ms": {f"c{k[0]}_{k[1]}": round(v * 1e3, 3) for k, v in tune.items()},
                    "phases_ms": phases, "verified_closed_form": bool(verified)},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -303,7 +303,10 @@ int zk_pairing_check(const uint64_t* g1, const uint64_t* g2, uint32_t n, int* ok
  * "ntt_table_cap_mb" (bytes of cached NTT tables a context keeps before
  * freeing the least recently used unpinned ones; 0 = no cap, default 16384),
  * "msm_segment" (pairs per accumulation thread), "msm_reduce_chunk"
- * (buckets per reduction thread).  No counterpart in the reference. */
+ * (buckets per reduction thread of the chunked reduction),
+ * "msm_reduce_split" (1, the default: single-launch MSMs reduce their
+ * buckets by digit split; 0: chunked running sums).  No counterpart in the
+ * reference. */
 int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value);
 
 /* Per-phase device time from HIP events recorded on the launch stream

@@ -94,6 +94,25 @@ void g1_window_combine(const xyzz* s, uint32_t W, int c, uint64_t out[12]) {
   memcpy(out + 8, acc.Z.v, 32);
 }
 
+// sum_i 2^pos[i] t[i] by Horner's rule, the terms given in strictly
+// decreasing bit position (XYZZ, R = 2^256 Montgomery words); out as
+// g1_window_combine's
+void g1_bits_combine(const xyzz* t, const uint32_t* pos, uint32_t count, uint64_t out[12]) {
+  memset(out, 0, 12 * sizeof(uint64_t));
+  if (count == 0) return;
+  g1h::J1 acc = g1h::from_xyzz(t[0]);
+  uint32_t at = pos[0];
+  for (uint32_t i = 1; i < count; i++) {
+    for (; at > pos[i]; at--) acc = g1h::jdbl(acc);
+    acc = g1h::jadd(acc, g1h::from_xyzz(t[i]));
+  }
+  for (; at > 0; at--) acc = g1h::jdbl(acc);
+  if (g1h::inf(acc)) return;
+  memcpy(out, acc.X.v, 32);
+  memcpy(out + 4, acc.Y.v, 32);
+  memcpy(out + 8, acc.Z.v, 32);
+}
+
 }  // namespace zk
 
 using namespace zk;

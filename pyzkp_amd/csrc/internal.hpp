@@ -68,6 +68,7 @@ struct Ctx {
   uint32_t msm_segment = 80;      // sorted (bucket, point) pairs per accumulation thread
   uint32_t msm_reduce_chunk = 0;  // buckets per reduction thread (0: sized to fill the device)
   uint32_t msm_groups = 0;        // window groups pipelined over two streams (0: chosen per shape)
+  uint32_t msm_reduce_split = 1;  // single-launch MSMs: digit-split bucket reduction (0: chunked running sums)
   // second stream (default priority) and events for pipelined launches, created on first use
   hipStream_t aux_stream = nullptr;
   std::vector<hipEvent_t> events;
@@ -144,6 +145,7 @@ int fixed_bases_create(Ctx* c, const fe* bases, uint64_t n, int window_bits, Fix
 // sum_w 2^(c w) S_w over an MSM's XYZZ window sums, on the host (g1_host.hip)
 struct xyzz;
 void g1_window_combine(const xyzz* s, uint32_t W, int c, uint64_t out[12]);
+void g1_bits_combine(const xyzz* t, const uint32_t* pos, uint32_t count, uint64_t out[12]);
 void fixed_bases_destroy(FixedBases* fb);
 int msm_fixed_batch_run(Ctx* c, const FixedBases* fb, const fe* scalars, const uint64_t* offsets,
                         const uint64_t* sizes, const uint64_t* base_offsets, uint32_t count, uint64_t* out_jac,

@@ -753,6 +753,108 @@ __global__ __launch_bounds__(RED_GROUP) void msm_reduce_groups_kernel(const pt9*
   if (tid == 0) store_xyzz9(out + (uint64_t)w * G + g, sh[0]);
 }
 
+// 6b. bucket reduction by digit split, for rows that are the windows of one
+//     MSM (the host already doubles between windows, so it can take terms at
+//     any bit position).  Row r holds buckets b = 1..M, M = 2^m.  Writing
+//     b = h 2^ml + l (ml = ceil(m / 2), mh = m - ml, l < 2^ml):
+//       S_r = sum_b b B_b = sum_l l L_l + 2^ml sum_h h H_h,
+//       H_h = sum_l B_(h 2^ml + l) for h < 2^mh,   H_(2^mh) = B_M,
+//       L_l = sum_(h < 2^mh) B_(h 2^ml + l),
+//     and by the bits of l and h
+//       S_r = sum_(j = 0..m) 2^j E_j,
+//       E_j = sum of L_l over the l with bit j set (j < ml), else of H_h over
+//       the h with bit j - ml set.
+//     Every bucket enters one H and one L sum: two additions per bucket, as
+//     in a running sum, but all of them in plain sums (trees), with no long
+//     dependent chains and no per-chunk scalar multiple.  The planes are
+//     folded SPLIT_FOLD at a time, G_g = sum_(i < SPLIT_FOLD) 2^i E_(g SPLIT_FOLD + i),
+//     and the host adds G_g at bit c r + SPLIT_FOLD g of its Horner combination.
+static constexpr uint32_t SPLIT_THREADS = 256;
+static constexpr uint32_t SPLIT_LANES = 16;                          // threads per H or L sum
+static constexpr uint32_t SPLIT_PER = SPLIT_THREADS / SPLIT_LANES;  // sums per workgroup
+static constexpr uint32_t SPLIT_FOLD = 4;                           // planes per folded term
+static constexpr uint32_t SPLIT_PLANE_LANES = 32;                   // threads per plane sum
+
+// blocks [0, nblk_h): H sums, SPLIT_LANES consecutive threads per sum reading
+// consecutive buckets; blocks from nblk_h on: L sums, threads o, o + SPLIT_PER,
+// ... per sum, so SPLIT_PER consecutive threads read consecutive buckets
+__global__ __launch_bounds__(SPLIT_THREADS) void msm_split_sums_kernel(const pt9* __restrict__ buckets,
+                                                                       const uint32_t* __restrict__ row_ptr,
+                                                                       uint32_t B, uint32_t ml, uint32_t mh,
+                                                                       uint32_t nblk_h, pt9* __restrict__ hsum,
+                                                                       pt9* __restrict__ lsum) {
+  __shared__ xyzz9 sh[SPLIT_THREADS];
+  const uint32_t r = blockIdx.y, tid = threadIdx.x;
+  const uint32_t M = 1u << (ml + mh);
+  const bool hrow = blockIdx.x < nblk_h;
+  uint32_t o, p;
+  if (hrow) {
+    o = blockIdx.x * SPLIT_PER + tid / SPLIT_LANES;
+    p = tid % SPLIT_LANES;
+  } else {
+    o = (blockIdx.x - nblk_h) * SPLIT_PER + tid % SPLIT_PER;
+    p = tid / SPLIT_PER;
+  }
+  const uint32_t nout = hrow ? (1u << mh) + 1 : (1u << ml);
+  xyzz9 acc = xyzz9_inf();
+  if (o < nout) {
+    const uint32_t cnt = hrow ? (o < (1u << mh) ? (1u << ml) : 1u) : (1u << mh);
+    for (uint32_t i = p; i < cnt; i += SPLIT_LANES) {
+      const uint32_t b = hrow ? (o << ml) + i : (i << ml) + o;  // H_(2^mh): o << ml = M
+      if (b == 0 || b > M) continue;
+      const uint32_t key = r * B + b;
+      if (row_ptr[key + 1] > row_ptr[key]) acc = xyzz9_add(acc, load_xyzz9(buckets + key));
+    }
+  }
+  sh[tid] = acc;
+  __syncthreads();
+  const uint32_t pstride = hrow ? 1u : SPLIT_PER;  // distance between a sum's partials
+  for (uint32_t step = SPLIT_LANES >> 1; step > 0; step >>= 1) {
+    if (p < step) sh[tid] = xyzz9_add(sh[tid], sh[tid + step * pstride]);
+    __syncthreads();
+  }
+  if (p == 0 && o < nout) store_xyzz9((hrow ? hsum : lsum) + (uint64_t)r * nout + o, sh[tid]);
+}
+
+// G_g of row r: SPLIT_FOLD plane sums side by side (SPLIT_PLANE_LANES threads
+// each), then 2^i-weighted by Horner's rule in one thread
+__global__ __launch_bounds__(SPLIT_FOLD* SPLIT_PLANE_LANES) void msm_split_planes_kernel(
+    const pt9* __restrict__ hsum, const pt9* __restrict__ lsum, uint32_t ml, uint32_t mh, uint32_t ng,
+    pt9* __restrict__ gsum) {
+  __shared__ xyzz9 sh[SPLIT_FOLD * SPLIT_PLANE_LANES];
+  const uint32_t r = blockIdx.y, g = blockIdx.x, tid = threadIdx.x;
+  const uint32_t q = tid / SPLIT_PLANE_LANES, p = tid % SPLIT_PLANE_LANES;
+  const uint32_t j = g * SPLIT_FOLD + q, m = ml + mh;
+  xyzz9 acc = xyzz9_inf();
+  if (j <= m) {
+    const bool low = j < ml;
+    const uint32_t jb = low ? j : j - ml;  // the index bit this plane selects
+    const uint32_t span = low ? ml : mh;   // index bits of the L / H sums below the top
+    const pt9* src = low ? lsum + (uint64_t)r * (1u << ml) : hsum + (uint64_t)r * ((1u << mh) + 1);
+    if (jb < span) {
+      // the indices below 2^span with bit jb set: k with a one inserted at bit jb
+      const uint32_t cnt = 1u << (span - 1);
+      for (uint32_t k = p; k < cnt; k += SPLIT_PLANE_LANES) {
+        const uint32_t x = ((k >> jb) << (jb + 1)) | (1u << jb) | (k & ((1u << jb) - 1));
+        acc = xyzz9_add(acc, load_xyzz9(src + x));
+      }
+    } else if (p == 0) {
+      acc = load_xyzz9(src + (1u << mh));  // j = m: only H_(2^mh) = B_M has bit mh
+    }
+  }
+  sh[tid] = acc;
+  __syncthreads();
+  for (uint32_t step = SPLIT_PLANE_LANES >> 1; step > 0; step >>= 1) {
+    if (p < step) sh[tid] = xyzz9_add(sh[tid], sh[tid + step]);
+    __syncthreads();
+  }
+  if (tid == 0) {
+    xyzz9 G = sh[(SPLIT_FOLD - 1) * SPLIT_PLANE_LANES];
+    for (int i = (int)SPLIT_FOLD - 2; i >= 0; i--) G = xyzz9_add(xyzz9_dbl(G), sh[i * SPLIT_PLANE_LANES]);
+    store_xyzz9(gsum + (uint64_t)r * ng + g, G);
+  }
+}
+
 // 8. fixed-base batches: the tables and the digit split.
 //    Table row w holds 2^(c w) * B_i (affine, R = 2^261 packed), so every
 //    signed digit d_(i,w) of any scalar maps to the same bucket set:
@@ -871,13 +973,15 @@ struct MsmShape {
   uint64_t Smax = 0;    // accumulation segments
   uint32_t L = 0, K = 0, cpw = 0;
   uint32_t nchunks = 0;
+  uint32_t ml = 0, mh = 0, ng = 0;  // digit-split reduction: bucket index bits below / above, folded terms per row
 };
 
 struct MsmBuffers {
   uint32_t *digits, *hist, *items_v, *row_ptr, *sums, *sorted, *hkey, *fc0, *fhist;
   uint16_t* items_f;
   pt9 *buckets, *head, *partial, *partial2, *rsum;
-  xyzz* rexp;
+  pt9 *hsum, *lsum, *gsum;  // digit-split reduction
+  xyzz *rexp, *gexp;
   MsmRow* rows;
 };
 
@@ -939,6 +1043,13 @@ static int make_shape(Ctx* c, int cb, uint32_t nrows, const std::vector<MsmRow>&
   sh.K = K;
   sh.cpw = (sh.nbins + K - 1) / K;
   {
+    uint32_t m = 0;  // nbins = 2^m
+    while ((1u << m) < sh.nbins) m++;
+    sh.ml = (m + 1) / 2;
+    sh.mh = m - sh.ml;
+    sh.ng = (m + 1 + SPLIT_FOLD - 1) / SPLIT_FOLD;
+  }
+  {
     uint64_t longest = sh.hist_len;  // scan partial sums, all levels, of the longest scan
     if ((uint64_t)sh.fchunks << sh.S > longest) longest = (uint64_t)sh.fchunks << sh.S;
     sh.nchunks = (uint32_t)(longest / (SCAN_CHUNK - 1) + 16);
@@ -972,6 +1083,10 @@ static size_t layout(const MsmShape& sh, char* base, MsmBuffers* b, uint32_t* di
   t.partial2 = (pt9*)take((size_t)sh.nrows * ((sh.cpw + RED_GROUP - 1) / RED_GROUP) * sizeof(pt9));
   t.rsum = (pt9*)take((size_t)sh.nrows * sizeof(pt9));
   t.rexp = (xyzz*)take((size_t)sh.nrows * sizeof(xyzz));
+  t.hsum = (pt9*)take((size_t)sh.nrows * ((1u << sh.mh) + 1) * sizeof(pt9));
+  t.lsum = (pt9*)take((size_t)sh.nrows * (1u << sh.ml) * sizeof(pt9));
+  t.gsum = (pt9*)take((size_t)sh.nrows * sh.ng * sizeof(pt9));
+  t.gexp = (xyzz*)take((size_t)sh.nrows * sh.ng * sizeof(xyzz));
   t.rows = (MsmRow*)take((size_t)sh.nrows * sizeof(MsmRow));
   if (b) *b = t;
   return off;
@@ -1024,8 +1139,11 @@ static void sort_placement(const MsmShape& sh, const MsmBuffers& b, hipStream_t 
 }
 
 // counted: the coarse histogram is already written (msm_digits_count_kernel)
+// split: rows are the windows of one MSM, reduced by digit split (their
+// folded terms in b.gexp, bit positions c r + SPLIT_FOLD g); otherwise one
+// sum per row in b.rexp
 static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* bases9, hipStream_t st,
-                    hipEvent_t sorted_ev = nullptr, bool counted = false) {
+                    hipEvent_t sorted_ev = nullptr, bool counted = false, bool split = false) {
   hipEvent_t ev;
   phase_begin(c, "msm_sort", st, &ev);
   const dim3 g(sh.max_tiles, sh.nrows);
@@ -1056,6 +1174,19 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
   phase_end(c, "msm_fixup", st, ev);
 
   phase_begin(c, "msm_reduce", st, &ev);
+  if (split) {
+    const uint32_t bh = ((1u << sh.mh) + 1 + SPLIT_PER - 1) / SPLIT_PER;
+    const uint32_t bl = ((1u << sh.ml) + SPLIT_PER - 1) / SPLIT_PER;
+    msm_split_sums_kernel<<<dim3(bh + bl, sh.nrows), SPLIT_THREADS, 0, st>>>(b.buckets, b.row_ptr, sh.B, sh.ml,
+                                                                            sh.mh, bh, b.hsum, b.lsum);
+    msm_split_planes_kernel<<<dim3(sh.ng, sh.nrows), SPLIT_FOLD * SPLIT_PLANE_LANES, 0, st>>>(
+        b.hsum, b.lsum, sh.ml, sh.mh, sh.ng, b.gsum);
+    msm_wsum_export_kernel<<<grid_for((uint64_t)sh.nrows * sh.ng, 64), 64, 0, st>>>(b.gsum, sh.nrows * sh.ng,
+                                                                                      b.gexp);
+    ZK_HIP(c, hipGetLastError());
+    phase_end(c, "msm_reduce", st, ev);
+    return ZK_OK;
+  }
   const uint32_t g1 = (sh.cpw + RED_CHUNK - 1) / RED_CHUNK;  // partials per window after the chunk kernel
   msm_reduce_chunks_kernel<<<dim3(g1, sh.nrows), RED_CHUNK, 0, st>>>(b.buckets, b.row_ptr, sh.B, sh.K, sh.cpw,
                                                                      b.partial);
@@ -1152,8 +1283,25 @@ static int msm_g1_single(Ctx* c, const fe* bases, const fe* bases9_ready, const 
       scalars, n, (uint32_t)cb, W, sh.S, sh.NC, nt, digits, b.hist);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_digits", st, ev);
-  rc = run_rows(c, sh, b, bases9, st, nullptr, true);
+  const bool split = c->msm_reduce_split != 0;
+  rc = run_rows(c, sh, b, bases9, st, nullptr, true, split);
   if (rc) return rc;
+  if (split) {
+    // term g of window w sits at bit c w + SPLIT_FOLD g; hand them over top first
+    const uint32_t T = W * sh.ng;
+    std::vector<xyzz> ts(T);
+    ZK_HIP(c, hipMemcpyAsync(ts.data(), b.gexp, (size_t)T * sizeof(xyzz), hipMemcpyDeviceToHost, st));
+    ZK_HIP(c, hipStreamSynchronize(st));
+    std::vector<xyzz> desc(T);
+    std::vector<uint32_t> pos(T);
+    for (uint32_t i = 0; i < T; i++) {
+      const uint32_t k = T - 1 - i, w = k / sh.ng, g = k % sh.ng;
+      desc[i] = ts[k];
+      pos[i] = (uint32_t)cb * w + SPLIT_FOLD * g;
+    }
+    g1_bits_combine(desc.data(), pos.data(), T, out_jac);
+    return ZK_OK;
+  }
   std::vector<xyzz> hs(W);
   ZK_HIP(c, hipMemcpyAsync(hs.data(), b.rexp, (size_t)W * sizeof(xyzz), hipMemcpyDeviceToHost, st));
   ZK_HIP(c, hipStreamSynchronize(st));

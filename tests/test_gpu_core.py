@@ -106,6 +106,26 @@ def test_msm_closed_form(zctx, n):
         assert msm_affine(zctx, bases, sc, n, c) == expect, c
 
 
+@pytest.mark.parametrize("split", [1, 0])
+def test_msm_both_reductions(zctx, split):
+    # digit-split and chunked bucket reductions, every window size with one
+    # or two index bits (c = 2, 3) up to the widest, on sparse windows (most
+    # buckets empty) and on dense ones
+    rng = random.Random(77)
+    for n in (5, 600):
+        ks = [rng.randrange(1, R_MOD) for _ in range(n)]
+        ss = [rng.randrange(R_MOD) for _ in range(n)]
+        bases = upload_bases(zctx, _bases(ks))
+        sc = upload_fr_mont(zctx, ss)
+        expect = M.msm_closed_form(ks, ss)
+        try:
+            zctx.set_tuning("msm_reduce_split", split)
+            for c in (2, 3, 4, 5, 6, 9, 12, 17, 21):
+                assert msm_affine(zctx, bases, sc, n, c) == expect, (n, c)
+        finally:
+            zctx.set_tuning("msm_reduce_split", 1)
+
+
 def test_msm_edge_cases(zctx):
     G = C.G1_GENERATOR
     P = C.g1_mul_affine(G, 77)
