@@ -76,12 +76,20 @@ enum { SCALE_NONE = 0, SCALE_CONST = 1, SCALE_TABLE = 2 };
 // n/2 (w^(n/2) = -1), so the twiddle is the entry n/2 - e negated; negating
 // the twiddle of a butterfly (u + w v, u - w v) only exchanges its two
 // outputs, so those stages write the sum where the difference goes and back.
-template <int TB, bool INV>
+// IP: in-place stages.  The tile is loaded with its rows in bit-reversed
+// order (row r of the LDS tile holds input row bitrev_k(r)); a radix-4 step
+// then reads its four values from the very rows it writes (o2, o2 + 2^(t+1),
+// o2 + 2^t, o2 + 3 2^t), the in-place form of the same butterflies with the
+// same twiddles, and the last stage leaves the rows in natural order.  A
+// thread only overwrites what it read, so a step needs one barrier (before
+// the next step's reads) instead of two.
+template <int TB, bool INV, bool IP>
 __global__ __launch_bounds__(NttTile<TB>::THREADS) void ntt_pass_kernel(
     const fe* __restrict__ in, fe* __restrict__ out, const uint32_t* __restrict__ tw, uint32_t log_n,
     uint32_t s0, uint32_t k, uint32_t log_t, uint64_t in_len, int pre_mode,
     const fe* __restrict__ pre_hi, const fe* __restrict__ pre_lo, int post_mode,
-    const fe* __restrict__ post_hi, const fe* __restrict__ post_lo, int last, uint32_t xcd_group) {
+    const fe* __restrict__ post_hi, const fe* __restrict__ post_lo, int last, uint32_t xcd_group,
+    int tw_stage) {
   constexpr int NTT_THREADS = NttTile<TB>::THREADS;
   constexpr int NTT_BFLY_PER_THREAD = NttTile<TB>::BFLY_PER_THREAD;
   __shared__ f9 lds[NttTile<TB>::ELEMS];
@@ -105,7 +113,8 @@ __global__ __launch_bounds__(NttTile<TB>::THREADS) void ntt_pass_kernel(
   // ---- load the R x T tile: element (r, c) <- in[j0 + c + r * n/R]
   for (uint32_t e = tid; e < RT; e += NTT_THREADS) {
     const uint32_t r = e >> log_t, c = e & (T - 1);
-    const uint64_t idx = j0 + c + (uint64_t)r * cols;
+    const uint32_t rr = IP ? (__brev(r) >> (32 - k)) : r;  // k >= 1
+    const uint64_t idx = j0 + c + (uint64_t)rr * cols;
     f9 v;
     if (idx < in_len) {
       v = unpack9(load_fe(in + idx));
@@ -142,9 +151,17 @@ __global__ __launch_bounds__(NttTile<TB>::THREADS) void ntt_pass_kernel(
   //      writing rows o2, o2 + 2^(t+1), o2 + 2^t, o2 + 2^t + 2^(t+1) with
   //      o2 = 4(q - d) + d.  An odd stage count ends with one radix-2 stage.
   const uint32_t halfRT = RT >> 1, quarterRT = RT >> 2;
+  // twiddle of global stage s = s0 + t: w_(2^(s+1))^x, x = low + d Ns.  One
+  // table of w^e (e <= n/2) reads entry x 2^(log_n - 1 - s); per-stage
+  // tables (tw_stage) hold w_(2^(s+1))^x for 0 <= x <= 2^s at 2^s - 1 + s,
+  // so the adjacent columns of a tile read adjacent entries at every stage.
+  // The inverse reads the reflected entry (n/2 - e, or 2^s - x).
   auto tw_cd = [&](uint32_t c, uint32_t d, uint32_t t) -> f9 {
     const uint64_t low = (j0 + c) & (Ns - 1);
-    const uint64_t e = (low + (uint64_t)d * Ns) << (log_n - 1 - (s0 + t));
+    const uint64_t x = low + (uint64_t)d * Ns;
+    const uint32_t s = s0 + t;
+    if (tw_stage) return tw9(tw, ((1ull << s) - 1 + s) + (INV ? (1ull << s) - x : x));
+    const uint64_t e = x << (log_n - 1 - s);
     return tw9(tw, INV ? (n >> 1) - e : e);
   };
   // the two outputs of a table-twiddle butterfly: (u + w v, u - w v + 2p),
@@ -166,17 +183,25 @@ __global__ __launch_bounds__(NttTile<TB>::THREADS) void ntt_pass_kernel(
     const uint32_t R4 = R >> 2;
     f9 x0, x1, x2, x3, w1, w2, w3;
     if (act) {
-      x0 = lds[(q << log_t) + c];
-      x1 = lds[((q + R4) << log_t) + c];
-      x2 = lds[((q + 2 * R4) << log_t) + c];
-      x3 = lds[((q + 3 * R4) << log_t) + c];
+      if (IP) {
+        const uint32_t o2 = 4 * (q - d) + d;
+        x0 = lds[(o2 << log_t) + c];
+        x1 = lds[((o2 + (2u << t)) << log_t) + c];
+        x2 = lds[((o2 + (1u << t)) << log_t) + c];
+        x3 = lds[((o2 + (3u << t)) << log_t) + c];
+      } else {
+        x0 = lds[(q << log_t) + c];
+        x1 = lds[((q + R4) << log_t) + c];
+        x2 = lds[((q + 2 * R4) << log_t) + c];
+        x3 = lds[((q + 3 * R4) << log_t) + c];
+      }
       if (s0 + t != 0) {
         w1 = tw_cd(c, d, t);
         w2 = tw_cd(c, d, t + 1);
       }
       w3 = tw_cd(c, d + (1u << t), t + 1);
     }
-    __syncthreads();
+    if (!IP) __syncthreads();
     if (act) {
       f9 A, B, C, D;
       f9 r0, r1, r2, r3;
@@ -221,12 +246,15 @@ __global__ __launch_bounds__(NttTile<TB>::THREADS) void ntt_pass_kernel(
       const uint32_t b = tid + i * NTT_THREADS;
       if (b < halfRT) {
         const uint32_t c = b & (T - 1), q = b >> log_t;
-        u[i] = lds[(q << log_t) + c];
-        v[i] = lds[((q + (R >> 1)) << log_t) + c];
-        if (s != 0) w[i] = tw_cd(c, q & ((1u << t) - 1), t);
+        const uint32_t d = q & ((1u << t) - 1);
+        const uint32_t pu = IP ? ((q - d) << 1) + d : q;
+        const uint32_t pv = IP ? pu + (1u << t) : q + (R >> 1);
+        u[i] = lds[(pu << log_t) + c];
+        v[i] = lds[(pv << log_t) + c];
+        if (s != 0) w[i] = tw_cd(c, d, t);
       }
     }
-    __syncthreads();
+    if (!IP) __syncthreads();
 #pragma unroll
     for (int i = 0; i < NTT_BFLY_PER_THREAD; i++) {
       const uint32_t b = tid + i * NTT_THREADS;
@@ -290,6 +318,21 @@ __global__ __launch_bounds__(256) void pow_expand_kernel(const fe* __restrict__ 
   const f9 t = Fr9::canonical(pow_lookup(hi, lo, i));
 #pragma unroll
   for (int k = 0; k < 9; k++) out[9 * i + k] = t.l[k];
+}
+
+// per-stage twiddle tables: stage s (0 <= s < log_n) holds w_(2^(s+1))^x =
+// w^(x 2^(log_n - 1 - s)) for 0 <= x <= 2^s at offset 2^s - 1 + s
+__global__ __launch_bounds__(256) void stage_expand_kernel(const fe* __restrict__ hi,
+                                                           const fe* __restrict__ lo, uint32_t log_n,
+                                                           uint32_t* __restrict__ out, uint64_t count) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= count) return;
+  uint32_t s = 0;
+  while (s + 1 < log_n && g >= (1ull << (s + 1)) - 1 + (s + 1)) s++;
+  const uint64_t x = g - ((1ull << s) - 1 + s);
+  const f9 t = Fr9::canonical(pow_lookup(hi, lo, x << (log_n - 1 - s)));
+#pragma unroll
+  for (int k = 0; k < 9; k++) out[9 * g + k] = t.l[k];
 }
 
 // host-built R = 2^256 tables -> R = 2^261 packed form, in place
@@ -410,11 +453,14 @@ int ntt_pow_tables(Ctx* c, uint32_t log_n, const zkh::Fe& base, const zkh::Fe& s
   return get_pow_tables(c, log_n, base, scale, hi, lo, st);
 }
 
-static int get_twiddles(Ctx* c, uint32_t log_n, const zkh::Fe& w, const uint32_t** out, hipStream_t st) {
-  std::array<uint64_t, 5> key = {log_n, w.v[0], w.v[1], w.v[2], w.v[3]};
+// stage: per-stage tables (2^s + 1 entries for each stage s) instead of
+// one table of w^e, 0 <= e <= n/2; either serves both directions
+static int get_twiddles(Ctx* c, uint32_t log_n, const zkh::Fe& w, const uint32_t** out, hipStream_t st,
+                        bool stage) {
+  std::array<uint64_t, 5> key = {log_n | (stage ? 0x100u : 0u), w.v[0], w.v[1], w.v[2], w.v[3]};
   auto it = c->twiddles.find(key);
   if (it == c->twiddles.end()) {
-    const uint64_t count = (1ull << log_n) / 2 + 1;  // w^e for 0 <= e <= n/2 (the inverse reads it reflected)
+    const uint64_t count = stage ? (1ull << log_n) - 1 + log_n : (1ull << log_n) / 2 + 1;
     const fe *hi, *lo;
     int rc = get_pow_tables(c, log_n, w, zkh::one(zkh::FR), &hi, &lo, st);
     if (rc) return rc;
@@ -423,7 +469,10 @@ static int get_twiddles(Ctx* c, uint32_t log_n, const zkh::Fe& w, const uint32_t
     CachedTable t;
     t.a.bytes = count * 36;
     ZK_HIP(c, hipMalloc(&t.a.ptr, t.a.bytes));
-    pow_expand_kernel<<<grid_for(count, 256), 256, 0, st>>>(hi, lo, (uint32_t*)t.a.ptr, count);
+    if (stage)
+      stage_expand_kernel<<<grid_for(count, 256), 256, 0, st>>>(hi, lo, log_n, (uint32_t*)t.a.ptr, count);
+    else
+      pow_expand_kernel<<<grid_for(count, 256), 256, 0, st>>>(hi, lo, (uint32_t*)t.a.ptr, count);
     ZK_HIP(c, hipGetLastError());
     c->table_bytes += t.bytes();
     it = c->twiddles.emplace(key, t).first;
@@ -514,7 +563,8 @@ int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint
   }
   // one table per (n, w) serves both directions
   const uint32_t* tw;
-  int rc = get_twiddles(c, log_n, w, &tw, st);
+  const bool tw_stage = c->ntt_stage_twiddles != 0;
+  int rc = get_twiddles(c, log_n, w, &tw, st, tw_stage);
   if (rc) return rc;
 
   int pre_mode = 0, post_mode = 0;
@@ -583,17 +633,23 @@ int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint
     const fe* s = (i == 0) ? in : dst[i - 1];
     const uint64_t len = (i == 0) ? in_len : n;
     dim3 grid((unsigned)tiles, batch);
-#define ZK_NTT_PASS(TBV, INVV)                                                                          \
-  ntt_pass_kernel<TBV, INVV><<<grid, NttTile<TBV>::THREADS, 0, st>>>(                                   \
+#define ZK_NTT_PASS(TBV, INVV, IPV)                                                                     \
+  ntt_pass_kernel<TBV, INVV, IPV><<<grid, NttTile<TBV>::THREADS, 0, st>>>(                              \
       s, dst[i], tw, log_n, s0, k, log_t, len, i == 0 ? pre_mode : 0, pre_hi, pre_lo,                   \
-      i == P - 1 ? post_mode : 0, post_hi, post_lo, i == P - 1 ? 1 : 0, xg)
+      i == P - 1 ? post_mode : 0, post_hi, post_lo, i == P - 1 ? 1 : 0, xg, tw_stage ? 1 : 0)
+#define ZK_NTT_PASS_IP(TBV, INVV)            \
+  do {                                       \
+    if (c->ntt_inplace) ZK_NTT_PASS(TBV, INVV, true); \
+    else ZK_NTT_PASS(TBV, INVV, false);      \
+  } while (0)
     if (big) {
-      if (inverse) ZK_NTT_PASS(NTT_BIG_TILE_BITS, true);
-      else ZK_NTT_PASS(NTT_BIG_TILE_BITS, false);
+      if (inverse) ZK_NTT_PASS_IP(NTT_BIG_TILE_BITS, true);
+      else ZK_NTT_PASS_IP(NTT_BIG_TILE_BITS, false);
     } else {
-      if (inverse) ZK_NTT_PASS(NTT_TILE_BITS, true);
-      else ZK_NTT_PASS(NTT_TILE_BITS, false);
+      if (inverse) ZK_NTT_PASS_IP(NTT_TILE_BITS, true);
+      else ZK_NTT_PASS_IP(NTT_TILE_BITS, false);
     }
+#undef ZK_NTT_PASS_IP
 #undef ZK_NTT_PASS
     ZK_HIP(c, hipGetLastError());
     s0 += k;
@@ -617,7 +673,7 @@ int ntt_prepare(Ctx* c, uint32_t log_n, bool inverse, const zkh::Fe* omega, cons
   c->table_call++;
   c->table_pin = true;
   const uint32_t* tw;
-  int rc = get_twiddles(c, log_n, w, &tw, st);
+  int rc = get_twiddles(c, log_n, w, &tw, st, c->ntt_stage_twiddles != 0);
   const fe *h, *l;
   if (rc == ZK_OK && !inverse) {
     if (shift) rc = get_pow_tables(c, log_n, *shift, zkh::one(zkh::FR), &h, &l, st);

@@ -66,6 +66,38 @@ def test_ntt_in_place_and_pass_splits(zctx, log_n):
     zctx.set_tuning("ntt_max_pass_bits", 0)
 
 
+@pytest.mark.parametrize("inplace,stage", [(1, 1), (0, 1), (1, 0), (0, 0)])
+def test_ntt_pass_forms(zctx, inplace, stage):
+    # in-place or Stockham-order pass stages, per-stage or single twiddle
+    # tables: all four variants against the oracle, one- and multi-pass
+    # plans, odd pass sizes (a lone radix-2 stage), many-column tiles
+    import numpy as np
+    from oracle import cref
+    from pyzkp_amd.field import words_to_ints
+    zctx.set_tuning("ntt_inplace", inplace)
+    zctx.set_tuning("ntt_stage_twiddles", stage)
+    try:
+        for log_n, bits in ((5, 0), (11, 0), (13, 5), (17, 0), (17, 9)):
+            n = 1 << log_n
+            rng = random.Random(log_n * 7 + bits)
+            a = [rng.randrange(R_MOD) for _ in range(n)]
+            aw = np.array([[(x >> (64 * i)) & ((1 << 64) - 1) for i in range(4)] for x in a], dtype=np.uint64)
+            w = N.omega_for_size(n)
+            src = upload_fr_mont(zctx, a)
+            dst = zctx.alloc(32 * n)
+            zctx.set_tuning("ntt_max_pass_bits", bits)
+            for inv, sh in ((False, None), (True, None), (False, 5), (True, 5)):
+                zctx.ntt(src.ptr, n, dst.ptr, log_n, inv, shift_mont=_mont(sh) if sh else None)
+                expect = words_to_ints(cref.ntt_words(aw.reshape(-1), n, log_n, w, inv, sh, threads=0).reshape(-1, 4))
+                assert download_fr(zctx, dst.ptr, n) == expect, (log_n, bits, inv, sh)
+            src.free()
+            dst.free()
+    finally:
+        zctx.set_tuning("ntt_max_pass_bits", 0)
+        zctx.set_tuning("ntt_inplace", 1)
+        zctx.set_tuning("ntt_stage_twiddles", 1)
+
+
 def test_ntt_batch(zctx):
     log_n, batch = 9, 5
     n = 1 << log_n
