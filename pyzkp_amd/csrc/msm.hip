@@ -415,8 +415,10 @@ __device__ __forceinline__ void fine_segment(const uint32_t* __restrict__ hoff, 
 __global__ __launch_bounds__(256) void msm_fine_chunks_kernel(const uint32_t* __restrict__ hoff,
                                                               const MsmRow* __restrict__ rows, uint32_t NC,
                                                               uint32_t nseg, uint32_t fch,
-                                                              uint32_t* __restrict__ nch) {
+                                                              uint32_t* __restrict__ nch,
+                                                              uint32_t* __restrict__ maxrun) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g == 0) *maxrun = 0;  // longest head run of this launch sequence, raised by the accumulation
   if (g >= nseg) return;
   uint32_t r, k, s0, s1;
   fine_segment(hoff, rows, NC, g, r, k, s0, s1);
@@ -595,7 +597,7 @@ __global__ __launch_bounds__(FINE_STAGE_THREADS) void msm_fine_scatter_staged_ke
 __global__ __launch_bounds__(256) void msm_accumulate_kernel(
     const fe* __restrict__ bases, const uint32_t* __restrict__ sorted,
     const uint32_t* __restrict__ row_ptr, uint32_t nkeys, uint32_t L, pt9* __restrict__ buckets,
-    uint32_t* __restrict__ head_key, pt9* __restrict__ head) {
+    uint32_t* __restrict__ head_key, pt9* __restrict__ head, uint32_t* __restrict__ maxrun) {
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t M = row_ptr[nkeys];
   const uint64_t start = s * L;
@@ -612,12 +614,19 @@ __global__ __launch_bounds__(256) void msm_accumulate_kernel(
   uint32_t hk = NO_KEY;
   xyzz9 acc = xyzz9_inf();
   bool acc_inf = true;
+  // a head partial of bucket k: the run of heads it belongs to spans `len`
+  // segments; the fix-up rounds only have work for runs longer than their stride
+  auto note_run = [&](uint32_t a, uint32_t b) {
+    const uint32_t len = (b - 1) / L - a / L;
+    if (len > 1) atomicMax(maxrun, len);
+  };
   for (uint64_t p = start; p < end; p++) {
     while (p >= rend) {
       if (rend > rbeg) {
         if (rbeg < start) {
           hk = k;
           store_xyzz9(head + s, acc);
+          note_run(rbeg, rend);
         } else {
           store_xyzz9(buckets + k, acc);
         }
@@ -635,6 +644,7 @@ __global__ __launch_bounds__(256) void msm_accumulate_kernel(
   if (rbeg < start) {
     hk = k;
     store_xyzz9(head + s, acc);
+    note_run(rbeg, rend);
   } else {
     store_xyzz9(buckets + k, acc);
   }
@@ -661,7 +671,9 @@ __global__ __launch_bounds__(256) void msm_fixup_round_kernel(const uint32_t* __
                                                               uint32_t nkeys, uint32_t L,
                                                               uint64_t stride,
                                                               const uint32_t* __restrict__ head_key,
-                                                              pt9* __restrict__ head) {
+                                                              pt9* __restrict__ head,
+                                                              const uint32_t* __restrict__ maxrun) {
+  if (stride >= *maxrun) return;  // no run of heads is longer than the stride: nothing to fold
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t M = row_ptr[nkeys];
   const uint64_t S = (M + L - 1) / L;
@@ -977,7 +989,7 @@ struct MsmShape {
 };
 
 struct MsmBuffers {
-  uint32_t *digits, *hist, *items_v, *row_ptr, *sums, *sorted, *hkey, *fc0, *fhist;
+  uint32_t *digits, *hist, *items_v, *row_ptr, *sums, *sorted, *hkey, *fc0, *fhist, *maxrun;
   uint16_t* items_f;
   pt9 *buckets, *head, *partial, *partial2, *rsum;
   pt9 *hsum, *lsum, *gsum;  // digit-split reduction
@@ -1088,6 +1100,7 @@ static size_t layout(const MsmShape& sh, char* base, MsmBuffers* b, uint32_t* di
   t.gsum = (pt9*)take((size_t)sh.nrows * sh.ng * sizeof(pt9));
   t.gexp = (xyzz*)take((size_t)sh.nrows * sh.ng * sizeof(xyzz));
   t.rows = (MsmRow*)take((size_t)sh.nrows * sizeof(MsmRow));
+  t.maxrun = (uint32_t*)take(4);
   if (b) *b = t;
   return off;
 }
@@ -1121,7 +1134,8 @@ static void sort_placement(const MsmShape& sh, const MsmBuffers& b, hipStream_t 
           b.digits, b.rows, sh.S, sh.NC, b.hist, b.items_v, b.items_f);
   }
   const uint32_t nseg = sh.nrows * sh.NC, NF = 1u << sh.S;
-  msm_fine_chunks_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(b.hist, b.rows, sh.NC, nseg, sh.fch, b.fc0);
+  msm_fine_chunks_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(b.hist, b.rows, sh.NC, nseg, sh.fch, b.fc0,
+                                                              b.maxrun);
   scan_exclusive(b.fc0, b.fc0, nseg, b.sums, st);  // fc0[nseg] = chunks in use (<= sh.fchunks)
   msm_fine_count_kernel<PK><<<sh.fchunks, FINE_THREADS, NF * 4, st>>>(b.items_v, b.items_f, b.hist, b.rows, sh.S,
                                                                       sh.NC, nseg, sh.fch, b.fc0, b.fhist);
@@ -1160,14 +1174,14 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
 
   phase_begin(c, "msm_accumulate", st, &ev);
   msm_accumulate_kernel<<<grid_for(sh.Smax, 256), 256, 0, st>>>(bases9, b.sorted, b.row_ptr, sh.nkeys, sh.L,
-                                                               b.buckets, b.hkey, b.head);
+                                                               b.buckets, b.hkey, b.head, b.maxrun);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_accumulate", st, ev);
 
   phase_begin(c, "msm_fixup", st, &ev);
   for (uint64_t stride = 1; stride < sh.Smax; stride *= FIXUP_FAN)
     msm_fixup_round_kernel<<<grid_for(sh.Smax, 256), 256, 0, st>>>(b.row_ptr, sh.nkeys, sh.L, stride, b.hkey,
-                                                                  b.head);
+                                                                  b.head, b.maxrun);
   msm_fixup_final_kernel<<<grid_for(sh.Smax, 256), 256, 0, st>>>(b.row_ptr, sh.nkeys, sh.L, b.buckets, b.hkey,
                                                                 b.head);
   ZK_HIP(c, hipGetLastError());

@@ -53,10 +53,11 @@ static constexpr int POW_LO_BITS = POW_TABLE_LO_BITS;  // two-level power tables
 // tables: R = 2^261 values packed in 32 bytes
 __device__ __forceinline__ f9 tab9(const fe* __restrict__ t, uint64_t i) { return unpack9(load_fe(t + i)); }
 // twiddles: 9 limbs per entry
-__device__ __forceinline__ f9 tw9(const uint32_t* __restrict__ t, uint64_t i) {
+__device__ __forceinline__ f9 tw9(const uint32_t* __restrict__ t, uint32_t i) {
+  const uint32_t* e = t + (uint64_t)i * 9;
   f9 r;
 #pragma unroll
-  for (int k = 0; k < 9; k++) r.l[k] = t[9 * i + k];
+  for (int k = 0; k < 9; k++) r.l[k] = e[k];
   return r;
 }
 
@@ -156,13 +157,14 @@ __global__ __launch_bounds__(NttTile<TB>::THREADS) void ntt_pass_kernel(
   // tables (tw_stage) hold w_(2^(s+1))^x for 0 <= x <= 2^s at 2^s - 1 + s,
   // so the adjacent columns of a tile read adjacent entries at every stage.
   // The inverse reads the reflected entry (n/2 - e, or 2^s - x).
+  //  Indices stay in 32 bits (n <= 2^28: x < 2^27, entries below 2^28 + 28).
+  const uint32_t jc0 = (uint32_t)j0, nmask = (uint32_t)Ns - 1u;
   auto tw_cd = [&](uint32_t c, uint32_t d, uint32_t t) -> f9 {
-    const uint64_t low = (j0 + c) & (Ns - 1);
-    const uint64_t x = low + (uint64_t)d * Ns;
+    const uint32_t x = ((jc0 + c) & nmask) + (d << s0);
     const uint32_t s = s0 + t;
-    if (tw_stage) return tw9(tw, ((1ull << s) - 1 + s) + (INV ? (1ull << s) - x : x));
-    const uint64_t e = x << (log_n - 1 - s);
-    return tw9(tw, INV ? (n >> 1) - e : e);
+    if (tw_stage) return tw9(tw, ((1u << s) - 1u + s) + (INV ? (1u << s) - x : x));
+    const uint32_t e = x << (log_n - 1 - s);
+    return tw9(tw, INV ? (uint32_t)(n >> 1) - e : e);
   };
   // the two outputs of a table-twiddle butterfly: (u + w v, u - w v + 2p),
   // exchanged in the inverse (the table gives -w there)
