@@ -1242,9 +1242,11 @@ static void export_jacobian(const xyzz& acc, uint64_t out[12]) {
 // groups alternate between the context stream and a second stream, and
 // group g + 1 starts sorting when group g's sort is done, so that the sort
 // and reduction of one group can run beside the accumulation of its
-// neighbour.  On MI355X at 2^22 this measured slower than one group (the
-// accumulation occupies every CU, so little overlaps, and every group pays
-// the reduction's latency), so the default is one group.
+// neighbour.  On MI355X at 2^22 this measured no faster than one group
+// (prepared bases, digit-split reduction: 6.01 ms for one or two groups,
+// 6.09 for three, 6.53 for five): the accumulation's three waves per SIMD
+// leave too few registers for a sort wave beside them, so the streams
+// interleave instead of overlapping.  The default is one group.
 // One launch sequence over all W windows (the default shape).  `arena`
 // null: the context's growable scratch; else the caller's arena of
 // `arena_bytes`, which must hold the layout (nothing is allocated then).
@@ -1362,6 +1364,10 @@ void msm_bases_destroy(MsmBases* b) {
   delete b;
 }
 
+static int msm_g1_grouped(Ctx* c, const fe* bases, const fe* bases9_ready, const fe* scalars, uint64_t n, int cb,
+                          uint32_t G, uint64_t out_jac[12], hipStream_t st);
+static uint32_t msm_group_count(Ctx* c, int cb);
+
 int msm_g1_prepared_run(Ctx* c, const MsmBases* b, const fe* scalars, uint64_t n, int window_bits,
                         uint64_t out_jac[12], hipStream_t st) {
   for (int i = 0; i < 12; i++) out_jac[i] = 0;
@@ -1370,6 +1376,8 @@ int msm_g1_prepared_run(Ctx* c, const MsmBases* b, const fe* scalars, uint64_t n
   int cb;
   int rc = msm_check_args(c, n, window_bits, &cb);
   if (rc) return rc;
+  const uint32_t G = msm_group_count(c, cb);
+  if (G > 1) return msm_g1_grouped(c, nullptr, (const fe*)b->conv.ptr, scalars, n, cb, G, out_jac, st);
   return msm_g1_single(c, nullptr, (const fe*)b->conv.ptr, scalars, n, cb, out_jac, st, nullptr, 0);
 }
 
@@ -1416,17 +1424,12 @@ int msm_g1_noalloc_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, i
   return msm_g1_single(c, bases, nullptr, scalars, n, cb, out_jac, st, arena, MSM_NOALLOC_BYTES);
 }
 
-int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
-               uint64_t out_jac[12], hipStream_t st) {
-  for (int i = 0; i < 12; i++) out_jac[i] = 0;
-  if (n == 0) return ZK_OK;
-  int cb;
-  int rc0 = msm_check_args(c, n, window_bits, &cb);
-  if (rc0) return rc0;
+// The windows split into G groups (the msm_groups tuning knob), each its own
+// sort -> accumulate -> fix-up -> reduce sequence, on two streams (see above).
+// `bases9_ready` as in msm_g1_single.
+static int msm_g1_grouped(Ctx* c, const fe* bases, const fe* bases9_ready, const fe* scalars, uint64_t n, int cb,
+                          uint32_t G, uint64_t out_jac[12], hipStream_t st) {
   const uint32_t W = (255 + cb - 1) / cb;
-  uint32_t G = c->msm_groups ? c->msm_groups : 1u;
-  if (G > W) G = W;
-  if (G == 1) return msm_g1_single(c, bases, nullptr, scalars, n, cb, out_jac, st, nullptr, 0);
   std::vector<uint32_t> wb(G + 1);
   for (uint32_t g = 0; g <= G; g++) wb[g] = (uint32_t)((uint64_t)g * W / G);
   // rows = windows: row w holds digits[w n .. w n + n), base index = position
@@ -1435,7 +1438,7 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   std::vector<MsmShape> sh(G);
   std::vector<size_t> off(G + 1);
   const size_t o_bases = align256((size_t)W * n * 4);
-  off[0] = align256(o_bases + (size_t)n * 64);
+  off[0] = bases9_ready ? o_bases : align256(o_bases + (size_t)n * 64);
   for (uint32_t g = 0; g < G; g++) {
     for (uint32_t w = wb[g]; w < wb[g + 1]; w++) {
       MsmRow r;
@@ -1456,12 +1459,10 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   void* base;
   int rc = ensure_scratch(c, 2, off[G], &base);
   if (rc) return rc;
-  if (G > 1) {
-    rc = ensure_aux(c, G + 2);
-    if (rc) return rc;
-  }
+  rc = ensure_aux(c, G + 2);
+  if (rc) return rc;
   uint32_t* digits = (uint32_t*)base;
-  fe* bases9 = (fe*)((char*)base + o_bases);
+  const fe* bases9 = bases9_ready ? bases9_ready : (const fe*)((char*)base + o_bases);
   std::vector<MsmBuffers> b(G);
   for (uint32_t g = 0; g < G; g++) {
     layout(sh[g], (char*)base + off[g], &b[g], digits + (uint64_t)wb[g] * n);
@@ -1470,34 +1471,45 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
 
   hipEvent_t ev;
   phase_begin(c, "msm_digits", st, &ev);
-  msm_bases_convert_kernel<<<grid_for(n, 256), 256, 0, st>>>(bases, n, bases9);
-  if (G == 1)
-    msm_digits_count_kernel<<<nt, DIGITS_COUNT_THREADS, (size_t)W * sh[0].NC * 4, st>>>(
-        scalars, n, (uint32_t)cb, W, sh[0].S, sh[0].NC, nt, digits, b[0].hist);
-  else
-    msm_digits_kernel<<<grid_for(n, 256), 256, 0, st>>>(scalars, n, (uint32_t)cb, W, digits);
+  if (!bases9_ready) msm_bases_convert_kernel<<<grid_for(n, 256), 256, 0, st>>>(bases, n, (fe*)bases9);
+  msm_digits_kernel<<<grid_for(n, 256), 256, 0, st>>>(scalars, n, (uint32_t)cb, W, digits);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_digits", st, ev);
 
-  if (G == 1) {
-    rc = run_rows(c, sh[0], b[0], bases9, st, nullptr, true);
+  const bool split = c->msm_reduce_split != 0;
+  hipStream_t S[2] = {st, c->aux_stream};
+  hipEvent_t* E = c->events.data();  // E[g]: group g sorted; E[G]: digits ready; E[G + 1]: aux done
+  ZK_HIP(c, hipEventRecord(E[G], st));
+  ZK_HIP(c, hipStreamWaitEvent(S[1], E[G], 0));
+  for (uint32_t g = 0; g < G; g++) {
+    hipStream_t s = S[g & 1];
+    if (g > 0) ZK_HIP(c, hipStreamWaitEvent(s, E[g - 1], 0));
+    rc = run_rows(c, sh[g], b[g], bases9, s, E[g], false, split);
     if (rc) return rc;
-  } else {
-    hipStream_t S[2] = {st, c->aux_stream};
-    hipEvent_t* E = c->events.data();  // E[g]: group g sorted; E[G]: digits ready; E[G + 1]: aux done
-    ZK_HIP(c, hipEventRecord(E[G], st));
-    ZK_HIP(c, hipStreamWaitEvent(S[1], E[G], 0));
-    for (uint32_t g = 0; g < G; g++) {
-      hipStream_t s = S[g & 1];
-      if (g > 0) ZK_HIP(c, hipStreamWaitEvent(s, E[g - 1], 0));
-      rc = run_rows(c, sh[g], b[g], bases9, s, E[g]);
-      if (rc) return rc;
-    }
-    ZK_HIP(c, hipEventRecord(E[G + 1], S[1]));
-    ZK_HIP(c, hipStreamWaitEvent(st, E[G + 1], 0));
   }
+  ZK_HIP(c, hipEventRecord(E[G + 1], S[1]));
+  ZK_HIP(c, hipStreamWaitEvent(st, E[G + 1], 0));
 
   // window combination on the host: acc = Sum_w 2^(c w) S_w (Horner)
+  if (split) {
+    // group g's term (row r, fold j) sits at bit c (wb[g] + r) + SPLIT_FOLD j
+    const uint32_t ng = sh[0].ng;
+    std::vector<xyzz> ts((size_t)W * ng);
+    for (uint32_t g = 0; g < G; g++)
+      ZK_HIP(c, hipMemcpyAsync(ts.data() + (size_t)wb[g] * ng, b[g].gexp,
+                               (size_t)(wb[g + 1] - wb[g]) * ng * sizeof(xyzz), hipMemcpyDeviceToHost, st));
+    ZK_HIP(c, hipStreamSynchronize(st));
+    const uint32_t T = W * ng;
+    std::vector<xyzz> desc(T);
+    std::vector<uint32_t> pos(T);
+    for (uint32_t i = 0; i < T; i++) {
+      const uint32_t k = T - 1 - i, w = k / ng, j = k % ng;
+      desc[i] = ts[k];
+      pos[i] = (uint32_t)cb * w + SPLIT_FOLD * j;
+    }
+    g1_bits_combine(desc.data(), pos.data(), T, out_jac);
+    return ZK_OK;
+  }
   std::vector<xyzz> hs(W);
   for (uint32_t g = 0; g < G; g++)
     ZK_HIP(c, hipMemcpyAsync(hs.data() + wb[g], b[g].rexp, (size_t)(wb[g + 1] - wb[g]) * sizeof(xyzz),
@@ -1505,6 +1517,24 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   ZK_HIP(c, hipStreamSynchronize(st));
   g1_window_combine(hs.data(), W, cb, out_jac);
   return ZK_OK;
+}
+
+static uint32_t msm_group_count(Ctx* c, int cb) {
+  const uint32_t W = (255 + cb - 1) / cb;
+  uint32_t G = c->msm_groups ? c->msm_groups : 1u;
+  return G > W ? W : G;
+}
+
+int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int window_bits,
+               uint64_t out_jac[12], hipStream_t st) {
+  for (int i = 0; i < 12; i++) out_jac[i] = 0;
+  if (n == 0) return ZK_OK;
+  int cb;
+  int rc0 = msm_check_args(c, n, window_bits, &cb);
+  if (rc0) return rc0;
+  const uint32_t G = msm_group_count(c, cb);
+  if (G == 1) return msm_g1_single(c, bases, nullptr, scalars, n, cb, out_jac, st, nullptr, 0);
+  return msm_g1_grouped(c, bases, nullptr, scalars, n, cb, G, out_jac, st);
 }
 
 // ------------------------------------------------------------ fixed-base sets
