@@ -208,13 +208,23 @@ def test_msm_window_groups(zctx):
     bases = workload.device_bases(zctx, kw)
     sc = workload.device_fr_mont(zctx, sw)
     expect = M.msm_closed_form(words_to_ints(kw), words_to_ints(sw))
+    from pyzkp_amd.curve import from_jacobian_mont
+    mb = zctx.msm_bases_create(bases.ptr, n)
+    aff = lambda r: None if r[2] == 0 else (r[0], r[1])
     try:
         for g in (2, 3, 5):
             zctx.set_tuning("msm_groups", g)
             for c in (8, 13):
                 assert msm_affine(zctx, bases, sc, n, c) == expect, (g, c)
+            # prepared base set, with the digit-split and the chunked reductions
+            for split in (1, 0):
+                zctx.set_tuning("msm_reduce_split", split)
+                assert aff(from_jacobian_mont(*zctx.msm_g1_prepared(mb, sc.ptr, n, 11))) == expect, (g, split)
+            zctx.set_tuning("msm_reduce_split", 1)
     finally:
         zctx.set_tuning("msm_groups", 0)
+        zctx.set_tuning("msm_reduce_split", 1)
+        mb.free()
 
 
 def test_msm_wide_windows(zctx):
