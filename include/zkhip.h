@@ -302,7 +302,9 @@ int zk_pairing_check(const uint64_t* g1, const uint64_t* g2, uint32_t n, int* ok
  * or 12), "ntt_xcd_group" (tile numbering per XCD; 0 = auto),
  * "ntt_inplace" (1, the default: in-place pass stages on a bit-reversed
  * tile load; 0: Stockham order), "ntt_stage_twiddles" (1, the default:
- * per-stage twiddle tables; 0: one table of w^e),
+ * per-stage twiddle tables; 0: one table of w^e), "ntt_full_scale" (1,
+ * the default: coset factors from a full table per size and shift up to
+ * 2^26; 0: two-level power tables),
  * "ntt_table_cap_mb" (bytes of cached NTT tables a context keeps before
  * freeing the least recently used unpinned ones; 0 = no cap, default 16384),
  * "msm_segment" (pairs per accumulation thread), "msm_reduce_chunk"

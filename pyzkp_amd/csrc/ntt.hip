@@ -66,7 +66,10 @@ __device__ __forceinline__ f9 pow_lookup(const fe* __restrict__ hi, const fe* __
 }
 
 // scale modes
-enum { SCALE_NONE = 0, SCALE_CONST = 1, SCALE_TABLE = 2 };
+// SCALE_TABLE: two-level power tables (one product per lookup); SCALE_FULL:
+// one 9-limb entry per element (scale * base^j, built once per size and
+// shift), read at the element's own index
+enum { SCALE_NONE = 0, SCALE_CONST = 1, SCALE_TABLE = 2, SCALE_FULL = 3 };
 
 // xcd_group > 1: tiles are numbered so that the workgroups one XCD receives
 // (the hardware deals blockIdx.x round robin over the 8 XCDs) take
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(NttTile<TB>::THREADS) void ntt_pass_kernel(
     uint32_t s0, uint32_t k, uint32_t log_t, uint64_t in_len, int pre_mode,
     const fe* __restrict__ pre_hi, const fe* __restrict__ pre_lo, int post_mode,
     const fe* __restrict__ post_hi, const fe* __restrict__ post_lo, int last, uint32_t xcd_group,
-    int tw_stage) {
+    int tw_stage, const uint32_t* __restrict__ pre_full, const uint32_t* __restrict__ post_full) {
   constexpr int NTT_THREADS = NttTile<TB>::THREADS;
   constexpr int NTT_BFLY_PER_THREAD = NttTile<TB>::BFLY_PER_THREAD;
   __shared__ f9 lds[NttTile<TB>::ELEMS];
@@ -121,6 +124,7 @@ __global__ __launch_bounds__(NttTile<TB>::THREADS) void ntt_pass_kernel(
       v = unpack9(load_fe(in + idx));
       if (s0 == 0) v = Fr9::reduce2p(v);  // caller data: any value < 2^256 (< 5.3 p) -> < 4p
       if (pre_mode == SCALE_TABLE) v = Fr9::mul(v, pow_lookup(pre_hi, pre_lo, idx));
+      else if (pre_mode == SCALE_FULL) v = Fr9::mul(v, tw9(pre_full, (uint32_t)idx));
     } else {
       v = Fr9::zero();
     }
@@ -302,6 +306,8 @@ __global__ __launch_bounds__(NttTile<TB>::THREADS) void ntt_pass_kernel(
       x = Fr9::mul(x, post_c);
     } else if (post_mode == SCALE_TABLE) {
       x = Fr9::mul(x, pow_lookup(post_hi, post_lo, pos));
+    } else if (post_mode == SCALE_FULL) {
+      x = Fr9::mul(x, tw9(post_full, (uint32_t)pos));
     } else {
       x = Fr9::reduce_est(x);  // limbs up to 2.5 * 2^30 in, normalised out
     }
@@ -449,6 +455,37 @@ static int get_pow_tables(Ctx* c, uint32_t log_n, const zkh::Fe& base, const zkh
   return ZK_OK;
 }
 
+// A full scaling table: scale * base^j for 0 <= j < 2^log_n, 9 limbs per
+// entry, expanded once from the two-level tables and cached with them (its
+// key marks log_n with 0x100).  Used for sizes up to 2^FULL_SCALE_MAX_BITS.
+static constexpr uint32_t FULL_SCALE_MAX_BITS = 26;
+static int get_scale_full(Ctx* c, uint32_t log_n, const zkh::Fe& base, const zkh::Fe& scale, const uint32_t** out,
+                          hipStream_t st) {
+  std::array<uint64_t, 9> key;
+  key[0] = log_n | 0x100u;
+  for (int i = 0; i < 4; i++) key[1 + i] = base.v[i];
+  for (int i = 0; i < 4; i++) key[5 + i] = scale.v[i];
+  auto it = c->pow_tables.find(key);
+  if (it == c->pow_tables.end()) {
+    const fe *hi, *lo;
+    int rc = get_pow_tables(c, log_n, base, scale, &hi, &lo, st);
+    if (rc) return rc;
+    const uint64_t count = 1ull << log_n;
+    rc = make_table_room(c, count * 36);
+    if (rc) return rc;
+    CachedTable t;
+    t.a.bytes = count * 36;
+    ZK_HIP(c, hipMalloc(&t.a.ptr, t.a.bytes));
+    pow_expand_kernel<<<grid_for(count, 256), 256, 0, st>>>(hi, lo, (uint32_t*)t.a.ptr, count);
+    ZK_HIP(c, hipGetLastError());
+    c->table_bytes += t.bytes();
+    it = c->pow_tables.emplace(key, t).first;
+  }
+  touch(c, it->second);
+  *out = (const uint32_t*)it->second.a.ptr;
+  return ZK_OK;
+}
+
 int ntt_pow_tables(Ctx* c, uint32_t log_n, const zkh::Fe& base, const zkh::Fe& scale, const fe** hi,
                    const fe** lo, hipStream_t st) {
   c->table_call++;
@@ -571,18 +608,30 @@ int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint
 
   int pre_mode = 0, post_mode = 0;
   const fe *pre_hi = nullptr, *pre_lo = nullptr, *post_hi = nullptr, *post_lo = nullptr;
+  const uint32_t *pre_full = nullptr, *post_full = nullptr;
+  const bool full = c->ntt_full_scale && log_n <= FULL_SCALE_MAX_BITS;
   if (!inverse) {
     if (shift) {
-      pre_mode = 2;
-      rc = get_pow_tables(c, log_n, *shift, zkh::one(zkh::FR), &pre_hi, &pre_lo, st);
+      if (full) {
+        pre_mode = SCALE_FULL;
+        rc = get_scale_full(c, log_n, *shift, zkh::one(zkh::FR), &pre_full, st);
+      } else {
+        pre_mode = SCALE_TABLE;
+        rc = get_pow_tables(c, log_n, *shift, zkh::one(zkh::FR), &pre_hi, &pre_lo, st);
+      }
       if (rc) return rc;
     }
   } else {
     zkh::Fe ninv = zkh::inv(zkh::FR, zkh::from_u64(zkh::FR, n));
     if (shift) {
       zkh::Fe ginv = zkh::is_zero(*shift) ? zkh::zero() : zkh::inv(zkh::FR, *shift);
-      post_mode = 2;
-      rc = get_pow_tables(c, log_n, ginv, ninv, &post_hi, &post_lo, st);
+      if (full) {
+        post_mode = SCALE_FULL;
+        rc = get_scale_full(c, log_n, ginv, ninv, &post_full, st);
+      } else {
+        post_mode = SCALE_TABLE;
+        rc = get_pow_tables(c, log_n, ginv, ninv, &post_hi, &post_lo, st);
+      }
     } else {
       post_mode = 1;
       // constant n^-1: a one-entry "hi" table
@@ -638,7 +687,8 @@ int ntt_run(Ctx* c, const fe* in, uint64_t in_len, fe* out, uint32_t log_n, uint
 #define ZK_NTT_PASS(TBV, INVV, IPV)                                                                     \
   ntt_pass_kernel<TBV, INVV, IPV><<<grid, NttTile<TBV>::THREADS, 0, st>>>(                              \
       s, dst[i], tw, log_n, s0, k, log_t, len, i == 0 ? pre_mode : 0, pre_hi, pre_lo,                   \
-      i == P - 1 ? post_mode : 0, post_hi, post_lo, i == P - 1 ? 1 : 0, xg, tw_stage ? 1 : 0)
+      i == P - 1 ? post_mode : 0, post_hi, post_lo, i == P - 1 ? 1 : 0, xg, tw_stage ? 1 : 0, pre_full,     \
+      post_full)
 #define ZK_NTT_PASS_IP(TBV, INVV)            \
   do {                                       \
     if (c->ntt_inplace) ZK_NTT_PASS(TBV, INVV, true); \
@@ -677,13 +727,18 @@ int ntt_prepare(Ctx* c, uint32_t log_n, bool inverse, const zkh::Fe* omega, cons
   const uint32_t* tw;
   int rc = get_twiddles(c, log_n, w, &tw, st, c->ntt_stage_twiddles != 0);
   const fe *h, *l;
+  const uint32_t* f;
+  const bool full = c->ntt_full_scale && log_n <= FULL_SCALE_MAX_BITS;
   if (rc == ZK_OK && !inverse) {
-    if (shift) rc = get_pow_tables(c, log_n, *shift, zkh::one(zkh::FR), &h, &l, st);
+    if (shift) {
+      rc = full ? get_scale_full(c, log_n, *shift, zkh::one(zkh::FR), &f, st)
+                : get_pow_tables(c, log_n, *shift, zkh::one(zkh::FR), &h, &l, st);
+    }
   } else if (rc == ZK_OK) {
     zkh::Fe ninv = zkh::inv(zkh::FR, zkh::from_u64(zkh::FR, n));
     if (shift) {
       zkh::Fe ginv = zkh::is_zero(*shift) ? zkh::zero() : zkh::inv(zkh::FR, *shift);
-      rc = get_pow_tables(c, log_n, ginv, ninv, &h, &l, st);
+      rc = full ? get_scale_full(c, log_n, ginv, ninv, &f, st) : get_pow_tables(c, log_n, ginv, ninv, &h, &l, st);
     } else {
       rc = get_pow_tables(c, 0, zkh::zero(), ninv, &h, &l, st);
     }

@@ -66,16 +66,18 @@ def test_ntt_in_place_and_pass_splits(zctx, log_n):
     zctx.set_tuning("ntt_max_pass_bits", 0)
 
 
-@pytest.mark.parametrize("inplace,stage", [(1, 1), (0, 1), (1, 0), (0, 0)])
-def test_ntt_pass_forms(zctx, inplace, stage):
+@pytest.mark.parametrize("inplace,stage,full", [(1, 1, 1), (0, 1, 1), (1, 0, 1), (0, 0, 0), (1, 1, 0)])
+def test_ntt_pass_forms(zctx, inplace, stage, full):
     # in-place or Stockham-order pass stages, per-stage or single twiddle
-    # tables: all four variants against the oracle, one- and multi-pass
-    # plans, odd pass sizes (a lone radix-2 stage), many-column tiles
+    # tables, coset scaling from full or two-level tables: all four
+    # variants against the oracle, one- and multi-pass plans, odd pass sizes
+    # (a lone radix-2 stage), many-column tiles
     import numpy as np
     from oracle import cref
     from pyzkp_amd.field import words_to_ints
     zctx.set_tuning("ntt_inplace", inplace)
     zctx.set_tuning("ntt_stage_twiddles", stage)
+    zctx.set_tuning("ntt_full_scale", full)
     try:
         for log_n, bits in ((5, 0), (11, 0), (13, 5), (17, 0), (17, 9)):
             n = 1 << log_n
@@ -96,6 +98,7 @@ def test_ntt_pass_forms(zctx, inplace, stage):
         zctx.set_tuning("ntt_max_pass_bits", 0)
         zctx.set_tuning("ntt_inplace", 1)
         zctx.set_tuning("ntt_stage_twiddles", 1)
+        zctx.set_tuning("ntt_full_scale", 1)
 
 
 def test_ntt_batch(zctx):
