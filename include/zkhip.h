@@ -310,8 +310,10 @@ int zk_pairing_check(const uint64_t* g1, const uint64_t* g2, uint32_t n, int* ok
  * "msm_segment" (pairs per accumulation thread), "msm_reduce_chunk"
  * (buckets per reduction thread of the chunked reduction),
  * "msm_reduce_split" (1, the default: single-launch MSMs reduce their
- * buckets by digit split; 0: chunked running sums).  No counterpart in the
- * reference. */
+ * buckets by digit split; 0: chunked running sums), "msm_sort_fused" (1,
+ * the default: bucket-sort segments that one workgroup holds are placed
+ * whole in one pass; 0: every segment through the chunked count / scan /
+ * place kernels).  No counterpart in the reference. */
 int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value);
 
 /* Per-phase device time from HIP events recorded on the launch stream

@@ -530,6 +530,9 @@ int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value) {
   } else if (k == "msm_reduce_split") {
     if (value != 0 && value != 1) return set_err(&ctx->c, ZK_ERR_INVALID, "msm_reduce_split must be 0 or 1");
     ctx->c.msm_reduce_split = (uint32_t)value;
+  } else if (k == "msm_sort_fused") {
+    if (value != 0 && value != 1) return set_err(&ctx->c, ZK_ERR_INVALID, "msm_sort_fused must be 0 or 1");
+    ctx->c.msm_sort_fused = (uint32_t)value;
   } else if (k == "msm_groups") {
     if (value < 0 || value > 16) return set_err(&ctx->c, ZK_ERR_INVALID, "msm_groups must be in [0, 16]");
     ctx->c.msm_groups = (uint32_t)value;

@@ -72,6 +72,7 @@ struct Ctx {
   uint32_t msm_reduce_chunk = 0;  // buckets per reduction thread (0: sized to fill the device)
   uint32_t msm_groups = 0;        // window groups pipelined over two streams (0: chosen per shape)
   uint32_t msm_reduce_split = 1;  // single-launch MSMs: digit-split bucket reduction (0: chunked running sums)
+  uint32_t msm_sort_fused = 1;    // bucket sort: segments one workgroup holds placed whole (0: all chunked)
   // second stream (default priority) and events for pipelined launches, created on first use
   hipStream_t aux_stream = nullptr;
   std::vector<hipEvent_t> events;

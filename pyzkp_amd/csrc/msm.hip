@@ -264,6 +264,8 @@ static constexpr int SCAN_THREADS = 256;
 static constexpr int SCAN_PER_THREAD = 8;
 static constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_PER_THREAD;
 
+// (A wave-shuffle form with two barriers measured slower in the bucket sort:
+// its six dependent lane permutes cost more than the barriers they save.)
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
   const uint32_t tid = threadIdx.x;
   sh[tid] = v;
@@ -322,7 +324,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void scan_add_kernel(uint32_t* out, c
 //    array is written, staged or read by the fine level.  KT / NCX: the
 //    staged coarse key's type and the key arrays' size; with at most 256
 //    coarse keys (every window up to 22 bits) u8 keys and 256-entry arrays
-//    keep the kernel at 45 KiB of LDS, three workgroups per CU.
+//    keep the kernel at 45 KiB of LDS, three workgroups per CU (1 024
+//    threads of 8 digits, two workgroups and 32 waves per CU, measured no
+//    faster).
 template <bool PK, typename KT, uint32_t NCX>
 __global__ __launch_bounds__(SORT_THREADS) void msm_sort_scatter_kernel(
     const uint32_t* __restrict__ digits, const MsmRow* __restrict__ rows, uint32_t S, uint32_t NC,
@@ -393,14 +397,17 @@ __global__ __launch_bounds__(SORT_THREADS) void msm_sort_scatter_kernel(
 }
 
 // d. fine level.  Segment g = (row r, coarse key k) holds the entries
-//    [s0, s1) of the coarse order.  It is cut into chunks of FINE_CHUNK
-//    entries (at least one chunk, so empty segments still write their row
-//    pointers); chunk j of g is one workgroup, which counts the fine keys
-//    in LDS into fhist[fc0(g) NF + f nch(g) + j] (columns in (segment, f)
-//    order, chunks contiguous).  One exclusive scan of fhist then gives the
-//    output offset of every (segment, f, chunk), and the bucket start of
-//    (r, bin = k 2^S + f) is the value at chunk 0.  A segment holding one
-//    huge bucket is thus spread over many workgroups.
+//    [s0, s1) of the coarse order, and the same range of the sorted order.
+//    A segment one workgroup can hold (packed entries) is placed whole by
+//    msm_fine_fused_kernel.  Any other segment is cut into chunks of
+//    FINE_CHUNK entries (at least one chunk, so empty segments still write
+//    their row pointers); chunk j of g is one workgroup, which counts the
+//    fine keys in LDS into fhist[fc0(g) NF + f nch(g) + j] (columns in
+//    (segment, f) order, chunks contiguous).  One exclusive scan of fhist
+//    then gives the offset of every (segment, f, chunk) among the chunked
+//    segments' entries; rebased on s0 it is the output position, and the
+//    bucket start of (r, bin = k 2^S + f) is the value at chunk 0.  A
+//    segment holding one huge bucket is thus spread over many workgroups.
 __device__ __forceinline__ void fine_segment(const uint32_t* __restrict__ hoff, const MsmRow* __restrict__ rows,
                                              uint32_t NC, uint32_t g, uint32_t& r, uint32_t& k, uint32_t& s0,
                                              uint32_t& s1) {
@@ -412,9 +419,12 @@ __device__ __forceinline__ void fine_segment(const uint32_t* __restrict__ hoff, 
   s1 = hoff[col + row.ntiles];  // the next column's start (the grand total after the last)
 }
 
+// Segments of at most FUSE_MAX entries are placed whole by one workgroup of
+// msm_fine_fused_kernel (fused_max = FUSE_MAX) and get no chunks here; with
+// fused_max = 0 every segment is chunked.
 __global__ __launch_bounds__(256) void msm_fine_chunks_kernel(const uint32_t* __restrict__ hoff,
                                                               const MsmRow* __restrict__ rows, uint32_t NC,
-                                                              uint32_t nseg, uint32_t fch,
+                                                              uint32_t nseg, uint32_t fch, uint32_t fused_max,
                                                               uint32_t* __restrict__ nch,
                                                               uint32_t* __restrict__ maxrun) {
   const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
@@ -422,6 +432,10 @@ __global__ __launch_bounds__(256) void msm_fine_chunks_kernel(const uint32_t* __
   if (g >= nseg) return;
   uint32_t r, k, s0, s1;
   fine_segment(hoff, rows, NC, g, r, k, s0, s1);
+  if (fused_max && s1 - s0 <= fused_max) {
+    nch[g] = 0;
+    return;
+  }
   const uint32_t m = (s1 - s0 + fch - 1) / fch;
   nch[g] = m ? m : 1u;
 }
@@ -453,6 +467,63 @@ __device__ __forceinline__ uint32_t fine_key(const uint32_t* __restrict__ items_
 template <bool PK>
 __device__ __forceinline__ uint32_t entry_value(uint32_t v, uint32_t S) {
   return PK ? (v & ~(((1u << S) - 1) << (31 - S))) : v;
+}
+
+// Fine level of one whole segment in one workgroup (packed entries): the
+// segment's entries are read once into registers, ranked per fine key in
+// LDS, staged in bucket order and written out as one contiguous run, and the
+// workgroup writes the CSR row pointers of the segment's buckets itself, so
+// no chunk histogram has to be written, scanned and read back.  At 2^22
+// points every coarse segment (about n / 256 entries) takes this path.
+static constexpr uint32_t FUSE_THREADS = 1024;
+static constexpr uint32_t FUSE_ITEMS = 17;
+static constexpr uint32_t FUSE_MAX = FUSE_THREADS * FUSE_ITEMS;  // 17 408 entries, 68 KiB staged
+static constexpr uint32_t FUSE_KEYS = FUSE_THREADS;             // one fine key per thread
+
+__global__ __launch_bounds__(FUSE_THREADS) void msm_fine_fused_kernel(
+    const uint32_t* __restrict__ items_v, const uint32_t* __restrict__ hoff, const MsmRow* __restrict__ rows,
+    uint32_t S, uint32_t NC, uint32_t nseg, uint32_t nbins, uint32_t B, uint32_t* __restrict__ row_ptr,
+    uint32_t* __restrict__ sorted) {
+  __shared__ uint32_t lcnt[FUSE_KEYS];
+  __shared__ uint32_t stage_v[FUSE_MAX];
+  __shared__ uint32_t scan_sh[FUSE_THREADS];
+  const uint32_t g = blockIdx.x;
+  uint32_t r, k, s0, s1;
+  fine_segment(hoff, rows, NC, g, r, k, s0, s1);
+  const uint32_t cnt = s1 - s0;
+  if (cnt > FUSE_MAX) return;  // placed by the chunked kernels
+  const uint32_t NF = 1u << S;
+  const uint32_t tid = threadIdx.x;
+  if (tid < NF) lcnt[tid] = 0;
+  __syncthreads();
+  uint32_t v[FUSE_ITEMS], rank[FUSE_ITEMS];
+#pragma unroll
+  for (uint32_t i = 0; i < FUSE_ITEMS; i++) {
+    const uint32_t q = i * FUSE_THREADS + tid;
+    if (q < cnt) {
+      v[i] = items_v[s0 + q];
+      rank[i] = atomicAdd(&lcnt[(v[i] >> (31 - S)) & (NF - 1)], 1u);
+    }
+  }
+  __syncthreads();
+  // bucket starts inside the segment: exclusive scan of the counts
+  uint32_t total;
+  const uint32_t c0 = (tid < NF) ? lcnt[tid] : 0u;
+  const uint32_t ex = block_exclusive_scan(c0, scan_sh, total);
+  if (tid < NF) {
+    lcnt[tid] = ex;
+    if ((k << S) + tid < nbins) row_ptr[(uint64_t)r * B + 1 + ((uint64_t)k << S) + tid] = s0 + ex;
+  }
+  if (k == 0 && tid == 0) row_ptr[(uint64_t)r * B] = s0;
+  if (g == nseg - 1 && tid == 0) row_ptr[(uint64_t)(nseg / NC) * B] = s1;  // the total
+  __syncthreads();
+#pragma unroll
+  for (uint32_t i = 0; i < FUSE_ITEMS; i++) {
+    const uint32_t q = i * FUSE_THREADS + tid;
+    if (q < cnt) stage_v[lcnt[(v[i] >> (31 - S)) & (NF - 1)] + rank[i]] = v[i];
+  }
+  __syncthreads();
+  for (uint32_t slot = tid; slot < cnt; slot += FUSE_THREADS) sorted[s0 + slot] = entry_value<true>(stage_v[slot], S);
 }
 
 template <bool PK>
@@ -493,7 +564,8 @@ __global__ __launch_bounds__(FINE_THREADS) void msm_fine_scatter_kernel(
   fine_segment(hoff, rows, NC, g, r, k, s0, s1);
   const uint32_t NF = 1u << S;
   const uint32_t* in = foff + (uint64_t)fc0[g] * NF + j;
-  for (uint32_t f = threadIdx.x; f < NF; f += FINE_THREADS) fpos[f] = in[(uint64_t)f * nc];
+  const uint32_t rebase = s0 - foff[(uint64_t)fc0[g] * NF];  // the scan counts chunked segments only
+  for (uint32_t f = threadIdx.x; f < NF; f += FINE_THREADS) fpos[f] = in[(uint64_t)f * nc] + rebase;
   __syncthreads();
   if (j == 0) {
     const uint64_t key0 = (uint64_t)r * B + 1 + ((uint64_t)k << S);
@@ -539,8 +611,9 @@ __global__ __launch_bounds__(FINE_STAGE_THREADS) void msm_fine_scatter_staged_ke
   const uint32_t NF = 1u << S;
   const uint32_t tid = threadIdx.x;
   const uint32_t* in = foff + (uint64_t)fc0[g] * NF + j;
+  const uint32_t rebase = s0 - foff[(uint64_t)fc0[g] * NF];  // the scan counts chunked segments only
   for (uint32_t f = tid; f < NF; f += FINE_STAGE_THREADS) {
-    gpos[f] = in[(uint64_t)f * nc];
+    gpos[f] = in[(uint64_t)f * nc] + rebase;
     lcnt[f] = 0;
   }
   __syncthreads();
@@ -1121,9 +1194,10 @@ static void scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t len, uint
 
 // sort -> accumulate -> fix-up -> per-row reduction -> export (R = 2^256 XYZZ
 // per row in b.rexp).  No allocation, no host synchronisation.
-// coarse scatter and fine placement (after the coarse histogram scan)
+// coarse scatter and fine placement (after the coarse histogram scan);
+// fuse: segments of at most FUSE_MAX entries by msm_fine_fused_kernel
 template <bool PK>
-static void sort_placement(const MsmShape& sh, const MsmBuffers& b, hipStream_t st) {
+static void sort_placement(const MsmShape& sh, const MsmBuffers& b, hipStream_t st, bool fuse) {
   const dim3 g(sh.max_tiles, sh.nrows);
   if (sh.max_tiles > 0) {
     if (sh.NC <= 256)
@@ -1134,8 +1208,12 @@ static void sort_placement(const MsmShape& sh, const MsmBuffers& b, hipStream_t 
           b.digits, b.rows, sh.S, sh.NC, b.hist, b.items_v, b.items_f);
   }
   const uint32_t nseg = sh.nrows * sh.NC, NF = 1u << sh.S;
-  msm_fine_chunks_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(b.hist, b.rows, sh.NC, nseg, sh.fch, b.fc0,
-                                                              b.maxrun);
+  fuse = fuse && PK && NF <= FUSE_KEYS;
+  if (fuse)
+    msm_fine_fused_kernel<<<nseg, FUSE_THREADS, 0, st>>>(b.items_v, b.hist, b.rows, sh.S, sh.NC, nseg, sh.nbins,
+                                                         sh.B, b.row_ptr, b.sorted);
+  msm_fine_chunks_kernel<<<grid_for(nseg, 256), 256, 0, st>>>(b.hist, b.rows, sh.NC, nseg, sh.fch,
+                                                              fuse ? FUSE_MAX : 0u, b.fc0, b.maxrun);
   scan_exclusive(b.fc0, b.fc0, nseg, b.sums, st);  // fc0[nseg] = chunks in use (<= sh.fchunks)
   msm_fine_count_kernel<PK><<<sh.fchunks, FINE_THREADS, NF * 4, st>>>(b.items_v, b.items_f, b.hist, b.rows, sh.S,
                                                                       sh.NC, nseg, sh.fch, b.fc0, b.fhist);
@@ -1165,9 +1243,9 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
     msm_sort_count_kernel<<<g, SORT_THREADS, 0, st>>>(b.digits, b.rows, sh.S, sh.NC, b.hist);
   scan_exclusive(b.hist, b.hist, sh.hist_len, b.sums, st);  // in place; hist[hist_len] = total
   if (sh.packed)
-    sort_placement<true>(sh, b, st);
+    sort_placement<true>(sh, b, st, c->msm_sort_fused != 0);
   else
-    sort_placement<false>(sh, b, st);
+    sort_placement<false>(sh, b, st, false);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_sort", st, ev);
   if (sorted_ev) ZK_HIP(c, hipEventRecord(sorted_ev, st));
@@ -1244,9 +1322,10 @@ static void export_jacobian(const xyzz& acc, uint64_t out[12]) {
 // and reduction of one group can run beside the accumulation of its
 // neighbour.  On MI355X at 2^22 this measured no faster than one group
 // (prepared bases, digit-split reduction: 6.01 ms for one or two groups,
-// 6.09 for three, 6.53 for five): the accumulation's three waves per SIMD
-// leave too few registers for a sort wave beside them, so the streams
-// interleave instead of overlapping.  The default is one group.
+// 6.09 for three, 6.53 for five).  Registers are not the reason: with the
+// accumulation held to 144 VGPRs, so that a sort wave fits beside its three
+// waves per SIMD, two groups were still no faster than one (6.08 against
+// 6.06 ms) and the cap cost the accumulation 1%.  The default is one group.
 // One launch sequence over all W windows (the default shape).  `arena`
 // null: the context's growable scratch; else the caller's arena of
 // `arena_bytes`, which must hold the layout (nothing is allocated then).

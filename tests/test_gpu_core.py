@@ -230,6 +230,31 @@ def test_msm_window_groups(zctx):
         mb.free()
 
 
+def test_msm_sort_fused_and_chunked(zctx):
+    # bucket-sort segments placed whole by one workgroup next to segments
+    # too large for one (40% of the scalars share one value, so that value's
+    # bucket fills its segment in every window), and windows whose fine keys
+    # exceed the fused kernel's table (c = 20: 2^11 fine keys); fused on and
+    # off, both against the closed form
+    from pyzkp_amd import workload
+    from pyzkp_amd.field import words_to_ints
+    n = 1 << 18
+    kw = workload.fr_uniform(0x71, n)
+    sw = workload.fr_uniform(0x72, n)
+    u = workload.uniform01(0x73, n)
+    sw[u < 0.4] = sw[0]
+    bases = workload.device_bases(zctx, kw)
+    sc = workload.device_fr_mont(zctx, sw)
+    expect = M.msm_closed_form(words_to_ints(kw), words_to_ints(sw))
+    try:
+        for fused in (1, 0):
+            zctx.set_tuning("msm_sort_fused", fused)
+            for c in (5, 10, 16, 17, 19, 20):
+                assert msm_affine(zctx, bases, sc, n, c) == expect, (fused, c)
+    finally:
+        zctx.set_tuning("msm_sort_fused", 1)
+
+
 def test_msm_wide_windows(zctx):
     # c >= 20: up to 2^13 fine keys per coarse key in the bucket sort
     n = 100

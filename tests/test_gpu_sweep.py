@@ -3,9 +3,10 @@ and the C oracle.
 
 MSM: random sizes up to 2^17 and window sizes 2..24, with scalars that mix
 uniform values, zeros, r - 1 and small values, and bases with repeats, run
-through the digit-split and the chunked bucket reductions, one or several
-window groups, and plain and prepared base sets; every result equals
-[sum s_i k_i] G for bases [k_i] G.
+through the digit-split and the chunked bucket reductions, the fused and the
+chunked fine level of the bucket sort, one or several window groups, and
+plain and prepared base sets; every result equals [sum s_i k_i] G for bases
+[k_i] G.
 
 NTT: random sizes 2^1..2^18, input lengths (zero-padded and truncated
 forward transforms), shifts (none, the generator, a random one, zero) and
@@ -38,6 +39,7 @@ def _aff(r):
 
 def test_msm_shape_sweep(zctx):
     rng = random.Random(0x51)
+    frng = random.Random(0x53)  # the sort form, drawn apart so the cases above stay as they were
     try:
         for case in range(30):
             n = rng.choice([1, 2, 7, 100, 1000, 4097]) if case < 6 else rng.randrange(1, 1 << 17)
@@ -45,7 +47,9 @@ def test_msm_shape_sweep(zctx):
             split = rng.randrange(2)
             groups = rng.choice([0, 0, 2, 3])
             prepared = rng.randrange(2)
-            print(f"MSM sweep case {case}: n={n} c={c} split={split} groups={groups} prepared={prepared}", flush=True)
+            fused = frng.randrange(2)
+            print(f"MSM sweep case {case}: n={n} c={c} split={split} groups={groups} prepared={prepared} "
+                  f"fused={fused}", flush=True)
             kw = workload.repeat_some(0x5200 + case, workload.fr_uniform(0x5100 + case, n), 0.1)
             sw = workload.fr_uniform(0x5300 + case, n)
             u = workload.uniform01(0x5400 + case, n)
@@ -62,18 +66,20 @@ def test_msm_shape_sweep(zctx):
             sc = workload.device_fr_mont(zctx, _words(sv))
             zctx.set_tuning("msm_reduce_split", split)
             zctx.set_tuning("msm_groups", groups)
+            zctx.set_tuning("msm_sort_fused", fused)
             if prepared:
                 mb = zctx.msm_bases_create(bases.ptr, n)
                 got = _aff(zctx.msm_g1_prepared(mb, sc.ptr, n, c))
                 mb.free()
             else:
                 got = _aff(zctx.msm_g1(bases.ptr, sc.ptr, n, c))
-            assert got == expect, (case, n, c, split, groups, prepared)
+            assert got == expect, (case, n, c, split, groups, prepared, fused)
             bases.free()
             sc.free()
     finally:
         zctx.set_tuning("msm_reduce_split", 1)
         zctx.set_tuning("msm_groups", 0)
+        zctx.set_tuning("msm_sort_fused", 1)
 
 
 def test_ntt_shape_sweep(zctx):
