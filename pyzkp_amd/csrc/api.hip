@@ -146,8 +146,15 @@ int zk_init(int ordinal, zk_ctx** out) {
     g_init_err = std::string("hipSetDevice failed: ") + hipGetErrorString(e);
     return ZK_ERR_RUNTIME;
   }
+  int cus = 0;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ordinal);
+  if (e != hipSuccess || cus <= 0) {
+    g_init_err = std::string("hipDeviceGetAttribute(compute units) failed: ") + hipGetErrorString(e);
+    return ZK_ERR_RUNTIME;
+  }
   zk_ctx* ctx = new zk_ctx();
   ctx->c.device = ordinal;
+  ctx->c.num_cus = (uint32_t)cus;
   e = hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     g_init_err = std::string("hipStreamCreate failed: ") + hipGetErrorString(e);

@@ -45,6 +45,7 @@ struct CachedTable {
 
 struct Ctx {
   int device = 0;
+  uint32_t num_cus = 0;  // compute units of the device
   hipStream_t stream = nullptr;
   std::string err;
   // NTT twiddle tables omega^e, e <= n/2 (both directions), keyed by (log_n, omega Montgomery words)

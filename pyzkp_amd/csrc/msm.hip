@@ -855,36 +855,34 @@ __global__ __launch_bounds__(RED_GROUP) void msm_reduce_groups_kernel(const pt9*
 //     folded SPLIT_FOLD at a time, G_g = sum_(i < SPLIT_FOLD) 2^i E_(g SPLIT_FOLD + i),
 //     and the host adds G_g at bit c r + SPLIT_FOLD g of its Horner combination.
 static constexpr uint32_t SPLIT_THREADS = 256;
-static constexpr uint32_t SPLIT_LANES = 16;                          // threads per H or L sum
-static constexpr uint32_t SPLIT_PER = SPLIT_THREADS / SPLIT_LANES;  // sums per workgroup
+static constexpr uint32_t SPLIT_WG_PER_CU = 3;  // sums workgroups resident per CU (164 VGPRs: 3 waves per SIMD)
 static constexpr uint32_t SPLIT_FOLD = 4;                           // planes per folded term
 static constexpr uint32_t SPLIT_PLANE_LANES = 32;                   // threads per plane sum
 
-// blocks [0, nblk_h): H sums, SPLIT_LANES consecutive threads per sum reading
-// consecutive buckets; blocks from nblk_h on: L sums, threads o, o + SPLIT_PER,
-// ... per sum, so SPLIT_PER consecutive threads read consecutive buckets
+// blocks [0, nblk_h): H sums, from nblk_h on: L sums.  The LANES partials
+// of one sum sit PER = SPLIT_THREADS / LANES threads apart, so the tree below
+// retires whole waves (step s keeps the threads below s PER busy) instead of
+// running every level in every wave.  LANES = 32 halves each thread's chain
+// of dependent additions and doubles the workgroups; the launch takes it
+// when the doubled grid still fits one round of resident workgroups.
+template <uint32_t LANES>
 __global__ __launch_bounds__(SPLIT_THREADS) void msm_split_sums_kernel(const pt9* __restrict__ buckets,
                                                                        const uint32_t* __restrict__ row_ptr,
                                                                        uint32_t B, uint32_t ml, uint32_t mh,
                                                                        uint32_t nblk_h, pt9* __restrict__ hsum,
                                                                        pt9* __restrict__ lsum) {
+  constexpr uint32_t PER = SPLIT_THREADS / LANES;
   __shared__ xyzz9 sh[SPLIT_THREADS];
   const uint32_t r = blockIdx.y, tid = threadIdx.x;
   const uint32_t M = 1u << (ml + mh);
   const bool hrow = blockIdx.x < nblk_h;
-  uint32_t o, p;
-  if (hrow) {
-    o = blockIdx.x * SPLIT_PER + tid / SPLIT_LANES;
-    p = tid % SPLIT_LANES;
-  } else {
-    o = (blockIdx.x - nblk_h) * SPLIT_PER + tid % SPLIT_PER;
-    p = tid / SPLIT_PER;
-  }
+  const uint32_t o = (hrow ? blockIdx.x : blockIdx.x - nblk_h) * PER + tid % PER;
+  const uint32_t p = tid / PER;
   const uint32_t nout = hrow ? (1u << mh) + 1 : (1u << ml);
   xyzz9 acc = xyzz9_inf();
   if (o < nout) {
     const uint32_t cnt = hrow ? (o < (1u << mh) ? (1u << ml) : 1u) : (1u << mh);
-    for (uint32_t i = p; i < cnt; i += SPLIT_LANES) {
+    for (uint32_t i = p; i < cnt; i += LANES) {
       const uint32_t b = hrow ? (o << ml) + i : (i << ml) + o;  // H_(2^mh): o << ml = M
       if (b == 0 || b > M) continue;
       const uint32_t key = r * B + b;
@@ -893,9 +891,8 @@ __global__ __launch_bounds__(SPLIT_THREADS) void msm_split_sums_kernel(const pt9
   }
   sh[tid] = acc;
   __syncthreads();
-  const uint32_t pstride = hrow ? 1u : SPLIT_PER;  // distance between a sum's partials
-  for (uint32_t step = SPLIT_LANES >> 1; step > 0; step >>= 1) {
-    if (p < step) sh[tid] = xyzz9_add(sh[tid], sh[tid + step * pstride]);
+  for (uint32_t step = LANES >> 1; step > 0; step >>= 1) {
+    if (p < step) sh[tid] = xyzz9_add(sh[tid], sh[tid + step * PER]);
     __syncthreads();
   }
   if (p == 0 && o < nout) store_xyzz9((hrow ? hsum : lsum) + (uint64_t)r * nout + o, sh[tid]);
@@ -1267,10 +1264,17 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
 
   phase_begin(c, "msm_reduce", st, &ev);
   if (split) {
-    const uint32_t bh = ((1u << sh.mh) + 1 + SPLIT_PER - 1) / SPLIT_PER;
-    const uint32_t bl = ((1u << sh.ml) + SPLIT_PER - 1) / SPLIT_PER;
-    msm_split_sums_kernel<<<dim3(bh + bl, sh.nrows), SPLIT_THREADS, 0, st>>>(b.buckets, b.row_ptr, sh.B, sh.ml,
-                                                                            sh.mh, bh, b.hsum, b.lsum);
+    // 32 partials per sum when that grid fits one round of resident workgroups
+    // (with 5% slack), else 16
+    const uint32_t bh32 = ((1u << sh.mh) + 1 + 7) / 8, bl32 = ((1u << sh.ml) + 7) / 8;
+    if ((uint64_t)sh.nrows * (bh32 + bl32) * 20 <= (uint64_t)c->num_cus * SPLIT_WG_PER_CU * 21) {
+      msm_split_sums_kernel<32><<<dim3(bh32 + bl32, sh.nrows), SPLIT_THREADS, 0, st>>>(
+          b.buckets, b.row_ptr, sh.B, sh.ml, sh.mh, bh32, b.hsum, b.lsum);
+    } else {
+      const uint32_t bh = ((1u << sh.mh) + 1 + 15) / 16, bl = ((1u << sh.ml) + 15) / 16;
+      msm_split_sums_kernel<16><<<dim3(bh + bl, sh.nrows), SPLIT_THREADS, 0, st>>>(b.buckets, b.row_ptr, sh.B,
+                                                                                  sh.ml, sh.mh, bh, b.hsum, b.lsum);
+    }
     msm_split_planes_kernel<<<dim3(sh.ng, sh.nrows), SPLIT_FOLD * SPLIT_PLANE_LANES, 0, st>>>(
         b.hsum, b.lsum, sh.ml, sh.mh, sh.ng, b.gsum);
     msm_wsum_export_kernel<<<grid_for((uint64_t)sh.nrows * sh.ng, 64), 64, 0, st>>>(b.gsum, sh.nrows * sh.ng,
