@@ -53,6 +53,21 @@ int ensure_scratch(Ctx* c, int slot, size_t bytes, void** out) {
   return ZK_OK;
 }
 
+int ensure_host_out(Ctx* c, size_t bytes, void** out) {
+  if (c->host_out_bytes < bytes) {
+    if (c->host_out) {
+      ZK_HIP(c, hipHostFree(c->host_out));
+      c->host_out = nullptr;
+      c->host_out_bytes = 0;
+    }
+    const size_t want = bytes < 65536 ? 65536 : bytes;
+    ZK_HIP(c, hipHostMalloc(&c->host_out, want, hipHostMallocDefault));
+    c->host_out_bytes = want;
+  }
+  *out = c->host_out;
+  return ZK_OK;
+}
+
 int upload_table(Ctx* c, const std::vector<zkh::Fe>& host, DevBuf* out, hipStream_t st) {
   out->bytes = host.size() * 32;
   ZK_HIP(c, hipMalloc(&out->ptr, out->bytes));
@@ -185,6 +200,7 @@ int zk_destroy(zk_ctx* ctx) {
   for (auto& b : ctx->c.scratch)
     if (b.ptr) hipFree(b.ptr);
   for (hipEvent_t e : ctx->c.events) hipEventDestroy(e);
+  if (ctx->c.host_out) hipHostFree(ctx->c.host_out);
   if (ctx->c.aux_stream) hipStreamDestroy(ctx->c.aux_stream);
   hipStreamDestroy(ctx->c.stream);
   delete ctx;

@@ -76,6 +76,9 @@ struct Ctx {
   uint32_t msm_sort_fused = 1;    // bucket sort: segments one workgroup holds placed whole (0: all chunked)
   // second stream (default priority) and events for pipelined launches, created on first use
   hipStream_t aux_stream = nullptr;
+  // pinned host staging for small results read back within one call (MSM window terms)
+  void* host_out = nullptr;
+  size_t host_out_bytes = 0;
   std::vector<hipEvent_t> events;
   bool profiling = false;
   std::map<std::string, PhaseTimer> phases;
@@ -92,6 +95,8 @@ int hip_check(Ctx* c, hipError_t e, const char* what);
 
 // device memory helpers (context-owned) -------------------------------------
 int ensure_scratch(Ctx* c, int slot, size_t bytes, void** out);
+// c->host_out grown to at least `bytes` (pinned; only used synchronously inside one call)
+int ensure_host_out(Ctx* c, size_t bytes, void** out);
 // the context's second stream and at least `count` timing-free events
 int ensure_aux(Ctx* c, size_t count);
 int upload_table(Ctx* c, const std::vector<zkh::Fe>& host, DevBuf* out, hipStream_t st);

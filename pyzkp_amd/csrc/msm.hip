@@ -726,10 +726,14 @@ __global__ __launch_bounds__(256) void msm_accumulate_kernel(
 
 // 5. fix-up of buckets that span several segments.  The head partials of
 //    bucket k sit in consecutive segments first..first+len-1 with
-//    first = row_ptr[k] / L + 1.  Round r folds FIXUP_FAN partials spaced
-//    FIXUP_FAN^r apart into the one at offset 0 mod FIXUP_FAN^(r+1); after
-//    enough rounds the run's sum is at `first` and is added to buckets[k]
-//    (which the owning segment wrote).  Depth log_FAN(len) instead of len.
+//    first = row_ptr[k] / L + 1.  Round r folds F partials spaced F^r apart
+//    into the one at offset 0 mod F^(r+1); after the launched rounds the
+//    run's sum is at `first` and is added to buckets[k] (which the owning
+//    segment wrote).  F is chosen on the device from the longest run the
+//    accumulation recorded (fixup_fan), so a chain of rounds (F - 1)
+//    dependent additions instead of len: a few huge buckets (the top window
+//    when c does not divide the scalar bits, equal scalars) fold in a fraction
+//    of the time a fixed fan-in of FIXUP_FAN took.
 static constexpr uint32_t FIXUP_FAN = 32;
 
 __device__ __forceinline__ bool head_run(const uint32_t* __restrict__ row_ptr, uint32_t k, uint32_t L,
@@ -740,13 +744,38 @@ __device__ __forceinline__ bool head_run(const uint32_t* __restrict__ row_ptr, u
   return s >= first && s < first + len;
 }
 
+// The fan-in of `rounds` launched rounds: the smallest F >= 2 with F^rounds
+// >= mr (the longest run), so the rounds fold every run and the chain of
+// dependent additions, rounds (F - 1), is as short as they allow.  The host
+// launches ceil(log_FIXUP_FAN(segments)) rounds, so F <= FIXUP_FAN.
+__device__ __forceinline__ uint64_t fixup_fan(uint32_t mr, uint32_t rounds) {
+  if (rounds >= 32 || mr <= (1u << rounds)) return 2;  // uniform scalars: runs of two heads at most
+  auto covers = [&](uint64_t f) {
+    uint64_t p = 1;
+    for (uint32_t i = 0; i < rounds; i++) {
+      p *= f;
+      if (p >= mr) return true;
+    }
+    return false;
+  };
+  uint64_t lo = 2, hi = FIXUP_FAN;  // covers(FIXUP_FAN) holds: mr < segments <= FIXUP_FAN^rounds
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (covers(mid)) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
 __global__ __launch_bounds__(256) void msm_fixup_round_kernel(const uint32_t* __restrict__ row_ptr,
-                                                              uint32_t nkeys, uint32_t L,
-                                                              uint64_t stride,
+                                                              uint32_t nkeys, uint32_t L, uint32_t round,
+                                                              uint32_t rounds,
                                                               const uint32_t* __restrict__ head_key,
                                                               pt9* __restrict__ head,
                                                               const uint32_t* __restrict__ maxrun) {
-  if (stride >= *maxrun) return;  // no run of heads is longer than the stride: nothing to fold
+  const uint32_t mr = *maxrun;
+  // F >= 2, so the stride F^round is at least 2^round: no run longer than
+  // that means nothing to fold (the common case, checked before anything else)
+  if (mr <= (1u << round)) return;
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t M = row_ptr[nkeys];
   const uint64_t S = (M + L - 1) / L;
@@ -756,9 +785,12 @@ __global__ __launch_bounds__(256) void msm_fixup_round_kernel(const uint32_t* __
   uint64_t first, len;
   if (!head_run(row_ptr, k, L, s, first, len)) return;
   const uint64_t off = s - first;
-  if (off % (stride * FIXUP_FAN) != 0 || off + stride >= len) return;
+  const uint64_t F = fixup_fan(mr, rounds);
+  uint64_t stride = 1;
+  for (uint32_t i = 0; i < round; i++) stride *= F;
+  if (off % (stride * F) != 0 || off + stride >= len) return;
   xyzz9 acc = load_xyzz9(head + s);
-  for (uint32_t j = 1; j < FIXUP_FAN; j++) {
+  for (uint32_t j = 1; j < F; j++) {
     const uint64_t t = off + (uint64_t)j * stride;
     if (t >= len) break;
     acc = xyzz9_add(acc, load_xyzz9(head + first + t));
@@ -1254,8 +1286,10 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
   phase_end(c, "msm_accumulate", st, ev);
 
   phase_begin(c, "msm_fixup", st, &ev);
-  for (uint64_t stride = 1; stride < sh.Smax; stride *= FIXUP_FAN)
-    msm_fixup_round_kernel<<<grid_for(sh.Smax, 256), 256, 0, st>>>(b.row_ptr, sh.nkeys, sh.L, stride, b.hkey,
+  uint32_t rounds = 0;  // ceil(log_FIXUP_FAN(Smax)): enough for a run of every segment
+  for (uint64_t span = 1; span < sh.Smax; span *= FIXUP_FAN) rounds++;
+  for (uint32_t r = 0; r < rounds; r++)
+    msm_fixup_round_kernel<<<grid_for(sh.Smax, 256), 256, 0, st>>>(b.row_ptr, sh.nkeys, sh.L, r, rounds, b.hkey,
                                                                   b.head, b.maxrun);
   msm_fixup_final_kernel<<<grid_for(sh.Smax, 256), 256, 0, st>>>(b.row_ptr, sh.nkeys, sh.L, b.buckets, b.hkey,
                                                                 b.head);
@@ -1386,10 +1420,22 @@ static int msm_g1_single(Ctx* c, const fe* bases, const fe* bases9_ready, const 
   rc = run_rows(c, sh, b, bases9, st, nullptr, true, split);
   if (rc) return rc;
   if (split) {
-    // term g of window w sits at bit c w + SPLIT_FOLD g; hand them over top first
+    // term g of window w sits at bit c w + SPLIT_FOLD g; hand them over top first.
+    // Read back through the context's pinned buffer (a direct copy), except
+    // on a caller's arena, where nothing may be allocated.
     const uint32_t T = W * sh.ng;
-    std::vector<xyzz> ts(T);
-    ZK_HIP(c, hipMemcpyAsync(ts.data(), b.gexp, (size_t)T * sizeof(xyzz), hipMemcpyDeviceToHost, st));
+    std::vector<xyzz> tv;
+    xyzz* ts;
+    if (arena) {
+      tv.resize(T);
+      ts = tv.data();
+    } else {
+      void* hp;
+      rc = ensure_host_out(c, (size_t)T * sizeof(xyzz), &hp);
+      if (rc) return rc;
+      ts = (xyzz*)hp;
+    }
+    ZK_HIP(c, hipMemcpyAsync(ts, b.gexp, (size_t)T * sizeof(xyzz), hipMemcpyDeviceToHost, st));
     ZK_HIP(c, hipStreamSynchronize(st));
     std::vector<xyzz> desc(T);
     std::vector<uint32_t> pos(T);
@@ -1577,9 +1623,12 @@ static int msm_g1_grouped(Ctx* c, const fe* bases, const fe* bases9_ready, const
   if (split) {
     // group g's term (row r, fold j) sits at bit c (wb[g] + r) + SPLIT_FOLD j
     const uint32_t ng = sh[0].ng;
-    std::vector<xyzz> ts((size_t)W * ng);
+    void* hp;
+    rc = ensure_host_out(c, (size_t)W * ng * sizeof(xyzz), &hp);
+    if (rc) return rc;
+    xyzz* ts = (xyzz*)hp;  // the context's pinned read-back buffer
     for (uint32_t g = 0; g < G; g++)
-      ZK_HIP(c, hipMemcpyAsync(ts.data() + (size_t)wb[g] * ng, b[g].gexp,
+      ZK_HIP(c, hipMemcpyAsync(ts + (size_t)wb[g] * ng, b[g].gexp,
                                (size_t)(wb[g + 1] - wb[g]) * ng * sizeof(xyzz), hipMemcpyDeviceToHost, st));
     ZK_HIP(c, hipStreamSynchronize(st));
     const uint32_t T = W * ng;
