@@ -307,7 +307,8 @@ int zk_pairing_check(const uint64_t* g1, const uint64_t* g2, uint32_t n, int* ok
  * 2^26; 0: two-level power tables),
  * "ntt_table_cap_mb" (bytes of cached NTT tables a context keeps before
  * freeing the least recently used unpinned ones; 0 = no cap, default 16384),
- * "msm_segment" (pairs per accumulation thread), "msm_reduce_chunk"
+ * "msm_segment" (pairs per accumulation thread; 0, the default: chosen
+ * per launch from its size, between 24 and 80), "msm_reduce_chunk"
  * (buckets per reduction thread of the chunked reduction),
  * "msm_reduce_split" (1, the default: single-launch MSMs reduce their
  * buckets by digit split; 0: chunked running sums), "msm_sort_fused" (1,

@@ -545,7 +545,7 @@ int zk_set_tuning(zk_ctx* ctx, const char* key, int64_t value) {
     if (value != 0 && value != 1) return set_err(&ctx->c, ZK_ERR_INVALID, "ntt_full_scale must be 0 or 1");
     ctx->c.ntt_full_scale = (int)value;
   } else if (k == "msm_segment") {
-    if (value < 1 || value > 4096) return set_err(&ctx->c, ZK_ERR_INVALID, "msm_segment must be in [1, 4096]");
+    if (value < 0 || value > 4096) return set_err(&ctx->c, ZK_ERR_INVALID, "msm_segment must be in [0, 4096]");
     ctx->c.msm_segment = (uint32_t)value;
   } else if (k == "msm_reduce_chunk") {
     if (value < 0 || value > 65536) return set_err(&ctx->c, ZK_ERR_INVALID, "msm_reduce_chunk must be in [0, 65536]");

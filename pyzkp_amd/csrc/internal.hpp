@@ -69,7 +69,7 @@ struct Ctx {
   int ntt_inplace = 1;            // in-place pass stages (bit-reversed tile load, one barrier per step); 0: Stockham order
   int ntt_stage_twiddles = 1;     // per-stage twiddle tables (adjacent columns read adjacent entries); 0: one table of w^e
   int ntt_full_scale = 1;         // coset scaling from a full table per (size, shift) up to 2^26; 0: two-level tables
-  uint32_t msm_segment = 80;      // sorted (bucket, point) pairs per accumulation thread
+  uint32_t msm_segment = 0;       // sorted (bucket, point) pairs per accumulation thread (0: by launch size)
   uint32_t msm_reduce_chunk = 0;  // buckets per reduction thread (0: sized to fill the device)
   uint32_t msm_groups = 0;        // window groups pipelined over two streams (0: chosen per shape)
   uint32_t msm_reduce_split = 1;  // single-launch MSMs: digit-split bucket reduction (0: chunked running sums)

@@ -1099,7 +1099,33 @@ struct MsmBuffers {
   MsmRow* rows;
 };
 
-static int make_shape(Ctx* c, int cb, uint32_t nrows, const std::vector<MsmRow>& rows, MsmShape& sh) {
+// Accumulation segment length when the msm_segment knob is 0: the launch's
+// waves run in rounds of num_cus x 4 SIMDs x ACC_WAVES_PER_SIMD (the
+// accumulation's 145-147 VGPRs allow three per SIMD), each round about as
+// long as its segments, so L in [SEG_MIN, SEG_MAX] minimises rounds x L
+// (the longer L on ties: fewer head partials to fold).  Measured against a
+// fixed 80 (one process): 2^16 0.79 vs 1.11 ms, 2^18 0.89 vs 1.11, 2^20
+// 1.86 vs 2.11, 2^21 3.27 vs 3.38; 2^22 keeps 80.
+static constexpr uint32_t SEG_MIN = 24, SEG_MAX = 80, ACC_WAVES_PER_SIMD = 3;
+static uint32_t auto_segment(const Ctx* c, uint64_t pairs) {
+  const uint64_t cap = (uint64_t)(c->num_cus ? c->num_cus : 1) * 4 * ACC_WAVES_PER_SIMD;
+  uint32_t best = SEG_MAX;
+  uint64_t best_cost = UINT64_MAX;
+  for (uint32_t L = SEG_MAX; L >= SEG_MIN; L--) {
+    const uint64_t waves = (pairs + 64ull * L - 1) / (64ull * L);
+    const uint64_t cost = ((waves + cap - 1) / cap) * L;
+    if (cost < best_cost) {
+      best_cost = cost;
+      best = L;
+    }
+  }
+  return best;
+}
+
+// fixed_segment: SEG_MAX when the knob is 0 (a caller's fixed-size arena,
+// whose capacity must not depend on the launch size)
+static int make_shape(Ctx* c, int cb, uint32_t nrows, const std::vector<MsmRow>& rows, MsmShape& sh,
+                      bool fixed_segment = false) {
   if (cb < 2 || cb > 24) return set_err(c, ZK_ERR_INVALID, "window_bits must be in [2, 24]");
   sh.c = cb;
   sh.W = (255 + cb - 1) / cb;
@@ -1144,7 +1170,7 @@ static int make_shape(Ctx* c, int cb, uint32_t nrows, const std::vector<MsmRow>&
     if (fc * (1ull << sh.S) >= (1ull << 32)) return set_err(c, ZK_ERR_INVALID, "MSM too large for the bucket sort");
     sh.fchunks = (uint32_t)fc;
   }
-  sh.L = c->msm_segment;
+  sh.L = c->msm_segment ? c->msm_segment : (fixed_segment ? SEG_MAX : auto_segment(c, sh.pairs));
   sh.Smax = (sh.pairs + sh.L - 1) / sh.L;
   if (sh.Smax == 0) sh.Smax = 1;
   uint32_t K = c->msm_reduce_chunk;
@@ -1385,7 +1411,7 @@ static int msm_g1_single(Ctx* c, const fe* bases, const fe* bases9_ready, const 
     r.boff = 0;
   }
   MsmShape sh;
-  int rc = make_shape(c, cb, W, rows, sh);
+  int rc = make_shape(c, cb, W, rows, sh, arena != nullptr);
   if (rc) return rc;
   const size_t o_bases = align256((size_t)W * n * 4);
   const size_t o_rest = bases9_ready ? o_bases : align256(o_bases + (size_t)n * 64);

@@ -71,7 +71,7 @@ class Tuner:
 
     # ---- plain MSM: accumulation segment length
     def msm_segment(self, bases: int, scalars: int, n: int, c: int,
-                    cands: Iterable[int] = (32, 48, 64, 80, 96, 128)) -> int:
+                    cands: Iterable[int] = (0, 32, 48, 64, 80, 96, 128)) -> int:
         ctx = self.ctx
 
         def run(seg):
