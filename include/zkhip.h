@@ -39,6 +39,7 @@ typedef struct zk_ctx zk_ctx;
 typedef struct zk_fixed_bases zk_fixed_bases;
 typedef struct zk_batch zk_batch;
 typedef struct zk_msm_bases zk_msm_bases;
+typedef struct zk_msm_g2_bases zk_msm_g2_bases;
 typedef struct zk_g2_bases zk_g2_bases;
 
 /* One NTT job of a batch: `batch` transforms of size 2^log_n laid out back
@@ -143,6 +144,20 @@ int zk_msm_g1(zk_ctx* ctx, const void* bases, const void* scalars, uint64_t n, i
  * bases of the set (n <= the set's size). */
 int zk_msm_bases_create(zk_ctx* ctx, const void* bases, uint64_t n, zk_msm_bases** out, void* stream);
 int zk_msm_bases_destroy(zk_ctx* ctx, zk_msm_bases* b);
+
+/* The G2 MSM on the device (replaces msm_pippenger_g2,
+ * crypto/msm/pippenger.py:67-121, the reference's CPU MSM_G2 kernel at
+ * runtime/kernels/cpu/kernels.py:338-348): zk_msm_g2_bases_create uploads a
+ * G2 base set once (host words as zk_g2_msm_host takes them: affine
+ * canonical plain (x0, x1, y0, y1), 16 words a point, zeros = infinity) and
+ * converts it on the device; zk_msm_g2 sums scalars[i] * base[i] over the
+ * first n bases with device Fr scalars in Montgomery form (as zk_msm_g1)
+ * and writes the affine canonical result as zk_g2_msm_host does (16 words,
+ * zeros for the identity).  window_bits 0 picks the window as for G1. */
+int zk_msm_g2_bases_create(zk_ctx* ctx, const uint64_t* bases, uint64_t n, zk_msm_g2_bases** out, void* stream);
+int zk_msm_g2_bases_destroy(zk_ctx* ctx, zk_msm_g2_bases* b);
+int zk_msm_g2(zk_ctx* ctx, const zk_msm_g2_bases* b, const void* scalars, uint64_t n, int window_bits,
+              uint64_t out[16], void* stream);
 int zk_msm_g1_prepared(zk_ctx* ctx, const zk_msm_bases* b, const void* scalars, uint64_t n, int window_bits,
                        uint64_t out_jac_mont[12], void* stream);
 

@@ -52,6 +52,9 @@ SIGNATURES = {
     "zk_msm_bases_create": [_vp, _vp, ct.c_uint64, ct.POINTER(_vp), _vp],
     "zk_msm_bases_destroy": [_vp, _vp],
     "zk_msm_g1_prepared": [_vp, _vp, _vp, ct.c_uint64, ct.c_int, _u64p, _vp],
+    "zk_msm_g2_bases_create": [_vp, _vp, ct.c_uint64, ct.POINTER(_vp), _vp],
+    "zk_msm_g2_bases_destroy": [_vp, _vp],
+    "zk_msm_g2": [_vp, _vp, _vp, ct.c_uint64, ct.c_int, _u64p, _vp],
     "zk_g1_fixed_base": [_vp, _vp, _vp, ct.c_uint64, _vp, _vp],
     "zk_fixed_bases_create": [_vp, _vp, ct.c_uint64, ct.c_int, ct.POINTER(_vp), _vp],
     "zk_fixed_bases_destroy": [_vp, _vp],
@@ -152,6 +155,20 @@ class DeviceBuffer:
 
     def offset(self, nbytes: int) -> int:
         return self.ptr + nbytes
+
+
+class MsmG2Bases:
+    """A G2 base set on the device, converted once (zk_msm_g2_bases_*)."""
+
+    def __init__(self, ctx: "Context", handle, n: int) -> None:
+        self.ctx = ctx
+        self.handle = handle
+        self.n = n
+
+    def free(self) -> None:
+        if self.handle is not None and self.handle.value:
+            self.ctx._check(self.ctx.lib.zk_msm_g2_bases_destroy(self.ctx.handle, self.handle))
+            self.handle = None
 
 
 class MsmBases:
@@ -327,6 +344,27 @@ class Context:
         h = _vp()
         self._check(self.lib.zk_msm_bases_create(self.handle, bases, int(n), ct.byref(h), stream))
         return MsmBases(self, h, int(n))
+
+    def msm_g2_bases_create(self, words, stream=None) -> "MsmG2Bases":
+        """Upload a G2 base set: (n, 16) uint64 host words, affine canonical
+        plain (x0, x1, y0, y1), zeros for the identity."""
+        import numpy as np
+        w = np.ascontiguousarray(words, dtype="<u8")
+        if w.ndim != 2 or w.shape[1] != 16:
+            raise ValueError(f"G2 base words must have shape (n, 16), got {tuple(w.shape)}")
+        h = _vp()
+        self._check(self.lib.zk_msm_g2_bases_create(self.handle, w.ctypes.data, int(w.shape[0]), ct.byref(h), stream))
+        return MsmG2Bases(self, h, int(w.shape[0]))
+
+    def msm_g2(self, gb: "MsmG2Bases", scalars: int, n: int, window_bits: int = 0, stream=None):
+        """sum_i s_i Q_i over the first n bases of a device G2 set, scalars on
+        the device (Fr Montgomery); the affine canonical result as 16 uint64
+        words (zeros for the identity), the layout zk_g2_msm_host writes."""
+        import numpy as np
+        out = np.zeros(16, dtype="<u8")
+        self._check(self.lib.zk_msm_g2(self.handle, gb.handle, scalars, int(n), int(window_bits),
+                                       out.ctypes.data_as(_u64p), stream))
+        return out
 
     def msm_g1_prepared(self, mb: "MsmBases", scalars: int, n: int, window_bits: int = 0,
                         stream=None) -> Sequence[int]:

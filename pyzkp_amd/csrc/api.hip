@@ -376,6 +376,32 @@ int zk_msm_g1_prepared(zk_ctx* ctx, const zk_msm_bases* b, const void* scalars, 
                              pick(ctx, stream));
 }
 
+int zk_msm_g2_bases_create(zk_ctx* ctx, const uint64_t* bases, uint64_t n, zk_msm_g2_bases** out, void* stream) {
+  CTX_CHECK(ctx);
+  if (!out) return set_err(&ctx->c, ZK_ERR_INVALID, "null output");
+  *out = nullptr;
+  if (n && !bases) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  MsmG2Bases* b = nullptr;
+  int rc = msm_g2_bases_create(&ctx->c, bases, n, &b, pick(ctx, stream));
+  *out = (zk_msm_g2_bases*)b;
+  return rc;
+}
+
+int zk_msm_g2_bases_destroy(zk_ctx* ctx, zk_msm_g2_bases* b) {
+  CTX_CHECK(ctx);
+  hipDeviceSynchronize();
+  msm_g2_bases_destroy((MsmG2Bases*)b);
+  return ZK_OK;
+}
+
+int zk_msm_g2(zk_ctx* ctx, const zk_msm_g2_bases* b, const void* scalars, uint64_t n, int window_bits,
+              uint64_t out[16], void* stream) {
+  CTX_CHECK(ctx);
+  if (!out || !b) return set_err(&ctx->c, ZK_ERR_INVALID, "null argument");
+  if (n && !scalars) return set_err(&ctx->c, ZK_ERR_INVALID, "null buffer");
+  return msm_g2_run(&ctx->c, (const MsmG2Bases*)b, (const fe*)scalars, n, window_bits, out, pick(ctx, stream));
+}
+
 int zk_msm_g1_noalloc(zk_ctx* ctx, const void* bases, const void* scalars, uint64_t n, int window_bits,
                       uint64_t out_jac_mont[12], void* stream) {
   CTX_CHECK(ctx);

@@ -706,6 +706,37 @@ struct Bases {
 };
 
 }  // namespace g2
+
+// Horner over the device G2 MSM's window terms (XYZZ -> Jacobian as
+// (X ZZ^2, Y ZZZ^2, ZZZ), which holds because ZZ^3 = ZZZ^2)
+void g2_bits_combine(const uint64_t* t, const uint32_t* pos, uint32_t count, uint64_t out[16]) {
+  using namespace g2;
+  memset(out, 0, 16 * sizeof(uint64_t));
+  auto term = [&](uint32_t i) {
+    const uint64_t* w = t + (size_t)32 * i;
+    F2 c[4];
+    for (int k = 0; k < 4; k++) {
+      memcpy(c[k].c0.v, w + 8 * k, 32);
+      memcpy(c[k].c1.v, w + 8 * k + 4, 32);
+    }
+    if (f2_zero(c[2])) return j_inf();
+    J2 r;
+    r.X = f2_mul(c[0], f2_sqr(c[2]));
+    r.Y = f2_mul(c[1], f2_sqr(c[3]));
+    r.Z = c[3];
+    return r;
+  };
+  if (count == 0) return;
+  J2 acc = term(0);
+  uint32_t at = pos[0];
+  for (uint32_t i = 1; i < count; i++) {
+    for (; at > pos[i]; at--) acc = j_dbl(acc);
+    acc = j_add(acc, term(i));
+  }
+  for (; at > 0; at--) acc = j_dbl(acc);
+  if (j_is_inf(acc)) return;
+  wr_jac_affine(out, acc);
+}
 }  // namespace zk
 
 using namespace zk;

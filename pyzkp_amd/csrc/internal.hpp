@@ -156,6 +156,15 @@ int fixed_bases_create(Ctx* c, const fe* bases, uint64_t n, int window_bits, Fix
 struct xyzz;
 void g1_window_combine(const xyzz* s, uint32_t W, int c, uint64_t out[12]);
 void g1_bits_combine(const xyzz* t, const uint32_t* pos, uint32_t count, uint64_t out[12]);
+// G2: terms as 32 words each (x, y, zz, zzz over Fq2, R = 2^256 Montgomery,
+// canonical; zz = 0 is the identity) at strictly decreasing bit positions;
+// out: affine canonical plain words (x0, x1, y0, y1), zeros for the identity
+void g2_bits_combine(const uint64_t* t, const uint32_t* pos, uint32_t count, uint64_t out[16]);
+struct MsmG2Bases;
+int msm_g2_bases_create(Ctx* c, const uint64_t* host_words, uint64_t n, MsmG2Bases** out, hipStream_t st);
+void msm_g2_bases_destroy(MsmG2Bases* b);
+int msm_g2_run(Ctx* c, const MsmG2Bases* b, const fe* scalars, uint64_t n, int window_bits, uint64_t out[16],
+               hipStream_t st);
 void fixed_bases_destroy(FixedBases* fb);
 int msm_fixed_batch_run(Ctx* c, const FixedBases* fb, const fe* scalars, const uint64_t* offsets,
                         const uint64_t* sizes, const uint64_t* base_offsets, uint32_t count, uint64_t* out_jac,

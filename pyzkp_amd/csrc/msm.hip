@@ -20,6 +20,8 @@
 #include "internal.hpp"
 #include "curve.hpp"
 #include "curve29.hpp"
+#include "curve29_g2.hpp"
+#include <type_traits>
 
 namespace zk {
 
@@ -95,6 +97,116 @@ __device__ __forceinline__ void store_xyzz9(pt9* p, const xyzz9& v) {
   for (int i = 0; i < 9; i++) p->q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
 }
 
+// G2 buckets, head partials and sums: 8 x 9 limbs (288 bytes)
+struct __align__(16) pt18 {
+  uint4 q[18];
+};
+
+// coordinate k of a G2 XYZZ point: x.c0, x.c1, y.c0, y.c1, zz.c0, zz.c1,
+// zzz.c0, zzz.c1 (k a compile-time constant after unrolling)
+__device__ __forceinline__ f9& g2_coord(xyzz9g2& p, int k) {
+  f92& e = k < 2 ? p.x : k < 4 ? p.y : k < 6 ? p.zz : p.zzz;
+  return (k & 1) ? e.c1 : e.c0;
+}
+__device__ __forceinline__ const f9& g2_coord(const xyzz9g2& p, int k) {
+  const f92& e = k < 2 ? p.x : k < 4 ? p.y : k < 6 ? p.zz : p.zzz;
+  return (k & 1) ? e.c1 : e.c0;
+}
+
+__device__ __forceinline__ xyzz9g2 load_xyzz9g2(const pt18* p) {
+  uint32_t w[72];
+#pragma unroll
+  for (int i = 0; i < 18; i++) {
+    const uint4 v = p->q[i];
+    w[4 * i] = v.x;
+    w[4 * i + 1] = v.y;
+    w[4 * i + 2] = v.z;
+    w[4 * i + 3] = v.w;
+  }
+  xyzz9g2 r;
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+#pragma unroll
+    for (int j = 0; j < 9; j++) g2_coord(r, k).l[j] = w[9 * k + j];
+  return r;
+}
+
+__device__ __forceinline__ void store_xyzz9g2(pt18* p, const xyzz9g2& v) {
+  uint32_t w[72];
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+#pragma unroll
+    for (int j = 0; j < 9; j++) w[9 * k + j] = g2_coord(v, k).l[j];
+#pragma unroll
+  for (int i = 0; i < 18; i++) p->q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+}
+
+// G2 bases: x.c0, x.c1, y.c0, y.c1 as R = 2^261 limb values packed in 32
+// bytes each (128 bytes a base)
+__device__ __forceinline__ aff9g2 load_aff9g2(const fe* __restrict__ bases, uint32_t idx) {
+  const fe* b = bases + 4 * (uint64_t)idx;
+  aff9g2 a;
+  a.x.c0 = unpack9(load_fe(b));
+  a.x.c1 = unpack9(load_fe(b + 1));
+  a.y.c0 = unpack9(load_fe(b + 2));
+  a.y.c1 = unpack9(load_fe(b + 3));
+  return a;
+}
+
+// The point kinds of the accumulation, fix-up and digit-split reduction:
+// the bases' affine form, the XYZZ accumulator and its stored form, the
+// group law, and the export for the host combination (R = 2^256 canonical
+// XYZZ words; the identity as ZZ = ZZZ = 0).
+struct PtG1 {
+  using Aff = aff9;
+  using Acc = xyzz9;
+  using Sto = pt9;
+  static constexpr uint32_t EXP_FE = 4;  // exported fe per point
+  __device__ __forceinline__ static Aff load_base(const fe* b, uint32_t i) { return load_aff9(b, i); }
+  __device__ __forceinline__ static bool base_inf(const Aff& a) { return aff9_is_inf(a); }
+  __device__ __forceinline__ static void madd(Acc& p, bool& inf, const Aff& a, bool neg) { xyzz9_madd_t(p, inf, a, neg); }
+  __device__ __forceinline__ static Acc inf() { return xyzz9_inf(); }
+  __device__ __forceinline__ static bool is_inf(const Acc& p) { return xyzz9_is_inf(p); }
+  __device__ __forceinline__ static Acc add(const Acc& p, const Acc& q) { return xyzz9_add(p, q); }
+  __device__ __forceinline__ static Acc dbl(const Acc& p) { return xyzz9_dbl(p); }
+  __device__ __forceinline__ static Acc load(const Sto* s) { return load_xyzz9(s); }
+  __device__ __forceinline__ static void store(Sto* s, const Acc& p) { store_xyzz9(s, p); }
+  __device__ __forceinline__ static void export_to(fe* out, const Acc& p) {
+    xyzz r;
+    if (xyzz9_is_inf(p)) {
+      r.x = Fq::one();
+      r.y = Fq::one();
+      r.zz = Fq::zero();
+      r.zzz = Fq::zero();
+    } else {
+      r = xyzz_from9(p);
+    }
+    store_xyzz(reinterpret_cast<xyzz*>(out), r);
+  }
+};
+
+struct PtG2 {
+  using Aff = aff9g2;
+  using Acc = xyzz9g2;
+  using Sto = pt18;
+  static constexpr uint32_t EXP_FE = 8;
+  __device__ __forceinline__ static Aff load_base(const fe* b, uint32_t i) { return load_aff9g2(b, i); }
+  __device__ __forceinline__ static bool base_inf(const Aff& a) { return aff9g2_is_inf(a); }
+  __device__ __forceinline__ static void madd(Acc& p, bool& inf, const Aff& a, bool neg) { xyzz9g2_madd_t(p, inf, a, neg); }
+  __device__ __forceinline__ static Acc inf() { return xyzz9g2_inf(); }
+  __device__ __forceinline__ static bool is_inf(const Acc& p) { return xyzz9g2_is_inf(p); }
+  __device__ __forceinline__ static Acc add(const Acc& p, const Acc& q) { return xyzz9g2_add(p, q); }
+  __device__ __forceinline__ static Acc dbl(const Acc& p) { return xyzz9g2_dbl(p); }
+  __device__ __forceinline__ static Acc load(const Sto* s) { return load_xyzz9g2(s); }
+  __device__ __forceinline__ static void store(Sto* s, const Acc& p) { store_xyzz9g2(s, p); }
+  __device__ __forceinline__ static void export_to(fe* out, const Acc& p) {
+    const bool inf = xyzz9g2_is_inf(p);
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      store_fe(out + k, inf ? Fq::zero() : pack9(Fq9::canonical(Fq9::to256(g2_coord(p, k)))));
+  }
+};
+
 // wire bases (affine, R = 2^256, x = y = 0 for infinity) -> R = 2^261 form
 __global__ __launch_bounds__(256) void msm_bases_convert_kernel(const fe* __restrict__ in, uint64_t n,
                                                                 fe* __restrict__ out) {
@@ -106,20 +218,11 @@ __global__ __launch_bounds__(256) void msm_bases_convert_kernel(const fe* __rest
 }
 
 // window sums: limb form -> R = 2^256 canonical XYZZ for the host combination
-__global__ void msm_wsum_export_kernel(const pt9* __restrict__ in, uint32_t W, xyzz* __restrict__ out) {
+template <class P>
+__global__ void msm_wsum_export_kernel(const typename P::Sto* __restrict__ in, uint32_t W, fe* __restrict__ out) {
   const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= W) return;
-  const xyzz9 p = load_xyzz9(in + w);
-  xyzz r;
-  if (xyzz9_is_inf(p)) {
-    r.x = Fq::one();
-    r.y = Fq::one();
-    r.zz = Fq::zero();
-    r.zzz = Fq::zero();
-  } else {
-    r = xyzz_from9(p);
-  }
-  store_xyzz(out + w, r);
+  P::export_to(out + (uint64_t)w * P::EXP_FE, P::load(in + w));
 }
 
 
@@ -667,10 +770,11 @@ __global__ __launch_bounds__(FINE_STAGE_THREADS) void msm_fine_scatter_staged_ke
 //    A run (bucket) that starts inside the segment is owned by it and its
 //    partial sum goes straight to buckets[key]; the run the segment starts
 //    in the middle of is its "head" partial, added in by the fix-up pass.
+template <class P>
 __global__ __launch_bounds__(256) void msm_accumulate_kernel(
     const fe* __restrict__ bases, const uint32_t* __restrict__ sorted,
-    const uint32_t* __restrict__ row_ptr, uint32_t nkeys, uint32_t L, pt9* __restrict__ buckets,
-    uint32_t* __restrict__ head_key, pt9* __restrict__ head, uint32_t* __restrict__ maxrun) {
+    const uint32_t* __restrict__ row_ptr, uint32_t nkeys, uint32_t L, typename P::Sto* __restrict__ buckets,
+    uint32_t* __restrict__ head_key, typename P::Sto* __restrict__ head, uint32_t* __restrict__ maxrun) {
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t M = row_ptr[nkeys];
   const uint64_t start = s * L;
@@ -685,7 +789,7 @@ __global__ __launch_bounds__(256) void msm_accumulate_kernel(
   uint32_t k = lo;
   uint32_t rbeg = row_ptr[k], rend = row_ptr[k + 1];
   uint32_t hk = NO_KEY;
-  xyzz9 acc = xyzz9_inf();
+  typename P::Acc acc = P::inf();
   bool acc_inf = true;
   // a head partial of bucket k: the run of heads it belongs to spans `len`
   // segments; the fix-up rounds only have work for runs longer than their stride
@@ -698,28 +802,28 @@ __global__ __launch_bounds__(256) void msm_accumulate_kernel(
       if (rend > rbeg) {
         if (rbeg < start) {
           hk = k;
-          store_xyzz9(head + s, acc);
+          P::store(head + s, acc);
           note_run(rbeg, rend);
         } else {
-          store_xyzz9(buckets + k, acc);
+          P::store(buckets + k, acc);
         }
       }
-      acc = xyzz9_inf();
+      acc = P::inf();
       acc_inf = true;
       k++;
       rbeg = rend;
       rend = row_ptr[k + 1];
     }
     const uint32_t v = sorted[p];
-    aff9 a = load_aff9(bases, v & ~SIGN);
-    if (!aff9_is_inf(a)) xyzz9_madd_t(acc, acc_inf, a, (v & SIGN) != 0);
+    const typename P::Aff a = P::load_base(bases, v & ~SIGN);
+    if (!P::base_inf(a)) P::madd(acc, acc_inf, a, (v & SIGN) != 0);
   }
   if (rbeg < start) {
     hk = k;
-    store_xyzz9(head + s, acc);
+    P::store(head + s, acc);
     note_run(rbeg, rend);
   } else {
-    store_xyzz9(buckets + k, acc);
+    P::store(buckets + k, acc);
   }
   head_key[s] = hk;
 }
@@ -766,11 +870,12 @@ __device__ __forceinline__ uint64_t fixup_fan(uint32_t mr, uint32_t rounds) {
   return lo;
 }
 
+template <class P>
 __global__ __launch_bounds__(256) void msm_fixup_round_kernel(const uint32_t* __restrict__ row_ptr,
                                                               uint32_t nkeys, uint32_t L, uint32_t round,
                                                               uint32_t rounds,
                                                               const uint32_t* __restrict__ head_key,
-                                                              pt9* __restrict__ head,
+                                                              typename P::Sto* __restrict__ head,
                                                               const uint32_t* __restrict__ maxrun) {
   const uint32_t mr = *maxrun;
   // F >= 2, so the stride F^round is at least 2^round: no run longer than
@@ -789,20 +894,21 @@ __global__ __launch_bounds__(256) void msm_fixup_round_kernel(const uint32_t* __
   uint64_t stride = 1;
   for (uint32_t i = 0; i < round; i++) stride *= F;
   if (off % (stride * F) != 0 || off + stride >= len) return;
-  xyzz9 acc = load_xyzz9(head + s);
+  typename P::Acc acc = P::load(head + s);
   for (uint32_t j = 1; j < F; j++) {
     const uint64_t t = off + (uint64_t)j * stride;
     if (t >= len) break;
-    acc = xyzz9_add(acc, load_xyzz9(head + first + t));
+    acc = P::add(acc, P::load(head + first + t));
   }
-  store_xyzz9(head + s, acc);
+  P::store(head + s, acc);
 }
 
+template <class P>
 __global__ __launch_bounds__(256) void msm_fixup_final_kernel(const uint32_t* __restrict__ row_ptr,
                                                               uint32_t nkeys, uint32_t L,
-                                                              pt9* __restrict__ buckets,
+                                                              typename P::Sto* __restrict__ buckets,
                                                               const uint32_t* __restrict__ head_key,
-                                                              const pt9* __restrict__ head) {
+                                                              const typename P::Sto* __restrict__ head) {
   const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t M = row_ptr[nkeys];
   const uint64_t S = (M + L - 1) / L;
@@ -811,7 +917,7 @@ __global__ __launch_bounds__(256) void msm_fixup_final_kernel(const uint32_t* __
   if (k == NO_KEY) return;
   uint64_t first, len;
   if (!head_run(row_ptr, k, L, s, first, len) || s != first) return;
-  store_xyzz9(buckets + k, xyzz9_add(load_xyzz9(buckets + k), load_xyzz9(head + s)));
+  P::store(buckets + k, P::add(P::load(buckets + k), P::load(head + s)));
 }
 
 // 6. bucket reduction, chunked: chunk t of window w covers buckets
@@ -897,75 +1003,76 @@ static constexpr uint32_t SPLIT_PLANE_LANES = 32;                   // threads p
 // running every level in every wave.  LANES = 32 halves each thread's chain
 // of dependent additions and doubles the workgroups; the launch takes it
 // when the doubled grid still fits one round of resident workgroups.
-template <uint32_t LANES>
-__global__ __launch_bounds__(SPLIT_THREADS) void msm_split_sums_kernel(const pt9* __restrict__ buckets,
+template <class P, uint32_t LANES>
+__global__ __launch_bounds__(SPLIT_THREADS) void msm_split_sums_kernel(const typename P::Sto* __restrict__ buckets,
                                                                        const uint32_t* __restrict__ row_ptr,
                                                                        uint32_t B, uint32_t ml, uint32_t mh,
-                                                                       uint32_t nblk_h, pt9* __restrict__ hsum,
-                                                                       pt9* __restrict__ lsum) {
+                                                                       uint32_t nblk_h, typename P::Sto* __restrict__ hsum,
+                                                                       typename P::Sto* __restrict__ lsum) {
   constexpr uint32_t PER = SPLIT_THREADS / LANES;
-  __shared__ xyzz9 sh[SPLIT_THREADS];
+  __shared__ typename P::Acc sh[SPLIT_THREADS];
   const uint32_t r = blockIdx.y, tid = threadIdx.x;
   const uint32_t M = 1u << (ml + mh);
   const bool hrow = blockIdx.x < nblk_h;
   const uint32_t o = (hrow ? blockIdx.x : blockIdx.x - nblk_h) * PER + tid % PER;
   const uint32_t p = tid / PER;
   const uint32_t nout = hrow ? (1u << mh) + 1 : (1u << ml);
-  xyzz9 acc = xyzz9_inf();
+  typename P::Acc acc = P::inf();
   if (o < nout) {
     const uint32_t cnt = hrow ? (o < (1u << mh) ? (1u << ml) : 1u) : (1u << mh);
     for (uint32_t i = p; i < cnt; i += LANES) {
       const uint32_t b = hrow ? (o << ml) + i : (i << ml) + o;  // H_(2^mh): o << ml = M
       if (b == 0 || b > M) continue;
       const uint32_t key = r * B + b;
-      if (row_ptr[key + 1] > row_ptr[key]) acc = xyzz9_add(acc, load_xyzz9(buckets + key));
+      if (row_ptr[key + 1] > row_ptr[key]) acc = P::add(acc, P::load(buckets + key));
     }
   }
   sh[tid] = acc;
   __syncthreads();
   for (uint32_t step = LANES >> 1; step > 0; step >>= 1) {
-    if (p < step) sh[tid] = xyzz9_add(sh[tid], sh[tid + step * PER]);
+    if (p < step) sh[tid] = P::add(sh[tid], sh[tid + step * PER]);
     __syncthreads();
   }
-  if (p == 0 && o < nout) store_xyzz9((hrow ? hsum : lsum) + (uint64_t)r * nout + o, sh[tid]);
+  if (p == 0 && o < nout) P::store((hrow ? hsum : lsum) + (uint64_t)r * nout + o, sh[tid]);
 }
 
 // G_g of row r: SPLIT_FOLD plane sums side by side (SPLIT_PLANE_LANES threads
 // each), then 2^i-weighted by Horner's rule in one thread
+template <class P>
 __global__ __launch_bounds__(SPLIT_FOLD* SPLIT_PLANE_LANES) void msm_split_planes_kernel(
-    const pt9* __restrict__ hsum, const pt9* __restrict__ lsum, uint32_t ml, uint32_t mh, uint32_t ng,
-    pt9* __restrict__ gsum) {
-  __shared__ xyzz9 sh[SPLIT_FOLD * SPLIT_PLANE_LANES];
+    const typename P::Sto* __restrict__ hsum, const typename P::Sto* __restrict__ lsum, uint32_t ml, uint32_t mh, uint32_t ng,
+    typename P::Sto* __restrict__ gsum) {
+  __shared__ typename P::Acc sh[SPLIT_FOLD * SPLIT_PLANE_LANES];
   const uint32_t r = blockIdx.y, g = blockIdx.x, tid = threadIdx.x;
   const uint32_t q = tid / SPLIT_PLANE_LANES, p = tid % SPLIT_PLANE_LANES;
   const uint32_t j = g * SPLIT_FOLD + q, m = ml + mh;
-  xyzz9 acc = xyzz9_inf();
+  typename P::Acc acc = P::inf();
   if (j <= m) {
     const bool low = j < ml;
     const uint32_t jb = low ? j : j - ml;  // the index bit this plane selects
     const uint32_t span = low ? ml : mh;   // index bits of the L / H sums below the top
-    const pt9* src = low ? lsum + (uint64_t)r * (1u << ml) : hsum + (uint64_t)r * ((1u << mh) + 1);
+    const typename P::Sto* src = low ? lsum + (uint64_t)r * (1u << ml) : hsum + (uint64_t)r * ((1u << mh) + 1);
     if (jb < span) {
       // the indices below 2^span with bit jb set: k with a one inserted at bit jb
       const uint32_t cnt = 1u << (span - 1);
       for (uint32_t k = p; k < cnt; k += SPLIT_PLANE_LANES) {
         const uint32_t x = ((k >> jb) << (jb + 1)) | (1u << jb) | (k & ((1u << jb) - 1));
-        acc = xyzz9_add(acc, load_xyzz9(src + x));
+        acc = P::add(acc, P::load(src + x));
       }
     } else if (p == 0) {
-      acc = load_xyzz9(src + (1u << mh));  // j = m: only H_(2^mh) = B_M has bit mh
+      acc = P::load(src + (1u << mh));  // j = m: only H_(2^mh) = B_M has bit mh
     }
   }
   sh[tid] = acc;
   __syncthreads();
   for (uint32_t step = SPLIT_PLANE_LANES >> 1; step > 0; step >>= 1) {
-    if (p < step) sh[tid] = xyzz9_add(sh[tid], sh[tid + step]);
+    if (p < step) sh[tid] = P::add(sh[tid], sh[tid + step]);
     __syncthreads();
   }
   if (tid == 0) {
-    xyzz9 G = sh[(SPLIT_FOLD - 1) * SPLIT_PLANE_LANES];
-    for (int i = (int)SPLIT_FOLD - 2; i >= 0; i--) G = xyzz9_add(xyzz9_dbl(G), sh[i * SPLIT_PLANE_LANES]);
-    store_xyzz9(gsum + (uint64_t)r * ng + g, G);
+    typename P::Acc G = sh[(SPLIT_FOLD - 1) * SPLIT_PLANE_LANES];
+    for (int i = (int)SPLIT_FOLD - 2; i >= 0; i--) G = P::add(P::dbl(G), sh[i * SPLIT_PLANE_LANES]);
+    P::store(gsum + (uint64_t)r * ng + g, G);
   }
 }
 
@@ -1104,6 +1211,8 @@ struct MsmShape {
   uint32_t L = 0, K = 0, cpw = 0;
   uint32_t nchunks = 0;
   uint32_t ml = 0, mh = 0, ng = 0;  // digit-split reduction: bucket index bits below / above, folded terms per row
+  uint32_t pt_bytes = sizeof(pt9);    // a stored point (bucket, head partial, sum): pt9, or pt18 for G2
+  uint32_t exp_bytes = sizeof(xyzz);  // an exported point: 4 fe, or 8 for G2
 };
 
 struct MsmBuffers {
@@ -1232,17 +1341,17 @@ static size_t layout(const MsmShape& sh, char* base, MsmBuffers* b, uint32_t* di
   t.sums = (uint32_t*)take((size_t)sh.nchunks * 4);
   t.fc0 = (uint32_t*)take(((size_t)sh.nrows * sh.NC + 1) * 4);
   t.fhist = (uint32_t*)take(((size_t)sh.fchunks << sh.S) * 4 + 4);
-  t.buckets = (pt9*)take((size_t)sh.nkeys * sizeof(pt9));
+  t.buckets = (pt9*)take((size_t)sh.nkeys * sh.pt_bytes);
   t.hkey = (uint32_t*)take(sh.Smax * 4);
-  t.head = (pt9*)take(sh.Smax * sizeof(pt9));
+  t.head = (pt9*)take(sh.Smax * sh.pt_bytes);
   t.partial = (pt9*)take((size_t)sh.nrows * sh.cpw * sizeof(pt9));
   t.partial2 = (pt9*)take((size_t)sh.nrows * ((sh.cpw + RED_GROUP - 1) / RED_GROUP) * sizeof(pt9));
   t.rsum = (pt9*)take((size_t)sh.nrows * sizeof(pt9));
   t.rexp = (xyzz*)take((size_t)sh.nrows * sizeof(xyzz));
-  t.hsum = (pt9*)take((size_t)sh.nrows * ((1u << sh.mh) + 1) * sizeof(pt9));
-  t.lsum = (pt9*)take((size_t)sh.nrows * (1u << sh.ml) * sizeof(pt9));
-  t.gsum = (pt9*)take((size_t)sh.nrows * sh.ng * sizeof(pt9));
-  t.gexp = (xyzz*)take((size_t)sh.nrows * sh.ng * sizeof(xyzz));
+  t.hsum = (pt9*)take((size_t)sh.nrows * ((1u << sh.mh) + 1) * sh.pt_bytes);
+  t.lsum = (pt9*)take((size_t)sh.nrows * (1u << sh.ml) * sh.pt_bytes);
+  t.gsum = (pt9*)take((size_t)sh.nrows * sh.ng * sh.pt_bytes);
+  t.gexp = (xyzz*)take((size_t)sh.nrows * sh.ng * sh.exp_bytes);
   t.rows = (MsmRow*)take((size_t)sh.nrows * sizeof(MsmRow));
   t.maxrun = (uint32_t*)take(4);
   if (b) *b = t;
@@ -1305,6 +1414,7 @@ static void sort_placement(const MsmShape& sh, const MsmBuffers& b, hipStream_t 
 // split: rows are the windows of one MSM, reduced by digit split (their
 // folded terms in b.gexp, bit positions c r + SPLIT_FOLD g); otherwise one
 // sum per row in b.rexp
+template <class P = PtG1>
 static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* bases9, hipStream_t st,
                     hipEvent_t sorted_ev = nullptr, bool counted = false, bool split = false) {
   hipEvent_t ev;
@@ -1321,9 +1431,12 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
   phase_end(c, "msm_sort", st, ev);
   if (sorted_ev) ZK_HIP(c, hipEventRecord(sorted_ev, st));
 
+  using Sto = typename P::Sto;
+  Sto* buckets = reinterpret_cast<Sto*>(b.buckets);
+  Sto* head = reinterpret_cast<Sto*>(b.head);
   phase_begin(c, "msm_accumulate", st, &ev);
-  msm_accumulate_kernel<<<grid_for(sh.Smax, 256), 256, 0, st>>>(bases9, b.sorted, b.row_ptr, sh.nkeys, sh.L,
-                                                               b.buckets, b.hkey, b.head, b.maxrun);
+  msm_accumulate_kernel<P><<<grid_for(sh.Smax, 256), 256, 0, st>>>(bases9, b.sorted, b.row_ptr, sh.nkeys, sh.L,
+                                                                  buckets, b.hkey, head, b.maxrun);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_accumulate", st, ev);
 
@@ -1331,10 +1444,10 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
   uint32_t rounds = 0;  // ceil(log_FIXUP_FAN(Smax)): enough for a run of every segment
   for (uint64_t span = 1; span < sh.Smax; span *= FIXUP_FAN) rounds++;
   for (uint32_t r = 0; r < rounds; r++)
-    msm_fixup_round_kernel<<<grid_for(sh.Smax, 256), 256, 0, st>>>(b.row_ptr, sh.nkeys, sh.L, r, rounds, b.hkey,
-                                                                  b.head, b.maxrun);
-  msm_fixup_final_kernel<<<grid_for(sh.Smax, 256), 256, 0, st>>>(b.row_ptr, sh.nkeys, sh.L, b.buckets, b.hkey,
-                                                                b.head);
+    msm_fixup_round_kernel<P><<<grid_for(sh.Smax, 256), 256, 0, st>>>(b.row_ptr, sh.nkeys, sh.L, r, rounds,
+                                                                     b.hkey, head, b.maxrun);
+  msm_fixup_final_kernel<P><<<grid_for(sh.Smax, 256), 256, 0, st>>>(b.row_ptr, sh.nkeys, sh.L, buckets, b.hkey,
+                                                                   head);
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_fixup", st, ev);
 
@@ -1344,21 +1457,25 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
     // (with 5% slack), else 16
     const uint32_t bh32 = ((1u << sh.mh) + 1 + 7) / 8, bl32 = ((1u << sh.ml) + 7) / 8;
     if ((uint64_t)sh.nrows * (bh32 + bl32) * 20 <= (uint64_t)c->num_cus * SPLIT_WG_PER_CU * 21) {
-      msm_split_sums_kernel<32><<<dim3(bh32 + bl32, sh.nrows), SPLIT_THREADS, 0, st>>>(
-          b.buckets, b.row_ptr, sh.B, sh.ml, sh.mh, bh32, b.hsum, b.lsum);
+      msm_split_sums_kernel<P, 32><<<dim3(bh32 + bl32, sh.nrows), SPLIT_THREADS, 0, st>>>(
+          buckets, b.row_ptr, sh.B, sh.ml, sh.mh, bh32, reinterpret_cast<Sto*>(b.hsum), reinterpret_cast<Sto*>(b.lsum));
     } else {
       const uint32_t bh = ((1u << sh.mh) + 1 + 15) / 16, bl = ((1u << sh.ml) + 15) / 16;
-      msm_split_sums_kernel<16><<<dim3(bh + bl, sh.nrows), SPLIT_THREADS, 0, st>>>(b.buckets, b.row_ptr, sh.B,
-                                                                                  sh.ml, sh.mh, bh, b.hsum, b.lsum);
+      msm_split_sums_kernel<P, 16><<<dim3(bh + bl, sh.nrows), SPLIT_THREADS, 0, st>>>(
+          buckets, b.row_ptr, sh.B, sh.ml, sh.mh, bh, reinterpret_cast<Sto*>(b.hsum), reinterpret_cast<Sto*>(b.lsum));
     }
-    msm_split_planes_kernel<<<dim3(sh.ng, sh.nrows), SPLIT_FOLD * SPLIT_PLANE_LANES, 0, st>>>(
-        b.hsum, b.lsum, sh.ml, sh.mh, sh.ng, b.gsum);
-    msm_wsum_export_kernel<<<grid_for((uint64_t)sh.nrows * sh.ng, 64), 64, 0, st>>>(b.gsum, sh.nrows * sh.ng,
-                                                                                      b.gexp);
+    msm_split_planes_kernel<P><<<dim3(sh.ng, sh.nrows), SPLIT_FOLD * SPLIT_PLANE_LANES, 0, st>>>(
+        reinterpret_cast<Sto*>(b.hsum), reinterpret_cast<Sto*>(b.lsum), sh.ml, sh.mh, sh.ng,
+        reinterpret_cast<Sto*>(b.gsum));
+    msm_wsum_export_kernel<P><<<grid_for((uint64_t)sh.nrows * sh.ng, 64), 64, 0, st>>>(
+        reinterpret_cast<Sto*>(b.gsum), sh.nrows * sh.ng, reinterpret_cast<fe*>(b.gexp));
     ZK_HIP(c, hipGetLastError());
     phase_end(c, "msm_reduce", st, ev);
     return ZK_OK;
   }
+  if constexpr (!std::is_same<P, PtG1>::value) {
+    return set_err(c, ZK_ERR_INVALID, "the chunked bucket reduction is G1 only");
+  } else {
   const uint32_t g1 = (sh.cpw + RED_CHUNK - 1) / RED_CHUNK;  // partials per window after the chunk kernel
   msm_reduce_chunks_kernel<<<dim3(g1, sh.nrows), RED_CHUNK, 0, st>>>(b.buckets, b.row_ptr, sh.B, sh.K, sh.cpw,
                                                                      b.partial);
@@ -1376,10 +1493,11 @@ static int run_rows(Ctx* c, const MsmShape& sh, const MsmBuffers& b, const fe* b
       m = G;
     }
   }
-  msm_wsum_export_kernel<<<grid_for(sh.nrows, 64), 64, 0, st>>>(b.rsum, sh.nrows, b.rexp);
+  msm_wsum_export_kernel<P><<<grid_for(sh.nrows, 64), 64, 0, st>>>(b.rsum, sh.nrows, reinterpret_cast<fe*>(b.rexp));
   ZK_HIP(c, hipGetLastError());
   phase_end(c, "msm_reduce", st, ev);
   return ZK_OK;
+  }
 }
 
 // XYZZ (R = 2^256) -> Jacobian (X ZZ^2, Y ZZZ^2, ZZZ) words; zeros for infinity
@@ -1711,6 +1829,122 @@ int msm_g1_run(Ctx* c, const fe* bases, const fe* scalars, uint64_t n, int windo
   const uint32_t G = msm_group_count(c, cb);
   if (G == 1) return msm_g1_single(c, bases, nullptr, scalars, n, cb, out_jac, st, nullptr, 0);
   return msm_g1_grouped(c, bases, nullptr, scalars, n, cb, G, out_jac, st);
+}
+
+// ------------------------------------------------------------ G2 MSM
+// The G2 sum over a device base set, through the same digits, bucket sort,
+// accumulation, fix-up and digit-split reduction as G1 (the PtG2 point kind:
+// Fq2 coordinates), with the window terms combined on the host.  A base set
+// holds the host's plain affine words (x0, x1, y0, y1; zeros for the
+// identity) converted once to the R = 2^261 limb form.
+struct MsmG2Bases {
+  uint64_t n = 0;
+  DevBuf conv;
+};
+
+__global__ __launch_bounds__(256) void msm_g2_bases_convert_kernel(const fe* __restrict__ in, uint64_t n,
+                                                                   fe* __restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const f9 r2 = Fq9::constant(Fq29Params::R2_522);  // plain x -> x 2^261 (zero stays zero)
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    store_fe(out + 4 * i + k, pack9(Fq9::canonical(Fq9::mul(unpack9(load_fe(in + 4 * i + k)), r2))));
+}
+
+int msm_g2_bases_create(Ctx* c, const uint64_t* host_words, uint64_t n, MsmG2Bases** out, hipStream_t st) {
+  *out = nullptr;
+  if (n >= (1ull << 31)) return set_err(c, ZK_ERR_INVALID, "MSM size must be below 2^31");
+  MsmG2Bases* b = new MsmG2Bases();
+  b->n = n;
+  b->conv.bytes = (n ? n : 1) * 128;
+  void* stage = nullptr;
+  hipError_t e = hipMalloc(&b->conv.ptr, b->conv.bytes);
+  if (e == hipSuccess && n) e = hipMalloc(&stage, n * 128);
+  if (e == hipSuccess && n) e = hipMemcpyAsync(stage, host_words, n * 128, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && n) {
+    msm_g2_bases_convert_kernel<<<grid_for(n, 256), 256, 0, st>>>((const fe*)stage, n, (fe*)b->conv.ptr);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (stage) (void)hipFree(stage);
+  if (e != hipSuccess) {
+    if (b->conv.ptr) (void)hipFree(b->conv.ptr);
+    delete b;
+    return hip_check(c, e, "G2 base set");
+  }
+  *out = b;
+  return ZK_OK;
+}
+
+void msm_g2_bases_destroy(MsmG2Bases* b) {
+  if (!b) return;
+  if (b->conv.ptr) (void)hipFree(b->conv.ptr);
+  delete b;
+}
+
+int msm_g2_run(Ctx* c, const MsmG2Bases* gb, const fe* scalars, uint64_t n, int window_bits, uint64_t out[16],
+               hipStream_t st) {
+  for (int i = 0; i < 16; i++) out[i] = 0;
+  if (n > gb->n) return set_err(c, ZK_ERR_INVALID, "more scalars than G2 bases");
+  if (n == 0) return ZK_OK;
+  int cb;
+  int rc = msm_check_args(c, n, window_bits, &cb);
+  if (rc) return rc;
+  const uint32_t W = (255 + cb - 1) / cb;
+  const uint32_t nt = (uint32_t)((n + SORT_TILE - 1) / SORT_TILE);
+  std::vector<MsmRow> rows(W);
+  for (uint32_t w = 0; w < W; w++) {
+    MsmRow& r = rows[w];
+    r.off = (uint64_t)w * n;
+    r.len = (uint32_t)n;
+    r.inner = (uint32_t)n;
+    r.stride = 0;
+    r.ntiles = nt;
+    r.tile0 = w * nt;
+    r.boff = 0;
+  }
+  MsmShape sh;
+  rc = make_shape(c, cb, W, rows, sh);
+  if (rc) return rc;
+  sh.pt_bytes = sizeof(pt18);
+  sh.exp_bytes = PtG2::EXP_FE * sizeof(fe);
+  const size_t o_rest = align256((size_t)W * n * 4);
+  uint32_t dummy = 0;
+  const size_t need = align256(o_rest + layout(sh, nullptr, nullptr, &dummy));
+  void* p;
+  rc = ensure_scratch(c, 3, need, &p);
+  if (rc) return rc;
+  char* base = (char*)p;
+  uint32_t* digits = (uint32_t*)base;
+  MsmBuffers b;
+  layout(sh, base + o_rest, &b, digits);
+  ZK_HIP(c, hipMemcpyAsync(b.rows, rows.data(), rows.size() * sizeof(MsmRow), hipMemcpyHostToDevice, st));
+  hipEvent_t ev;
+  phase_begin(c, "msm_digits", st, &ev);
+  msm_digits_count_kernel<<<nt, DIGITS_COUNT_THREADS, (size_t)W * sh.NC * 4, st>>>(
+      scalars, n, (uint32_t)cb, W, sh.S, sh.NC, nt, digits, b.hist);
+  ZK_HIP(c, hipGetLastError());
+  phase_end(c, "msm_digits", st, ev);
+  rc = run_rows<PtG2>(c, sh, b, (const fe*)gb->conv.ptr, st, nullptr, true, true);
+  if (rc) return rc;
+  // term g of window w (8 fe) sits at bit c w + SPLIT_FOLD g; hand them over top first
+  const uint32_t T = W * sh.ng;
+  void* hp;
+  rc = ensure_host_out(c, (size_t)T * sh.exp_bytes, &hp);
+  if (rc) return rc;
+  ZK_HIP(c, hipMemcpyAsync(hp, b.gexp, (size_t)T * sh.exp_bytes, hipMemcpyDeviceToHost, st));
+  ZK_HIP(c, hipStreamSynchronize(st));
+  const uint64_t* ts = (const uint64_t*)hp;
+  std::vector<uint64_t> desc((size_t)T * 32);
+  std::vector<uint32_t> pos(T);
+  for (uint32_t i = 0; i < T; i++) {
+    const uint32_t k = T - 1 - i, w = k / sh.ng, g = k % sh.ng;
+    memcpy(desc.data() + (size_t)32 * i, ts + (size_t)32 * k, 256);
+    pos[i] = (uint32_t)cb * w + SPLIT_FOLD * g;
+  }
+  g2_bits_combine(desc.data(), pos.data(), T, out);
+  return ZK_OK;
 }
 
 // ------------------------------------------------------------ fixed-base sets
