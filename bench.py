@@ -19,7 +19,8 @@ metric:
   continuous-batching server and by the plain one-proof-at-a-time loop, and
   the highest rate each sustains (at most its own capacity) with p99
   latency within the objective; every proof verified and equal to the
-  plain loop's byte for byte.
+  plain loop's byte for byte.  The G2 MSMs run on the device by default
+  (--g2-host: on host threads).
 
 ``--workload ntt`` / ``serve`` print those legs as the line;
 ``--workload batch`` the batched fixed-base MSMs + NTTs replayed as one
@@ -69,6 +70,8 @@ def parse():
     ap.add_argument("--no-ntt", action="store_true")
     ap.add_argument("--no-serve", action="store_true", help="MSM line: leave out the served-prover leg")
     ap.add_argument("--device-slots", type=int, default=2, help="serve: concurrent device slots")
+    ap.add_argument("--g2-host", action="store_true",
+                    help="serve: G2 MSMs on host threads (--cpu-threads, --g2-workers) instead of on the device")
     ap.add_argument("--loads", type=str, default="0.8,0.9,1.0",
                     help="serve: Poisson arrival rates as fractions of the plain loop's capacity")
     ap.add_argument("--server-loads", type=str, default="0.85,0.92,1.0",
@@ -476,6 +479,7 @@ def serve_leg(args, ctx, with_cpu: bool) -> dict:
 
     lo, hi = (int(x) for x in args.serve_log.split(","))
     R = args.requests
+    g2d = not args.g2_host
     classes = request_classes(R, lo, hi)
     ks = sorted(set(classes))
     log(f"serve: {R} requests, classes {dict((k, classes.count(k)) for k in ks)}")
@@ -488,9 +492,9 @@ def serve_leg(args, ctx, with_cpu: bool) -> dict:
     svc = {}
     for k in ks:
         w = synthetic_witness(circuits[k], 0x5500 + k)
-        prove(keys[k], w, 1, 2, g2_threads=args.cpu_threads)
+        prove(keys[k], w, 1, 2, g2_threads=args.cpu_threads, g2_device=g2d)
         t1 = time.perf_counter()
-        prove(keys[k], w, 3, 4, g2_threads=args.cpu_threads)
+        prove(keys[k], w, 3, 4, g2_threads=args.cpu_threads, g2_device=g2d)
         svc[k] = time.perf_counter() - t1
     log(f"  plain service ms per class {dict((k, round(v * 1e3, 1)) for k, v in svc.items())}")
     budget = int(args.hbm_budget_gb * 2**30)
@@ -499,7 +503,7 @@ def serve_leg(args, ctx, with_cpu: bool) -> dict:
     # one long-lived server, as in deployment: its slots build their NTT
     # tables and workspaces in a warm-up (two requests per class, a burst)
     srv = ProveServer(keys, hbm_budget=budget, max_batch=16, g2_workers=args.g2_workers,
-                      cpu_threads=args.cpu_threads, device_slots=args.device_slots)
+                      cpu_threads=args.cpu_threads, device_slots=args.device_slots, g2_device=g2d)
     warm = make_requests(circuits, [k for k in ks for _ in range(2)], [0.0] * (2 * len(ks)), witness_seed=0x5600)
 
     def server_run(reqs):
@@ -518,7 +522,7 @@ def serve_leg(args, ctx, with_cpu: bool) -> dict:
         srv.run(warm)
         sat = make_requests(circuits, classes, [0.0] * R)
         log("  capacity: plain loop")
-        plain_sat = serve_plain(keys, sat, cpu_threads=args.cpu_threads)
+        plain_sat = serve_plain(keys, sat, cpu_threads=args.cpu_threads, g2_device=g2d)
         log("  capacity: server")
         srv_sat, sat_stats = server_run(sat)
         check(sat, srv_sat, plain_sat)
@@ -531,7 +535,7 @@ def serve_leg(args, ctx, with_cpu: bool) -> dict:
         for rate, what in rates:
             reqs = make_requests(circuits, classes, poisson_arrivals(R, rate))
             log(f"  Poisson {rate:.2f}/s ({what}): plain, server")
-            pl = serve_plain(keys, reqs, cpu_threads=args.cpu_threads)
+            pl = serve_plain(keys, reqs, cpu_threads=args.cpu_threads, g2_device=g2d)
             sv, st = server_run(reqs)
             check(reqs, sv, pl)
             sweep.append({"load": round(rate / base, 4), "rate_of": what, "rate_per_s": round(rate, 3),
@@ -569,7 +573,8 @@ def serve_leg(args, ctx, with_cpu: bool) -> dict:
         "key_setup_s": round(t_setup, 2), "keys_hbm_gib": round(key_bytes / 2**30, 3),
         "working_set_max_mib": round(max(working_set_bytes(pk) for pk in keys.values()) / 2**20, 2),
         "cpu_threads": args.cpu_threads, "g2_workers": args.g2_workers, "device_slots": args.device_slots,
-        "hbm_budget_gb": args.hbm_budget_gb, "cpu_baseline": None,
+        "hbm_budget_gb": args.hbm_budget_gb, "g2_msm_on": "device" if g2d else "host threads",
+        "cpu_baseline": None,
     }
     if with_cpu:
         from oracle import groth16 as OG

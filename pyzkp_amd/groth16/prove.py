@@ -9,9 +9,11 @@ non-zero remainder); H comes from 3 iNTTs, 3 coset NTTs (shift g = 5),
 coefficients as the reference's poly_mul_ntt + div_by_xn_minus_1, since
 Z_H(g w^i) = g^n - 1 on the coset.  The four G1 MSMs (a_query . w,
 b_query_g1 . w, h_query . h[:n-1], l_query . w[aux]) run as one fixed-base
-batch over the key's query set; the G2 MSM (b_query_g2 . w) runs on host
-threads concurrently.  r and s are arguments so proofs are reproducible
-(the reference draws them with fr_rand, prove.py:84-85).
+batch over the key's query set.  The G2 MSM (b_query_g2 . w) runs either
+on host threads concurrently with the device work, or (g2_device) on the
+device after the G1 batch, reading w where the G1 MSMs read it.  r and s are
+arguments so proofs are reproducible (the reference draws them with fr_rand,
+prove.py:84-85).
 """
 from __future__ import annotations
 
@@ -166,13 +168,30 @@ def g2_part(pk: ProvingKey, w_words: np.ndarray, threads: int = 0):
     return G2.g2_msm_words(pk.b_query_g2, w_words, pk.n_vars, threads=threads)
 
 
-def device_batch(pk: ProvingKey, witnesses: Sequence[np.ndarray], work: _Work = None) -> List[tuple]:
+def g2_device_set(pk: ProvingKey):
+    """The key's b_query_g2 as a device G2 base set, made on the key's
+    context at the first call and kept until pk.free().  Creation ends with
+    its stream synchronised, so the set serves MSMs on any context of the
+    same device."""
+    with _G2_SET_LOCK:
+        if pk.b_query_g2_dev is None:
+            pk.b_query_g2_dev = pk.ctx.msm_g2_bases_create(pk.b_query_g2.words)
+        return pk.b_query_g2_dev
+
+
+_G2_SET_LOCK = threading.Lock()
+
+
+def device_batch(pk: ProvingKey, witnesses: Sequence[np.ndarray], work: _Work = None,
+                 g2: bool = False) -> List[tuple]:
     """Device part of several proofs of one key: per proof the H pipeline,
     then one fixed-base batch for all their G1 MSMs.  Runs on `work`'s
     context (default: the key's own workspace on the key's context), so
     batches of different keys can run concurrently from different host
     threads on different contexts.  Returns per proof (a_lin, b_lin_g1,
-    h_acc, l_acc)."""
+    h_acc, l_acc); with `g2` set, per proof ((a_lin, b_lin_g1, h_acc,
+    l_acc), b_lin_g2), the G2 MSM run on the device over the witness
+    segment the G1 MSMs read."""
     k = len(witnesses)
     if k == 0:
         return []
@@ -183,7 +202,13 @@ def device_batch(pk: ProvingKey, witnesses: Sequence[np.ndarray], work: _Work = 
         _device_part(pk, wk, i, w)
     offs, sizes, boffs = _msm_entries(pk, wk, range(k))
     res = wk.ctx.msm_g1_fixed_batch(pk.fixed, wk.S.ptr, offs, sizes, boffs)
-    return [tuple(from_jacobian_mont(*res[4 * i + j]) for j in range(4)) for i in range(k)]
+    out = [tuple(from_jacobian_mont(*res[4 * i + j]) for j in range(4)) for i in range(k)]
+    if g2:
+        gset = g2_device_set(pk)
+        for i in range(k):
+            words = wk.ctx.msm_g2(gset, wk.S.ptr + 32 * wk.seg * i, pk.n_vars)
+            out[i] = (out[i], G2.point_from_words(words))
+    return out
 
 
 def assemble(pk: ProvingKey, g1_parts: tuple, b_lin_g2, r: int, s: int) -> Proof:
@@ -192,15 +217,18 @@ def assemble(pk: ProvingKey, g1_parts: tuple, b_lin_g2, r: int, s: int) -> Proof
 
 
 def prove_many(pk: ProvingKey, witnesses: Sequence[np.ndarray], rs: Sequence[Tuple[int, int]],
-               g2_threads: int = 0) -> List[Proof]:
+               g2_threads: int = 0, g2_device: bool = False) -> List[Proof]:
     """Proofs for several witnesses of one key; their G1 MSMs share one
     fixed-base batch launch and their G2 MSMs run on host threads while the
-    device works."""
+    device works, or on the device after the G1 batch (g2_device)."""
     k = len(witnesses)
     if k == 0:
         return []
     for w in witnesses:
         check_witness(pk, w)
+    if g2_device:
+        parts = device_batch(pk, witnesses, g2=True)
+        return [assemble(pk, g1, g2, *rs[i]) for i, (g1, g2) in enumerate(parts)]
     g2_res: List = [None] * k
 
     def g2_job():
@@ -216,5 +244,6 @@ def prove_many(pk: ProvingKey, witnesses: Sequence[np.ndarray], rs: Sequence[Tup
     return [assemble(pk, parts[i], g2_res[i], *rs[i]) for i in range(k)]
 
 
-def prove(pk: ProvingKey, w_words: np.ndarray, r: int, s: int, g2_threads: int = 0) -> Proof:
-    return prove_many(pk, [w_words], [(r, s)], g2_threads=g2_threads)[0]
+def prove(pk: ProvingKey, w_words: np.ndarray, r: int, s: int, g2_threads: int = 0,
+          g2_device: bool = False) -> Proof:
+    return prove_many(pk, [w_words], [(r, s)], g2_threads=g2_threads, g2_device=g2_device)[0]

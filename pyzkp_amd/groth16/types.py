@@ -43,7 +43,9 @@ class ProvingKey:
     """Reference fields (protocols/groth16/types.py:27-41) plus the device
     state.  The G1 query vectors live in one device fixed-base set laid out
     [a_query | b_query_g1 | h_query | l_query]; `query_offsets` gives each
-    start.  b_query_g2 is a packed host copy for the CPU G2 MSM."""
+    start.  b_query_g2 is a packed host copy for the CPU G2 MSM;
+    b_query_g2_dev its device base set for the device G2 MSM, made at the
+    first device G2 proof."""
     vk: VerifyingKey
     beta_g1: Any
     delta_g1: Any
@@ -62,6 +64,7 @@ class ProvingKey:
     query_offsets: tuple = ()             # (a, b_g1, h, l) starts in `fixed`
     query_points: Any = None              # DeviceBuffer of the G1 queries (affine R=2^256)
     b_query_g2: Any = None                # g2.G2Bases
+    b_query_g2_dev: Any = None            # _native.MsmG2Bases (device), or None
     csr: Any = None                       # dict name -> (row_ptr, cols, coeffs) device buffers
     aux_dev: Any = None                   # DeviceBuffer of aux ids (uint32)
     hbm_bytes: int = 0
@@ -79,6 +82,9 @@ class ProvingKey:
                 b.free()
         self.query_points = self.aux_dev = None
         self.csr = None
+        if self.b_query_g2_dev is not None:
+            self.b_query_g2_dev.free()
+            self.b_query_g2_dev = None
         if self.b_query_g2 is not None and hasattr(self.b_query_g2, "free"):
             self.b_query_g2.free()        # its prepared host set; the words stay
 

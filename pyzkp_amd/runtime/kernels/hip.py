@@ -78,6 +78,7 @@ def register_hip_kernels(registry: KernelRegistry) -> None:
     registry.register(OpType.POLY_SUB, H, _poly_sub, backend=B)
     registry.register(OpType.MSM_G1, H, _msm_g1, backend=B)
     registry.register(OpType.MSM_G1_BATCH, H, _msm_g1_batch, backend=B)
+    registry.register(OpType.MSM_G2, H, _msm_g2, backend=B)
     registry.register(OpType.KZG_COMMIT, H, _kzg_commit, backend=B)
     registry.register(OpType.KZG_BATCH_COMMIT, H, _kzg_batch_commit, backend=B)
     registry.register(OpType.BATCH_INV, H, _batch_inv, backend=B)
@@ -473,3 +474,48 @@ def _kzg_batch_commit(ctx):
     outs = _commit_many(ctx, _srs_points(srs.data), [list(p) for p in polys.data])
     oid = node.outputs[0]
     return {"outputs": {oid: Buffer(id=oid, device=Device.CPU, dtype=DType.OBJ, data=outs)}}
+
+
+# ---------------------------------------------------------------------- MSM G2
+# Device G2 base sets, cached by the identity of the host sequence like the
+# G1 bases above.  The reference registers MSM_G2 on the CPU only
+# (runtime/kernels/cpu/kernels.py:89, :338-348); here it runs on the device.
+_G2_CACHE: Dict[tuple, tuple] = {}
+
+
+def device_g2_bases(ctx, points, n_needed: int):
+    """A device G2 set for `points` (a g2.G2Bases or a sequence of G2
+    points), made once per context and sequence."""
+    from ... import g2 as G2
+    hip = _hip(ctx)
+    key = (id(hip), id(points))
+    hit = _G2_CACHE.get(key)
+    n = points.n if isinstance(points, G2.G2Bases) else len(points)
+    if hit is not None and hit[0] is points and hit[1].n == n:
+        gb = hit[1]
+    else:
+        words = points.words if isinstance(points, G2.G2Bases) else G2.G2Bases(points).words
+        gb = hip.msm_g2_bases_create(words)
+        _G2_CACHE[key] = (points, gb)
+    if n_needed > gb.n:
+        raise ValueError("scalars length exceeds points length")
+    return gb
+
+
+def _msm_g2(ctx):
+    """MSM_G2 (reference: runtime/kernels/cpu/kernels.py:338-348 over
+    crypto/msm/pippenger.py:67-121): sum s_i Q_i on the device, scalars an
+    FR HipBuffer; the result a host G2 point."""
+    from ... import g2 as G2
+    node = ctx["node"]
+    points: Buffer = ctx["inputs"][0]
+    scalars: Buffer = ctx["inputs"][1]
+    if scalars.dtype != DType.FR:
+        raise ValueError("hip msm_g2 scalars must be FR")
+    if not isinstance(scalars.data, HipBuffer):
+        raise ValueError("hip msm_g2 expects scalars to be a HipBuffer")
+    n = int(scalars.data.n)
+    gb = device_g2_bases(ctx, points.data, n)
+    acc = G2.point_from_words(_hip(ctx).msm_g2(gb, scalars.data.ptr, n, _window_bits(ctx))) if n else G2.G2_ZERO
+    oid = node.outputs[0]
+    return {"outputs": {oid: Buffer(id=oid, device=Device.CPU, dtype=DType.G2, data=acc)}}

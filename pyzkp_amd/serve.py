@@ -26,6 +26,9 @@ Requests (circuit, witness, r, s) arrive over time.
     the budget; a group waits for memory to free up.
   * Completion.  When a request's G1 parts (device) and G2 part (host) are
     both ready, a small assembler pool forms the proof.
+  * Device G2 (g2_device).  The G2 MSMs run on the group's slot after its
+    G1 batch, no host G2 workers are used, and a group forms whenever a
+    slot and HBM are free, so the batch size follows the load.
 
 A witness that fails its R1CS fails only its own request (its group is
 re-run one request at a time).  `serve_plain` is the baseline: `prove()`
@@ -174,16 +177,20 @@ class _Slot:
 class ProveServer:
     def __init__(self, keys: Dict[object, ProvingKey], hbm_budget: int, max_batch: int = 16,
                  g2_workers: int = 2, cpu_threads: int = 16, device_slots: int = 2,
-                 assemblers: int = 2, slot_contexts: Optional[list] = None) -> None:
+                 assemblers: int = 2, slot_contexts: Optional[list] = None, g2_device: bool = False) -> None:
         """`slot_contexts`: contexts for the device slots (owned by the
         caller); by default each slot opens a context of its own on the
-        keys' device."""
+        keys' device.  `g2_device`: each group's G2 MSMs run on its slot
+        after the G1 batch instead of on the host G2 workers; a group then
+        forms as soon as a slot is free (every pending request of the first
+        circuit, up to max_batch)."""
         if device_slots < 1 or max_batch < 1:
             raise ValueError("device_slots and max_batch must be at least 1")
         self.keys = keys
         self.budget = int(hbm_budget)
         self.max_batch = int(max_batch)
         self.cpu_threads = cpu_threads
+        self.g2_device = bool(g2_device)
         self.slots: List[_Slot] = []
         if slot_contexts is not None:
             self.slots = [_Slot(i, c, False) for i, c in enumerate(slot_contexts)]
@@ -277,7 +284,8 @@ class _Run:
             self.pending.append(q)
             self.arrived += 1
             self.cv.notify_all()
-        self.srv.pool.submit(pk.n_vars, self._g2_job, q)
+        if not self.srv.g2_device:
+            self.srv.pool.submit(pk.n_vars, self._g2_job, q)
 
     def _g2_job(self, q: Request) -> None:
         with self.cv:
@@ -330,7 +338,7 @@ class _Run:
         for q in self.pending:                  # arrival order within a circuit
             by_key.setdefault(q.key, []).append(q)
         for key, qs in sorted(by_key.items(), key=lambda kv: kv[1][0].arrival):
-            due = len(qs) >= srv.max_batch or any(q.id in self.g2_started for q in qs)
+            due = srv.g2_device or len(qs) >= srv.max_batch or any(q.id in self.g2_started for q in qs)
             if not due:
                 continue
             group, used = [], self.inflight
@@ -368,24 +376,29 @@ class _Run:
         """Runs on the slot's thread: the group's device work on its context."""
         key = group[0].key
         pk = self.srv.keys[key]
-        outcome: Dict[int, tuple] = {}
+        outcome: Dict[int, list] = {}
         launches = 0
+        g2 = self.srv.g2_device
+
+        def parts_of(part) -> list:
+            return [("g1", part[0]), ("g2", part[1])] if g2 else [("g1", part)]
+
         try:
             try:
-                parts = P.device_batch(pk, [q.witness for q in group], slot.workspace(key, pk, len(group)))
+                parts = P.device_batch(pk, [q.witness for q in group], slot.workspace(key, pk, len(group)), g2=g2)
                 launches += 1
-                outcome = {q.id: ("g1", part) for q, part in zip(group, parts)}
+                outcome = {q.id: parts_of(part) for q, part in zip(group, parts)}
             except ValueError:
                 # a bad witness fails the group's check: isolate it
                 for q in group:
                     try:
-                        part = P.device_batch(pk, [q.witness], slot.workspace(key, pk, 1))[0]
-                        outcome[q.id] = ("g1", part)
+                        part = P.device_batch(pk, [q.witness], slot.workspace(key, pk, 1), g2=g2)[0]
+                        outcome[q.id] = parts_of(part)
                         launches += 1
                     except ValueError as e:
-                        outcome[q.id] = ("err", str(e))
+                        outcome[q.id] = [("err", str(e))]
         except Exception as e:                  # device failure: every request of the group fails
-            outcome = {q.id: ("err", f"{type(e).__name__}: {e}") for q in group}
+            outcome = {q.id: [("err", f"{type(e).__name__}: {e}")] for q in group}
         with self.cv:
             self.srv.stats["msm_launches"] += launches
             self.inflight -= nbytes
@@ -393,7 +406,8 @@ class _Run:
             self.free_slots.append(slot)
             self.cv.notify_all()
         for q in group:
-            self._deliver(q, *outcome[q.id])
+            for kind, value in outcome[q.id]:
+                self._deliver(q, kind, value)
 
     def serve(self, requests: List[Request]) -> List[Result]:
         feeder = threading.Thread(target=self._feed, daemon=True)
@@ -412,10 +426,11 @@ class _Run:
         return [self.results[q.id] for q in requests]
 
 
-def serve_plain(keys: Dict[object, ProvingKey], requests: List[Request], cpu_threads: int = 16) -> List[Result]:
+def serve_plain(keys: Dict[object, ProvingKey], requests: List[Request], cpu_threads: int = 16,
+                g2_device: bool = False) -> List[Result]:
     """Baseline: one proof at a time in arrival order, each with prove() (its
     G2 MSM on `cpu_threads` host threads, overlapped only with its own
-    device work)."""
+    device work, or on the device with g2_device)."""
     t0 = time.perf_counter()
     out = {}
     for q in sorted(requests, key=lambda x: x.arrival):
@@ -424,7 +439,7 @@ def serve_plain(keys: Dict[object, ProvingKey], requests: List[Request], cpu_thr
             time.sleep(delay)
         start = time.perf_counter() - t0
         try:
-            pf = P.prove(keys[q.key], q.witness, q.r, q.s, g2_threads=cpu_threads)
+            pf = P.prove(keys[q.key], q.witness, q.r, q.s, g2_threads=cpu_threads, g2_device=g2_device)
             out[q.id] = Result(q.id, pf, q.arrival, start, time.perf_counter() - t0)
         except ValueError as e:
             out[q.id] = Result(q.id, None, q.arrival, start, time.perf_counter() - t0, str(e))

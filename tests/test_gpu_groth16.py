@@ -72,6 +72,29 @@ def test_groth16_batch_equals_alone(zctx):
     pk.free()
 
 
+def test_groth16_device_g2_equals_host(zctx):
+    # the G2 MSM on the device (b_query_g2 . w over the witness segment the
+    # G1 MSMs read) gives the host G2 MSM's proof byte for byte
+    circ = synthetic_circuit(11, 0x5320)
+    pk = setup(circ, zctx, seed=0x5321)
+    ws = [synthetic_witness(circ, 0x5322 + i) for i in range(3)]
+    rs = [(3000 + i, 4000 + i) for i in range(3)]
+    host = prove_many(pk, ws, rs)
+    dev = prove_many(pk, ws, rs, g2_device=True)
+    for i in range(3):
+        assert dev[i].to_bytes() == host[i].to_bytes()
+        pub = words_to_ints(ws[i][circ.public_ids])
+        assert verify(pk.vk, pub, dev[i])
+    assert prove(pk, ws[1], *rs[1], g2_device=True).to_bytes() == host[1].to_bytes()
+    bad = ws[0].copy()
+    bad[0, 0] ^= 1
+    with pytest.raises(ValueError):
+        prove(pk, bad, 1, 2, g2_device=True)
+    assert pk.b_query_g2_dev is not None
+    pk.free()
+    assert pk.b_query_g2_dev is None
+
+
 def test_key_cache_roundtrip(zctx, tmp_path):
     from pyzkp_amd.groth16.keycache import CacheMismatchError, load_key, save_key
     circ = synthetic_circuit(9, 0x5310)

@@ -101,6 +101,26 @@ def test_msm_g1_matches_cpu():
     assert normalize(g.buffers["msm_res"].data) == M.msm_pippenger(pts_aff, scalars, M.default_window_bits(n))
 
 
+def test_msm_g2_matches_host():
+    # MSM_G2 through the registry (the reference registers it on the CPU
+    # only): the device sum over bases [k_i] Q equals [sum s_i k_i] Q, with
+    # the scalars handed over on the host as the reference's graph does
+    from pyzkp_amd import g2 as G2
+    n = 200
+    ks = [(i * 7919 + 3) % R_MOD for i in range(n)]
+    scalars = [(i * 12345 + 6789) % R_MOD for i in range(n)]
+    bases = G2.from_words(G2.g2_mul_many(G2.G2_GENERATOR, ks))
+    expect = G2.g2_mul(G2.G2_GENERATOR, sum(k * s for k, s in zip(ks, scalars)) % R_MOD)
+    for window in (0, 5):
+        g = Graph()
+        g.add_buffer(id="b_query_g2", device=Device.CPU, dtype=DType.OBJ, data=bases)
+        g.add_buffer(id="w", device=Device.CPU, dtype=DType.FR, data=scalars)
+        g.add_node(op=OpType.MSM_G2, inputs=["b_query_g2", "w"], outputs=["b_lin_g2"],
+                   attrs={"hip_window_bits": window})
+        _exe().run(g, runtime_config=RuntimeConfig(backend=Backend.HIP))
+        assert G2.g2_eq(g.buffers["b_lin_g2"].data, expect)
+
+
 def test_kzg_batch_commit_golden():
     with open(os.path.join(os.path.dirname(__file__), "golden", "plonk_pk_n16.json")) as f:
         pk = json.load(f)

@@ -1,6 +1,7 @@
 """Served Groth16 on the device: every proof of a Poisson mix of circuit
 sizes verifies, equals byte for byte the proof the plain one-at-a-time loop
-makes for the same request, and a witness that does not satisfy its R1CS
+makes for the same request (G2 MSMs on host workers or on the device
+slots), and a witness that does not satisfy its R1CS
 fails only its own request."""
 import pytest
 
@@ -37,6 +38,15 @@ def test_served_equals_plain_and_verifies(zctx):
         pub = words_to_ints(q.witness[circuits[q.key].public_ids])
         assert verify(keys[q.key].vk, pub, a.proof)
     assert latency_stats(served)["count"] == len(reqs) - 1
+    # the same requests with the G2 MSMs on the device slots
+    srv = ProveServer(keys, hbm_budget=budget, max_batch=4, cpu_threads=4, g2_device=True)
+    served_dev = srv.run(reqs)
+    srv.close()
+    for q, a, b in zip(reqs, served_dev, plain):
+        if q.id == 3:
+            assert a.proof is None and "R1CS" in a.error
+            continue
+        assert a.error is None and a.proof.to_bytes() == b.proof.to_bytes()
     with pytest.raises(ValueError):
         ProveServer(keys, hbm_budget=1024).run(reqs[:1])
     for pk in keys.values():

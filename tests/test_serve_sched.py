@@ -40,9 +40,9 @@ class _Work:
 @pytest.fixture
 def fake(monkeypatch):
     calls = {"device": [], "lock": threading.Lock(), "active": 0, "max_active": 0, "keys_live": set(),
-             "cross": False, "dev_s": 0.0}
+             "cross": False, "dev_s": 0.0, "device_g2": 0}
 
-    def device_batch(pk, ws, work):
+    def device_batch(pk, ws, work, g2=False):
         with calls["lock"]:
             calls["device"].append((pk.n, len(ws), work.ctx.i))
             calls["active"] += 1
@@ -58,6 +58,9 @@ def fake(monkeypatch):
         for w in ws:
             if int(w[0, 0]) == 0xBAD:
                 raise ValueError("witness does not satisfy R1CS (non-zero remainder)")
+        if g2:
+            calls["device_g2"] += len(ws)
+            return [(("g1", int(w[0, 0])), ("g2", int(w[0, 0]))) for w in ws]
         return [("g1", int(w[0, 0])) for w in ws]
 
     def grow_work(pk, w, k, ctx=None):
@@ -112,6 +115,29 @@ def test_every_request_answered_within_budget(fake):
     assert sum(k for _, k, _ in fake["device"]) >= len(reqs)
     assert max(per[n] * k for n, k, _ in fake["device"]) <= budget
     assert srv.stats["peak_inflight_bytes"] <= budget
+
+
+def test_device_g2_mode_uses_no_host_workers(fake, monkeypatch):
+    # g2_device: every G2 part comes from the slot's device batch, none from
+    # the host pool; a burst still forms batches; a bad witness fails alone
+    def no_host_g2(pk, w, threads):
+        raise AssertionError("host G2 MSM called in device G2 mode")
+
+    monkeypatch.setattr(S.P, "g2_part", no_host_g2)
+    fake["dev_s"] = 0.01
+    keys = {6: _Key(64, 70), 8: _Key(256, 260)}
+    classes = [6, 8, 6, 6, 8, 8, 6, 8, 6, 6, 8, 6]
+    reqs = _reqs(keys, classes, bad={4}, gap=0.0)
+    srv = _server(keys, 1 << 40, max_batch=4, cpu_threads=2, g2_device=True)
+    res = srv.run(reqs)
+    srv.close()
+    for q, r in zip(reqs, res):
+        if q.id == 4:
+            assert r.proof is None and "R1CS" in r.error
+        else:
+            assert r.error is None and r.proof == ("proof", 1000 + q.id, q.id, 2 * q.id)
+    assert fake["device_g2"] >= len(reqs) - 1
+    assert srv.stats["max_batch_seen"] >= 2 and srv.stats["max_batch_seen"] <= 4
 
 
 def test_groups_of_different_circuits_overlap_and_queue_forms_batches(fake):
