@@ -90,3 +90,26 @@ def test_msm_g2_matches_host_2_16(zctx):
     got = _device_msm(zctx, words, ss, 0)
     assert G2.point_from_words(got) == host
     assert np.array_equal(got, _closed_form(ks, ss))
+
+
+def test_msm_g2_prefix_empty_and_refused(zctx):
+    # a call over the first n bases of a larger set, n = 0 (the identity) and
+    # more scalars than bases (refused with ValueError before any device work)
+    rng = random.Random(0x96)
+    m = 400
+    ks = [rng.randrange(1, R_MOD) for _ in range(m)]
+    ss = [rng.randrange(R_MOD) for _ in range(m)]
+    gb = zctx.msm_g2_bases_create(_bases(ks))
+    sc = workload.device_fr_mont(zctx, _words(ss))
+    try:
+        for n in (1, 77, m):
+            print(f"G2 MSM prefix n={n} of {m}", flush=True)
+            assert np.array_equal(zctx.msm_g2(gb, sc.ptr, n, 0), _closed_form(ks[:n], ss[:n])), n
+        assert not zctx.msm_g2(gb, sc.ptr, 0, 0).any()
+        with pytest.raises(ValueError):
+            zctx.msm_g2(gb, sc.ptr, m + 1, 0)
+        # the context stays usable after the refused call
+        assert np.array_equal(zctx.msm_g2(gb, sc.ptr, m, 9), _closed_form(ks, ss))
+    finally:
+        sc.free()
+        gb.free()
